@@ -1,0 +1,395 @@
+"""Python bindings for the MI355X-native HVVR render path.
+
+Two in-tree shared libraries are loaded with ctypes:
+
+* ``lib/libhvvr_gpu.so``  - gfx950 HIP kernels behind the C ABI of ``include/hvvr_gpu.h``;
+* ``lib/libhvvr.so``      - host C++ (BVH builder, scene flattening, sample setup, seeded
+  scene generators, the ``hvvr::Raycaster`` C++ API) with the C interface of
+  ``include/hvvr_host.h``.
+
+Arrays cross the boundary as numpy arrays with the reference's byte layouts (see
+``DTYPES``). Every GPU entry point raises :class:`HvvrError` on failure; nothing here
+falls back to a CPU implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, byref, c_char_p, c_float, c_int, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+
+__all__ = [
+    "HvvrError", "DTYPES", "load_gpu", "load_host", "HostScene", "HostView", "GpuContext", "GpuScene",
+    "GpuView", "subsample_offsets", "grid_samples", "transform_matrix",
+]
+
+
+class HvvrError(RuntimeError):
+    pass
+
+
+# ---------------------------------------------------------------------------------------
+# struct layouts (byte-identical to the reference's)
+# ---------------------------------------------------------------------------------------
+DTYPES = {
+    "vertex": np.dtype([("pos", "<f4", 3), ("normal", "<u2", 4), ("uv", "<f4", 2), ("pad", "<u4")]),
+    "tri_shade": np.dtype([("indices", "<u4", 3), ("material", "<u4")]),
+    "tri_intersect": np.dtype([("v0", "<f4", 3), ("edge0", "<f4", 3), ("edge1", "<f4", 3)]),
+    "node": np.dtype([("bounds", "<f4", (6, 4)), ("leafMask", "<u4"), ("data", "<u4", 5), ("pad", "<u4", 2)]),
+    "material": np.dtype([("emissive", "<f4", 4), ("diffuse", "<f4", 4), ("specular", "<f4", 4),
+                          ("glossiness", "<f4"), ("opacity", "<f4"), ("textureIDsAndShadingModel", "<u8")]),
+    "light": np.dtype([("type", "<u4"), ("body", "<f4", 13)]),
+    "beam": np.dtype([("centerRay", "<f4", 3), ("du", "<f4", 3), ("dv", "<f4", 3)]),
+    "sfrustum": np.dtype([("origins", "<f4", (4, 3)), ("directions", "<f4", (4, 3))]),
+    "sample": np.dtype([("pixel", "<u4", 2), ("position", "<f4", 2), ("minorAxis", "<f4", 2),
+                        ("majorAxisLength", "<f4")]),
+}
+assert DTYPES["vertex"].itemsize == 32 and DTYPES["node"].itemsize == 128
+assert DTYPES["material"].itemsize == 64 and DTYPES["light"].itemsize == 56
+assert DTYPES["beam"].itemsize == 36 and DTYPES["sfrustum"].itemsize == 96 and DTYPES["sample"].itemsize == 28
+
+_gpu = None
+_host = None
+
+
+def _ptr(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
+
+
+def load_gpu():
+    """Load lib/libhvvr_gpu.so (raises if it has not been built)."""
+    global _gpu
+    if _gpu is None:
+        path = os.path.join(LIB_DIR, "libhvvr_gpu.so")
+        if not os.path.exists(path):
+            raise HvvrError(f"{path} is missing: run __graft_entry__.build() (or make -C hvvr_amd)")
+        lib = ctypes.CDLL(path)
+        lib.hvvr_gpu_last_error.restype = c_char_p
+        lib.hvvr_gpu_version.restype = c_char_p
+        _gpu = lib
+    return _gpu
+
+
+def load_host():
+    """Load lib/libhvvr.so (host C++ library; depends on libhvvr_gpu.so)."""
+    global _host
+    if _host is None:
+        load_gpu()
+        path = os.path.join(LIB_DIR, "libhvvr.so")
+        if not os.path.exists(path):
+            raise HvvrError(f"{path} is missing: run __graft_entry__.build() (or make -C hvvr_amd)")
+        lib = ctypes.CDLL(path)
+        lib.hvvr_host_last_error.restype = c_char_p
+        _host = lib
+    return _host
+
+
+def _check_gpu(rc):
+    if rc < 0:
+        msg = load_gpu().hvvr_gpu_last_error().decode()
+        raise HvvrError(f"hvvr_gpu error {rc}: {msg}")
+    return rc
+
+
+def _check_host(rc):
+    if rc < 0:
+        msg = load_host().hvvr_host_last_error().decode()
+        raise HvvrError(f"hvvr_host error {rc}: {msg}")
+    return rc
+
+
+# ---------------------------------------------------------------------------------------
+# host-side building blocks
+# ---------------------------------------------------------------------------------------
+SCENE_HEIGHTFIELD_BOXES = 1
+SCENE_ROOM = 2
+VIEW_PRESET_HEIGHTFIELD = 1
+VIEW_PRESET_ROOM_EYE = 2
+
+_SCENE_ARRAYS = [("verts", "vertex", 0), ("vertexToTransform", np.uint32, 0), ("triShade", "tri_shade", 1),
+                 ("nodes", "node", 2), ("materials", "material", 3), ("transforms", (np.float32, 16), 4),
+                 ("lights", "light", 5)]
+
+
+def _dt(spec):
+    if isinstance(spec, str):
+        return DTYPES[spec], ()
+    if isinstance(spec, tuple):
+        return np.dtype(spec[0]), (spec[1],)
+    return np.dtype(spec), ()
+
+
+class HostScene:
+    """A flattened scene (BVH built, models merged) as numpy arrays."""
+
+    def __init__(self, kind=SCENE_HEIGHTFIELD_BOXES, seed=0, mesh_count=0, _handle=None):
+        lib = load_host()
+        h = c_void_p()
+        if _handle is None:
+            _check_host(lib.hvvr_host_scene_create(c_int(kind), c_uint64(seed), c_uint32(mesh_count), byref(h)))
+        else:
+            h = _handle
+        counts = np.zeros(7, np.uint64)
+        _check_host(lib.hvvr_host_scene_counts(h, _ptr(counts)))
+        n_verts, n_tris, n_nodes, n_mats, n_xf, n_lights, n_models = (int(c) for c in counts)
+        sizes = [n_verts, n_verts, n_tris, n_nodes, n_mats, n_xf, n_lights]
+        for (name, spec, _), n in zip(_SCENE_ARRAYS, sizes):
+            dt, extra = _dt(spec)
+            arr = np.zeros((n,) + extra, dt)
+            if n:
+                idx = [a[0] for a in _SCENE_ARRAYS].index(name)
+                _check_host(lib.hvvr_host_scene_copy(h, c_int(idx), _ptr(arr), c_uint64(arr.nbytes)))
+            setattr(self, name, arr)
+        self.model_count = n_models
+        lib.hvvr_host_scene_destroy(h)
+
+    @classmethod
+    def from_meshes(cls, meshes, materials, lights=None):
+        """meshes: list of (verts[vertex dtype], tri_indices[n,3] local, tri_material[n], transform16 or None)."""
+        lib = load_host()
+        vstart = np.zeros(len(meshes) + 1, np.uint32)
+        tstart = np.zeros(len(meshes) + 1, np.uint32)
+        for i, m in enumerate(meshes):
+            vstart[i + 1] = vstart[i] + len(m[0])
+            tstart[i + 1] = tstart[i] + len(m[1])
+        verts = np.concatenate([m[0] for m in meshes]).astype(DTYPES["vertex"])
+        tris = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.uint32) for m in meshes]).reshape(-1, 3))
+        mats = np.ascontiguousarray(np.concatenate([np.asarray(m[2], np.uint32) for m in meshes]))
+        xf = np.stack([np.asarray(m[3] if m[3] is not None else np.eye(4, dtype=np.float32).reshape(16),
+                                  np.float32).reshape(16) for m in meshes])
+        materials = np.ascontiguousarray(materials, DTYPES["material"])
+        lights = np.zeros(0, DTYPES["light"]) if lights is None else np.ascontiguousarray(lights, DTYPES["light"])
+        h = c_void_p()
+        _check_host(lib.hvvr_host_scene_create_from_meshes(
+            c_uint32(len(meshes)), _ptr(vstart), _ptr(verts), _ptr(tstart), _ptr(tris), _ptr(mats), _ptr(xf),
+            c_uint32(len(materials)), _ptr(materials), c_uint32(len(lights)), _ptr(lights), byref(h)))
+        return cls(_handle=h)
+
+
+class HostView:
+    """Camera-space sample hierarchy: beams, tile/block frusta, remap."""
+
+    def __init__(self, preset=VIEW_PRESET_HEIGHTFIELD, eye=0, width=0, height=0, _handle=None):
+        lib = load_host()
+        h = c_void_p()
+        if _handle is None:
+            _check_host(lib.hvvr_host_view_create_preset(c_int(preset), c_uint32(eye), c_uint32(width),
+                                                         c_uint32(height), byref(h)))
+        else:
+            h = _handle
+        counts = np.zeros(7, np.uint64)
+        _check_host(lib.hvvr_host_view_counts(h, _ptr(counts)))
+        (self.valid_sample_count, self.sample_count, self.block_count, self.width, self.height, self.split,
+         self.tile_count) = (int(c) for c in counts)
+        self.beams = np.zeros(self.sample_count, DTYPES["beam"])
+        self.tile_frusta = np.zeros(self.tile_count, DTYPES["sfrustum"])
+        self.block_frusta = np.zeros(self.block_count, DTYPES["sfrustum"])
+        self.remap = np.zeros(self.width * self.height * self.split, np.int32)
+        for idx, arr in enumerate([self.beams, self.tile_frusta, self.block_frusta, self.remap]):
+            if arr.size:
+                _check_host(lib.hvvr_host_view_copy(h, c_int(idx), _ptr(arr), c_uint64(arr.nbytes)))
+        self.camera_to_world = np.zeros(16, np.float32)
+        self.lens = np.zeros(2, np.float32)
+        _check_host(lib.hvvr_host_view_camera(h, _ptr(self.camera_to_world), _ptr(self.lens)))
+        lib.hvvr_host_view_destroy(h)
+
+    @classmethod
+    def from_samples(cls, samples, width, height, viewport, split=1, lens_radius=0.0, focal_distance=1.0):
+        lib = load_host()
+        samples = np.ascontiguousarray(samples, DTYPES["sample"])
+        vp = np.asarray(viewport, np.float32)
+        h = c_void_p()
+        _check_host(lib.hvvr_host_view_create_from_samples(
+            _ptr(samples), c_uint32(len(samples)), c_uint32(split), c_uint32(width), c_uint32(height), _ptr(vp),
+            c_float(lens_radius), c_float(focal_distance), byref(h)))
+        return cls(_handle=h)
+
+
+def subsample_offsets(msaa_rate, spin=0.0):
+    out = np.zeros((msaa_rate, 2), np.float32)
+    _check_host(load_host().hvvr_host_subsample_offsets(c_uint32(msaa_rate), c_float(spin), _ptr(out)))
+    return out
+
+
+def grid_samples(width, height, k1=0.0, k2=0.0):
+    out = np.zeros(width * height, DTYPES["sample"])
+    _check_host(load_host().hvvr_host_grid_samples(c_uint32(width), c_uint32(height), c_float(k1), c_float(k2),
+                                                   _ptr(out)))
+    return out
+
+
+def transform_matrix(translation, rotation_xyzw, scale=1.0):
+    out = np.zeros(16, np.float32)
+    t = np.asarray(translation, np.float32)
+    r = np.asarray(rotation_xyzw, np.float32)
+    _check_host(load_host().hvvr_host_transform_matrix(_ptr(t), _ptr(r), c_float(scale), _ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# GPU C ABI
+# ---------------------------------------------------------------------------------------
+class _SceneDesc(ctypes.Structure):
+    _fields_ = [("vertices", c_void_p), ("vertexToTransform", c_void_p), ("vertexCount", c_uint32),
+                ("triangles", c_void_p), ("triangleCount", c_uint32), ("nodes", c_void_p), ("nodeCount", c_uint32),
+                ("materials", c_void_p), ("materialCount", c_uint32), ("transforms", c_void_p),
+                ("transformCount", c_uint32)]
+
+
+class _ViewDesc(ctypes.Structure):
+    _fields_ = [("msaaRate", c_uint32), ("sampleCount", c_uint32), ("validSampleCount", c_uint32),
+                ("beams", c_void_p), ("tileFrusta", c_void_p), ("blockFrusta", c_void_p), ("remap", c_void_p),
+                ("width", c_uint32), ("height", c_uint32), ("split", c_uint32), ("lensRadius", c_float),
+                ("focalDistance", c_float)]
+
+
+class ViewStats(ctypes.Structure):
+    _fields_ = [("listEntries", c_uint64), ("survivorPairs", c_uint64), ("occupiedTiles", c_uint32),
+                ("tileCount", c_uint32), ("maxTileStack", c_uint32), ("overflow", c_uint32)]
+
+
+class GpuContext:
+    def __init__(self, device=0):
+        self.lib = load_gpu()
+        self.handle = c_void_p()
+        _check_gpu(self.lib.hvvr_gpu_context_create(c_int(device), byref(self.handle)))
+
+    def synchronize(self):
+        _check_gpu(self.lib.hvvr_gpu_context_synchronize(self.handle))
+
+    def set_profiling(self, enable):
+        _check_gpu(self.lib.hvvr_gpu_context_set_profiling(self.handle, c_int(1 if enable else 0)))
+
+    def close(self):
+        if self.handle:
+            self.lib.hvvr_gpu_context_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuScene:
+    def __init__(self, ctx: GpuContext, scene: HostScene):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.scene = scene
+        self._keep = [np.ascontiguousarray(a) for a in (scene.verts, scene.vertexToTransform, scene.triShade,
+                                                        scene.nodes, scene.materials, scene.transforms)]
+        v, vt, ts, nd, mt, xf = self._keep
+        d = _SceneDesc(_ptr(v), _ptr(vt), len(v), _ptr(ts), len(ts), _ptr(nd), len(nd), _ptr(mt), len(mt), _ptr(xf),
+                       len(xf))
+        self.handle = c_void_p()
+        _check_gpu(self.lib.hvvr_gpu_scene_create(ctx.handle, byref(d), byref(self.handle)))
+        self.set_lights(scene.lights)
+
+    def set_lights(self, lights):
+        lights = np.ascontiguousarray(lights, DTYPES["light"])
+        dropped = c_uint32(0)
+        _check_gpu(self.lib.hvvr_gpu_scene_set_lights(self.handle, _ptr(lights), c_uint32(len(lights)),
+                                                      byref(dropped)))
+        return dropped.value
+
+    def set_transforms(self, transforms):
+        t = np.ascontiguousarray(transforms, np.float32).reshape(-1, 16)
+        _check_gpu(self.lib.hvvr_gpu_scene_set_transforms(self.handle, _ptr(t), c_uint32(len(t))))
+
+    def set_vertices(self, first, verts):
+        v = np.ascontiguousarray(verts, DTYPES["vertex"])
+        _check_gpu(self.lib.hvvr_gpu_scene_set_vertices(self.handle, c_uint32(first), c_uint32(len(v)), _ptr(v)))
+
+    def update(self):
+        _check_gpu(self.lib.hvvr_gpu_scene_update(self.handle))
+
+    def read_nodes(self):
+        out = np.zeros(len(self.scene.nodes), DTYPES["node"])
+        _check_gpu(self.lib.hvvr_gpu_scene_read_nodes(self.handle, _ptr(out)))
+        return out
+
+    def read_triangles(self):
+        out = np.zeros(len(self.scene.triShade), DTYPES["tri_intersect"])
+        _check_gpu(self.lib.hvvr_gpu_scene_read_triangles(self.handle, _ptr(out)))
+        return out
+
+    def read_vertices(self):
+        out = np.zeros(len(self.scene.verts), DTYPES["vertex"])
+        _check_gpu(self.lib.hvvr_gpu_scene_read_vertices(self.handle, _ptr(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            self.lib.hvvr_gpu_scene_destroy(self.handle)
+            self.handle = c_void_p()
+
+
+class GpuView:
+    def __init__(self, ctx: GpuContext, view: HostView, msaa_rate=1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.view = view
+        self.msaa_rate = msaa_rate
+        self._keep = [np.ascontiguousarray(a) for a in (view.beams, view.tile_frusta, view.block_frusta, view.remap)]
+        b, tf, bf, rm = self._keep
+        d = _ViewDesc(msaa_rate, view.sample_count, view.valid_sample_count, _ptr(b), _ptr(tf), _ptr(bf), _ptr(rm),
+                      view.width, view.height, view.split, float(view.lens[0]), float(view.lens[1]))
+        self.handle = c_void_p()
+        _check_gpu(self.lib.hvvr_gpu_view_create(ctx.handle, byref(d), byref(self.handle)))
+
+    def render(self, scene: GpuScene, camera_to_world=None, jitter=(0.0, 0.0)):
+        m = np.ascontiguousarray(self.view.camera_to_world if camera_to_world is None else camera_to_world,
+                                 np.float32).reshape(16)
+        j = np.asarray(jitter, np.float32)
+        _check_gpu(self.lib.hvvr_gpu_view_render(self.handle, scene.handle, _ptr(m), _ptr(j)))
+
+    def set_jitter_spin(self, enable):
+        _check_gpu(self.lib.hvvr_gpu_view_set_jitter_spin(self.handle, c_int(1 if enable else 0)))
+
+    def read_image(self):
+        out = np.zeros((self.view.height, self.view.width), np.uint32)
+        _check_gpu(self.lib.hvvr_gpu_view_read_image(self.handle, _ptr(out), c_uint32(self.view.width)))
+        return out
+
+    def read_sample_colors(self):
+        out = np.zeros(self.view.sample_count, np.uint32)
+        _check_gpu(self.lib.hvvr_gpu_view_read_sample_colors(self.handle, _ptr(out)))
+        return out
+
+    def read_gbuffer(self):
+        n = self.view.sample_count * self.msaa_rate
+        tri = np.zeros(n, np.uint32)
+        mask = np.zeros(n, np.uint32)
+        _check_gpu(self.lib.hvvr_gpu_view_read_gbuffer(self.handle, _ptr(tri), _ptr(mask)))
+        return tri.reshape(-1, self.msaa_rate), mask.reshape(-1, self.msaa_rate)
+
+    def read_tile_lists(self):
+        counts = np.zeros(self.view.tile_count, np.uint32)
+        total = c_uint64(0)
+        _check_gpu(self.lib.hvvr_gpu_view_read_tile_lists(self.handle, _ptr(counts), None, c_uint64(0),
+                                                          byref(total)))
+        lists = np.zeros(max(total.value, 1), np.uint32)
+        _check_gpu(self.lib.hvvr_gpu_view_read_tile_lists(self.handle, _ptr(counts), _ptr(lists),
+                                                          c_uint64(len(lists)), byref(total)))
+        return counts, lists[: total.value]
+
+    def stats(self):
+        st = ViewStats()
+        _check_gpu(self.lib.hvvr_gpu_view_get_stats(self.handle, byref(st)))
+        return st
+
+    def stage_times(self):
+        out = np.zeros(8, np.float32)
+        n = _check_gpu(self.lib.hvvr_gpu_view_stage_times(self.handle, _ptr(out), c_int(8)))
+        return out[:n]
+
+    def close(self):
+        if self.handle:
+            self.lib.hvvr_gpu_view_destroy(self.handle)
+            self.handle = c_void_p()

@@ -1,0 +1,612 @@
+// C ABI over the gfx950 render path (include/hvvr_gpu.h): resource management,
+// the refit plan derived from the node topology, and per-frame kernel sequencing.
+
+#include "../common/subsample.h"
+#include "gpu_internal.h"
+
+#include <algorithm>
+#include <string.h>
+
+namespace hvvr {
+
+thread_local std::string gLastError;
+void setLastError(const std::string& msg) {
+    gLastError = msg;
+}
+
+static int fail(const std::string& msg, int code = HVVR_GPU_ERR_INVALID) {
+    setLastError(msg);
+    return code;
+}
+
+template <class T>
+static hipError_t upload(DeviceBuffer<T>& buf, const void* src, size_t count, hipStream_t stream) {
+    hipError_t e = buf.resize(count);
+    if (e != hipSuccess || count == 0)
+        return e;
+    e = hipMemcpyAsync(buf.ptr, src, count * sizeof(T), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess)
+        return e;
+    return hipStreamSynchronize(stream);
+}
+
+// Refit plan (raycaster/gpu_scene_state.cu:506-589): leaf slot of every triangle, and
+// child->parent tasks for internal nodes grouped by parent depth, deepest first.
+static int buildRefitPlan(const BVHNode* nodes, uint32_t nodeCount, uint32_t triCount, std::vector<uint32_t>& triToNode,
+                          std::vector<uint32_t>& taskChild, std::vector<uint32_t>& taskNodeChild,
+                          std::vector<RefitLevel>& levels) {
+    if (nodeCount == 0)
+        return fail("scene has no BVH nodes");
+    if (nodeCount >= (1u << 28))
+        return fail("scene has too many BVH nodes (limit 2^28)");
+    triToNode.assign(triCount, 0xffffffffu);
+    std::vector<uint32_t> depth(nodeCount, 0);
+    for (uint32_t n = 0; n < nodeCount; n++) {
+        for (int k = 0; k < 4; k++) {
+            if ((nodes[n].leafMask >> k) & 1u)
+                continue;
+            uint64_t child = uint64_t(n) + nodes[n].data[1 + k];
+            if (child <= n || child >= nodeCount)
+                return fail("BVH child offset out of range");
+            depth[child] = depth[n] + 1;
+        }
+    }
+    struct Task {
+        uint32_t child, nodeChild, parentDepth;
+    };
+    std::vector<Task> tasks;
+    for (uint32_t n = 0; n < nodeCount; n++) {
+        for (int k = 0; k < 4; k++) {
+            uint32_t nodeChild = (n << 2) | uint32_t(k);
+            if (!((nodes[n].leafMask >> k) & 1u)) {
+                tasks.push_back({n + nodes[n].data[1 + k], nodeChild, depth[n]});
+            } else {
+                uint32_t start = nodes[n].data[k], end = nodes[n].data[k + 1];
+                if (end < start || end > triCount)
+                    return fail("BVH leaf triangle range out of range");
+                for (uint32_t t = start; t < end; t++)
+                    triToNode[t] = nodeChild;
+            }
+        }
+    }
+    for (uint32_t t = 0; t < triCount; t++)
+        if (triToNode[t] == 0xffffffffu)
+            return fail("triangle not referenced by any BVH leaf");
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) {
+        if (a.parentDepth != b.parentDepth)
+            return a.parentDepth > b.parentDepth;
+        return a.child < b.child;
+    });
+    taskChild.clear();
+    taskNodeChild.clear();
+    levels.clear();
+    for (size_t i = 0; i < tasks.size(); i++) {
+        if (i == 0 || tasks[i].parentDepth != tasks[i - 1].parentDepth)
+            levels.push_back({uint32_t(i), 0});
+        levels.back().count++;
+        taskChild.push_back(tasks[i].child);
+        taskNodeChild.push_back(tasks[i].nodeChild);
+    }
+    return HVVR_GPU_OK;
+}
+
+static bool validRate(uint32_t a) {
+    return a == 1 || a == 2 || a == 4 || a == 8 || a == 16 || a == 32;
+}
+
+template <class T>
+static int readBack(hvvr_gpu_scene* s, DeviceBuffer<T>& buf, void* dst) {
+    if (!s || !dst)
+        return fail("null argument");
+    if (!s->everUpdated)
+        return fail("scene has not been updated yet", HVVR_GPU_ERR_STATE);
+    HVVR_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    if (buf.count)
+        HVVR_HIP_CHECK(hipMemcpy(dst, buf.ptr, buf.bytes(), hipMemcpyDeviceToHost));
+    return HVVR_GPU_OK;
+}
+} // namespace hvvr
+
+using namespace hvvr;
+
+extern "C" {
+
+const char* hvvr_gpu_last_error(void) {
+    return gLastError.c_str();
+}
+
+const char* hvvr_gpu_version(void) {
+    return "hvvr_gpu 0.1 gfx950";
+}
+
+int hvvr_gpu_context_create(int device, hvvr_gpu_context** out) {
+    if (!out)
+        return fail("null output");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail("no HIP device available", HVVR_GPU_ERR_NO_DEVICE);
+    if (device < 0 || device >= count)
+        return fail("device index out of range", HVVR_GPU_ERR_NO_DEVICE);
+    hipDeviceProp_t prop;
+    HVVR_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(std::string("device is ") + prop.gcnArchName + ", this build targets gfx950",
+                    HVVR_GPU_ERR_NO_DEVICE);
+    HVVR_HIP_CHECK(hipSetDevice(device));
+    hvvr_gpu_context* c = new hvvr_gpu_context();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail("hipStreamCreate failed", HVVR_GPU_ERR_HIP);
+    }
+    for (auto& e : c->stageEvents)
+        (void)hipEventCreate(&e);
+    *out = c;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_context_destroy(hvvr_gpu_context* c) {
+    if (!c)
+        return HVVR_GPU_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->stageEvents)
+        if (e)
+            (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_context_synchronize(hvvr_gpu_context* c) {
+    if (!c)
+        return fail("null context");
+    HVVR_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_context_set_profiling(hvvr_gpu_context* c, int enable) {
+    if (!c)
+        return fail("null context");
+    c->profiling = enable != 0;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_create(hvvr_gpu_context* ctx, const hvvr_gpu_scene_desc* d, hvvr_gpu_scene** out) {
+    if (!ctx || !d || !out)
+        return fail("null argument");
+    // GBuffer sort keys are (id << 5 | subsample), decoded with a sign-filling shift
+    // (intersect.cu:67,71): ids must stay below 2^26 to round-trip
+    if (d->triangleCount > (1u << 26))
+        return fail("triangleCount must be at most 2^26");
+    if ((d->vertexCount && (!d->vertices || !d->vertexToTransform)) || (d->triangleCount && !d->triangles) ||
+        !d->nodes || !d->nodeCount || (d->materialCount && !d->materials) ||
+        (d->transformCount && !d->transforms))
+        return fail("scene description has null arrays");
+    const BVHNode* nodes = static_cast<const BVHNode*>(d->nodes);
+    const PrecomputedTriangleShade* tris = static_cast<const PrecomputedTriangleShade*>(d->triangles);
+    for (uint32_t t = 0; t < d->triangleCount; t++) {
+        for (int k = 0; k < 3; k++)
+            if (tris[t].indices[k] >= d->vertexCount)
+                return fail("triangle vertex index out of range");
+        if (tris[t].material >= d->materialCount)
+            return fail("triangle material index out of range");
+    }
+    for (uint32_t v = 0; v < d->vertexCount; v++)
+        if (d->vertexToTransform[v] >= d->transformCount)
+            return fail("vertex transform index out of range");
+    std::vector<uint32_t> triToNode, taskChild, taskNodeChild;
+    std::vector<RefitLevel> levels;
+    int rc = buildRefitPlan(nodes, d->nodeCount, d->triangleCount, triToNode, taskChild, taskNodeChild, levels);
+    if (rc)
+        return rc;
+
+    hvvr_gpu_scene* s = new hvvr_gpu_scene();
+    s->ctx = ctx;
+    s->vertexCount = d->vertexCount;
+    s->triangleCount = d->triangleCount;
+    s->nodeCount = d->nodeCount;
+    s->materialCount = d->materialCount;
+    s->transformCount = d->transformCount;
+    s->levels = levels;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    step(upload(s->vertsModel, d->vertices, d->vertexCount, st));
+    step(s->vertsWorld.resize(d->vertexCount));
+    step(upload(s->vertexToTransform, d->vertexToTransform, d->vertexCount, st));
+    step(upload(s->transforms, d->transforms, d->transformCount, st));
+    step(upload(s->triShade, d->triangles, d->triangleCount, st));
+    step(s->triIntersect.resize(d->triangleCount));
+    step(upload(s->nodes, d->nodes, d->nodeCount, st));
+    step(upload(s->materials, d->materials, d->materialCount, st));
+    step(upload(s->triToNodeChild, triToNode.data(), triToNode.size(), st));
+    step(upload(s->taskChild, taskChild.data(), taskChild.size(), st));
+    step(upload(s->taskNodeChild, taskNodeChild.data(), taskNodeChild.size(), st));
+    if (e != hipSuccess) {
+        delete s;
+        return fail(std::string("scene upload failed: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+    }
+    *out = s;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_destroy(hvvr_gpu_scene* s) {
+    if (s) {
+        (void)hipStreamSynchronize(s->ctx->stream);
+        delete s;
+    }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_set_transforms(hvvr_gpu_scene* s, const float* transforms, uint32_t count) {
+    if (!s || (!transforms && count))
+        return fail("null argument");
+    if (count != s->transformCount)
+        return fail("transform count differs from the scene's");
+    if (count)
+        HVVR_HIP_CHECK(hipMemcpyAsync(s->transforms.ptr, transforms, count * sizeof(matrix4x4),
+                                      hipMemcpyHostToDevice, s->ctx->stream));
+    HVVR_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->dirty = true;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_set_vertices(hvvr_gpu_scene* s, uint32_t first, uint32_t count, const void* verts) {
+    if (!s || (!verts && count))
+        return fail("null argument");
+    if (uint64_t(first) + count > s->vertexCount)
+        return fail("vertex range out of bounds");
+    if (count)
+        HVVR_HIP_CHECK(hipMemcpyAsync(s->vertsModel.ptr + first, verts, size_t(count) * sizeof(ShadingVertex),
+                                      hipMemcpyHostToDevice, s->ctx->stream));
+    HVVR_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->dirty = true;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_set_lights(hvvr_gpu_scene* s, const void* lights, uint32_t count, uint32_t* dropped) {
+    if (!s || (!lights && count))
+        return fail("null argument");
+    LightingEnvironment env;
+    memset(&env, 0, sizeof(env));
+    uint32_t drop = 0;
+    const LightUnion* L = static_cast<const LightUnion*>(lights);
+    for (uint32_t i = 0; i < count; i++) {
+        switch (L[i].type) {
+            case LightTypeDirectional:
+                if (env.directionalLightCount < MAX_DIRECTIONAL_LIGHTS)
+                    env.directionalLights[env.directionalLightCount++] = L[i].directional;
+                else
+                    drop++;
+                break;
+            case LightTypePoint:
+                if (env.pointLightCount < MAX_POINT_LIGHTS)
+                    env.pointLights[env.pointLightCount++] = L[i].point;
+                else
+                    drop++;
+                break;
+            case LightTypeSpot:
+                if (env.spotLightCount < MAX_SPOT_LIGHTS)
+                    env.spotLights[env.spotLightCount++] = L[i].spot;
+                else
+                    drop++;
+                break;
+            default:
+                drop++;
+                break;
+        }
+    }
+    s->lighting = env;
+    if (dropped)
+        *dropped = drop;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_update(hvvr_gpu_scene* s) {
+    if (!s)
+        return fail("null scene");
+    if (!s->dirty)
+        return HVVR_GPU_OK;
+    HVVR_HIP_CHECK(hipSetDevice(s->ctx->device));
+    launchSceneRefit(s, s->ctx->stream);
+    HVVR_HIP_CHECK(hipGetLastError());
+    s->dirty = false;
+    s->everUpdated = true;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_scene_read_nodes(hvvr_gpu_scene* s, void* dst) {
+    return readBack(s, s->nodes, dst);
+}
+int hvvr_gpu_scene_read_triangles(hvvr_gpu_scene* s, void* dst) {
+    return readBack(s, s->triIntersect, dst);
+}
+int hvvr_gpu_scene_read_vertices(hvvr_gpu_scene* s, void* dst) {
+    return readBack(s, s->vertsWorld, dst);
+}
+
+int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvvr_gpu_view** out) {
+    if (!ctx || !d || !out)
+        return fail("null argument");
+    if (!validRate(d->msaaRate))
+        return fail("msaaRate must be 1, 2, 4, 8, 16 or 32");
+    if (d->sampleCount == 0 || d->sampleCount % BLOCK_SIZE != 0)
+        return fail("sampleCount must be a positive multiple of 8192");
+    if (d->validSampleCount > d->sampleCount)
+        return fail("validSampleCount exceeds sampleCount");
+    if (!d->beams || !d->tileFrusta || !d->blockFrusta || !d->remap)
+        return fail("view description has null arrays");
+    if (d->split < 1 || d->split > 3 || d->width == 0 || d->height == 0)
+        return fail("invalid image size or split");
+    if (d->lensRadius != 0.0f)
+        return fail("depth of field (lensRadius > 0) is not supported by this build");
+    uint64_t remapCount = uint64_t(d->width) * d->height * d->split;
+    for (uint64_t i = 0; i < remapCount; i++)
+        if (d->remap[i] >= int32_t(d->sampleCount))
+            return fail("remap entry out of range");
+
+    hvvr_gpu_view* v = new hvvr_gpu_view();
+    v->ctx = ctx;
+    v->msaaRate = d->msaaRate;
+    v->sampleCount = d->sampleCount;
+    v->validSampleCount = d->validSampleCount;
+    v->tileCount = d->sampleCount / TILE_SIZE;
+    v->blockCount = d->sampleCount / BLOCK_SIZE;
+    v->width = d->width;
+    v->height = d->height;
+    v->split = d->split;
+    v->lens = {d->lensRadius, d->focalDistance};
+    subsampleOffsets(v->msaaRate, 0.0f, v->offsets);
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    step(upload(v->beams, d->beams, v->sampleCount, st));
+    step(upload(v->tileFrustaCam, d->tileFrusta, v->tileCount, st));
+    step(upload(v->blockFrustaCam, d->blockFrusta, v->blockCount, st));
+    step(upload(v->remap, d->remap, remapCount, st));
+    step(v->tileFrustaWorld.resize(v->tileCount));
+    step(v->tilePre.resize(v->tileCount));
+    step(v->blockPre.resize(v->blockCount));
+    step(v->blockStacks.resize(size_t(v->blockCount) * BLOCK_STACK_SIZE));
+    step(v->blockStackSize.resize(v->blockCount));
+    step(v->tileStackScratch.resize(size_t(TILE_STACK_SIZE) * v->tileCount));
+    step(v->tileCounts.resize(v->tileCount));
+    step(v->tileOffsets.resize(v->tileCount + 1));
+    step(v->scanPartials.resize((v->tileCount + 1023) / 1024 + 1));
+    v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
+    step(v->triIndices.resize(v->listCapacity));
+    step(v->gbuffer.resize(size_t(v->sampleCount) * v->msaaRate));
+    step(v->sampleResults.resize(v->sampleCount));
+    step(v->image.resize(size_t(v->width) * v->height));
+    step(v->counters.resize(4));
+    if (e == hipSuccess)
+        e = hipMemsetAsync(v->image.ptr, 0, v->image.bytes(), st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete v;
+        return fail(std::string("view allocation failed: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+    }
+    *out = v;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_destroy(hvvr_gpu_view* v) {
+    if (v) {
+        (void)hipStreamSynchronize(v->ctx->stream);
+        delete v;
+    }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
+    if (!v)
+        return fail("null view");
+    v->jitterSpin = enable != 0;
+    if (!v->jitterSpin)
+        subsampleOffsets(v->msaaRate, 0.0f, v->offsets);
+    return HVVR_GPU_OK;
+}
+
+static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter) {
+    hvvr_gpu_context* c = v->ctx;
+    hipStream_t st = c->stream;
+    matrix4x4 m;
+    memcpy(&m, m16, sizeof(m));
+    v->lastCameraToWorld = m;
+    if (v->jitterSpin && jitter)
+        subsampleOffsets(v->msaaRate, jitter[0] * 6.28f, v->offsets);
+    bool prof = c->profiling;
+    auto mark = [&](int i) {
+        if (prof)
+            (void)hipEventRecord(c->stageEvents[i], st);
+    };
+    HVVR_HIP_CHECK(hipMemsetAsync(v->counters.ptr, 0, v->counters.bytes(), st));
+    mark(0);
+    launchFrustumSetup(v, m, st);
+    mark(1);
+    launchBlockTraverse(v, s, st);
+    mark(2);
+    launchTileTraverse(v, s, false, st);
+    mark(3);
+    launchScan(v, st);
+    mark(4);
+    launchTileTraverse(v, s, true, st);
+    mark(5);
+    launchIntersect(v, s, m, st);
+    mark(6);
+    launchResolve(v, s, m, st);
+    mark(7);
+    launchRemap(v, st);
+    mark(8);
+    HVVR_HIP_CHECK(hipGetLastError());
+    if (prof) {
+        HVVR_HIP_CHECK(hipEventSynchronize(c->stageEvents[kStageCount]));
+        for (int i = 0; i < kStageCount; i++)
+            (void)hipEventElapsedTime(&v->stageMs[i], c->stageEvents[i], c->stageEvents[i + 1]);
+    }
+    v->rendered = true;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter) {
+    if (!v || !s || !m16)
+        return fail("null argument");
+    if (v->ctx != s->ctx)
+        return fail("view and scene belong to different contexts");
+    if (!s->everUpdated || s->dirty) {
+        int rc = hvvr_gpu_scene_update(s);
+        if (rc)
+            return rc;
+    }
+    return renderOne(v, s, m16, jitter);
+}
+
+int hvvr_gpu_render_views(hvvr_gpu_scene* s, uint32_t count, hvvr_gpu_view* const* views, const float* m16,
+                          const float* jitter) {
+    if (!s || (count && (!views || !m16)))
+        return fail("null argument");
+    for (uint32_t i = 0; i < count; i++) {
+        int rc = hvvr_gpu_view_render(views[i], s, m16 + 16 * size_t(i), jitter ? jitter + 2 * size_t(i) : nullptr);
+        if (rc)
+            return rc;
+    }
+    return HVVR_GPU_OK;
+}
+
+static int viewSync(hvvr_gpu_view* v) {
+    if (!v)
+        return fail("null view");
+    if (!v->rendered)
+        return fail("view has not been rendered", HVVR_GPU_ERR_STATE);
+    HVVR_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    uint64_t counters[4];
+    HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+    if (counters[1])
+        return fail("tile traversal exceeded its stack or triangle-list budget", HVVR_GPU_ERR_OVERFLOW);
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_image(hvvr_gpu_view* v, uint32_t* image, uint32_t stride) {
+    if (!image)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    if (stride < v->width)
+        return fail("stride smaller than image width");
+    HVVR_HIP_CHECK(hipMemcpy2D(image, size_t(stride) * 4, v->image.ptr, size_t(v->width) * 4, size_t(v->width) * 4,
+                               v->height, hipMemcpyDeviceToHost));
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_sample_colors(hvvr_gpu_view* v, uint32_t* colors) {
+    if (!colors)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    HVVR_HIP_CHECK(hipMemcpy(colors, v->sampleResults.ptr, v->sampleResults.bytes(), hipMemcpyDeviceToHost));
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_gbuffer(hvvr_gpu_view* v, uint32_t* triIndex, uint32_t* sampleMask) {
+    if (!triIndex || !sampleMask)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    const uint32_t A = v->msaaRate;
+    std::vector<GBufferSubsample> raw(v->gbuffer.count);
+    std::vector<uint32_t> counts(v->tileCount);
+    HVVR_HIP_CHECK(hipMemcpy(raw.data(), v->gbuffer.ptr, v->gbuffer.bytes(), hipMemcpyDeviceToHost));
+    HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
+    const uint32_t maskAll = A < 32 ? ((1u << A) - 1u) : 0xffffffffu;
+    for (uint32_t s = 0; s < v->sampleCount; s++) {
+        uint32_t* ot = triIndex + size_t(s) * A;
+        uint32_t* om = sampleMask + size_t(s) * A;
+        for (uint32_t k = 0; k < A; k++) {
+            ot[k] = badTriIndex;
+            om[k] = 0;
+        }
+        if (counts[s / TILE_SIZE] == 0) { // empty tile: every subsample missed
+            om[0] = maskAll;
+            continue;
+        }
+        const GBufferSubsample* base = raw.data() + size_t(s / 64) * 64 * A;
+        uint32_t lane = s % 64, combined = 0;
+        for (uint32_t k = 0; k < A; k++) {
+            if (combined == maskAll)
+                break;
+            GBufferSubsample g = base[k * 64 + lane];
+            ot[k] = g.triIndex;
+            om[k] = g.sampleMask;
+            combined |= g.sampleMask;
+            if (g.triIndex == badTriIndex)
+                break;
+        }
+    }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_tile_lists(hvvr_gpu_view* v, uint32_t* counts, uint32_t* tris, uint64_t capacity,
+                                  uint64_t* total) {
+    if (!counts)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    uint32_t t = 0;
+    HVVR_HIP_CHECK(hipMemcpy(counts, v->tileCounts.ptr, size_t(v->tileCount) * 4, hipMemcpyDeviceToHost));
+    HVVR_HIP_CHECK(hipMemcpy(&t, v->tileOffsets.ptr + v->tileCount, 4, hipMemcpyDeviceToHost));
+    if (total)
+        *total = t;
+    if (tris) {
+        if (capacity < t)
+            return fail("tile list capacity too small", HVVR_GPU_ERR_OVERFLOW);
+        if (t)
+            HVVR_HIP_CHECK(hipMemcpy(tris, v->triIndices.ptr, size_t(t) * 4, hipMemcpyDeviceToHost));
+    }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
+    if (!st)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc && rc != HVVR_GPU_ERR_OVERFLOW)
+        return rc;
+    uint64_t counters[4];
+    HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> counts(v->tileCount);
+    HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
+    uint32_t total = 0;
+    HVVR_HIP_CHECK(hipMemcpy(&total, v->tileOffsets.ptr + v->tileCount, 4, hipMemcpyDeviceToHost));
+    memset(st, 0, sizeof(*st));
+    st->listEntries = total;
+    st->survivorPairs = counters[0];
+    st->overflow = counters[1] ? 1u : 0u;
+    st->maxTileStack = uint32_t(counters[2]);
+    st->tileCount = v->tileCount;
+    for (uint32_t c : counts)
+        st->occupiedTiles += c ? 1u : 0u;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_stage_times(hvvr_gpu_view* v, float* ms, int maxCount) {
+    if (!v || !ms)
+        return fail("null argument");
+    int n = std::min(maxCount, int(kStageCount));
+    for (int i = 0; i < n; i++)
+        ms[i] = v->stageMs[i];
+    return n;
+}
+
+} // extern "C"

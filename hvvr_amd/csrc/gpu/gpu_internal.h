@@ -1,0 +1,127 @@
+#pragma once
+// Internal state behind the C ABI (include/hvvr_gpu.h) and the kernel launchers.
+
+#include "../../../include/hvvr_gpu.h"
+#include "../common/hvvr_types.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace hvvr {
+
+void setLastError(const std::string& msg);
+
+#define HVVR_HIP_CHECK(expr)                                                                    \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            setLastError(std::string(#expr) + ": " + hipGetErrorString(_e));                    \
+            return HVVR_GPU_ERR_HIP;                                                            \
+        }                                                                                       \
+    } while (0)
+
+// Owning device allocation.
+template <class T>
+struct DeviceBuffer {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    hipError_t resize(size_t n) {
+        if (n == count && ptr)
+            return hipSuccess;
+        release();
+        count = n;
+        if (n == 0)
+            return hipSuccess;
+        return hipMalloc((void**)&ptr, n * sizeof(T));
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+enum { kStageCount = 8 };
+
+struct RefitLevel {
+    uint32_t start, count;
+};
+
+} // namespace hvvr
+
+struct hvvr_gpu_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
+};
+
+struct hvvr_gpu_scene {
+    hvvr_gpu_context* ctx = nullptr;
+    uint32_t vertexCount = 0, triangleCount = 0, nodeCount = 0, materialCount = 0, transformCount = 0;
+    hvvr::DeviceBuffer<hvvr::ShadingVertex> vertsModel, vertsWorld;
+    hvvr::DeviceBuffer<uint32_t> vertexToTransform;
+    hvvr::DeviceBuffer<hvvr::matrix4x4> transforms;
+    hvvr::DeviceBuffer<hvvr::PrecomputedTriangleShade> triShade;
+    hvvr::DeviceBuffer<hvvr::PrecomputedTriangleIntersect> triIntersect;
+    hvvr::DeviceBuffer<hvvr::BVHNode> nodes;
+    hvvr::DeviceBuffer<hvvr::SimpleMaterial> materials;
+    hvvr::DeviceBuffer<uint32_t> triToNodeChild;
+    hvvr::DeviceBuffer<uint32_t> taskChild, taskNodeChild;
+    std::vector<hvvr::RefitLevel> levels; // deepest first
+    hvvr::LightingEnvironment lighting = {};
+    bool dirty = true;
+    bool everUpdated = false;
+};
+
+struct hvvr_gpu_view {
+    hvvr_gpu_context* ctx = nullptr;
+    uint32_t msaaRate = 1;
+    uint32_t sampleCount = 0, validSampleCount = 0, tileCount = 0, blockCount = 0;
+    uint32_t width = 0, height = 0, split = 1;
+    hvvr::ThinLens lens = {0.0f, 1.0f};
+    bool jitterSpin = false;
+    hvvr::vector2 offsets[32];
+    hvvr::matrix4x4 lastCameraToWorld = hvvr::matrix4x4::identity();
+
+    hvvr::DeviceBuffer<hvvr::DirectionalBeam> beams;
+    hvvr::DeviceBuffer<hvvr::SimpleRayFrustum> tileFrustaCam, blockFrustaCam, tileFrustaWorld;
+    hvvr::DeviceBuffer<hvvr::FrustumPre> tilePre, blockPre;
+    hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
+    hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
+    hvvr::DeviceBuffer<hvvr::StackEntry> tileStackScratch; // [TILE_STACK_SIZE][tileCount], slot-major
+    hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount]
+    hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount + 1] exclusive scan, [tileCount] = total
+    hvvr::DeviceBuffer<uint32_t> scanPartials;
+    hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage
+    hvvr::DeviceBuffer<hvvr::GBufferSubsample> gbuffer; // [sampleCount * A], wave-interleaved
+    hvvr::DeviceBuffer<uint32_t> sampleResults;         // [sampleCount]
+    hvvr::DeviceBuffer<int32_t> remap;                  // [width*height*split]
+    hvvr::DeviceBuffer<uint32_t> image;                 // [width*height]
+    hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack
+    uint64_t listCapacity = 0;
+    float stageMs[hvvr::kStageCount] = {};
+    bool rendered = false;
+};
+
+namespace hvvr {
+
+// kernel launchers (all asynchronous on `stream`)
+void launchSceneRefit(hvvr_gpu_scene* s, hipStream_t stream);
+void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream);
+void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
+void launchTileTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, bool emit, hipStream_t stream);
+void launchScan(hvvr_gpu_view* v, hipStream_t stream);
+void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
+void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
+void launchRemap(hvvr_gpu_view* v, hipStream_t stream);
+
+} // namespace hvvr

@@ -1,0 +1,292 @@
+// Triangle-list intersection for one view (raycaster/intersect.cu:117-342 with the
+// primitive tests of raycaster/prim_tests.h:18-231), one 128-thread workgroup per tile
+// (= two wave64s), one thread per sample, A subsamples per thread.
+//
+// Per tile the list is processed in chunks of 128 entries. Each thread culls one
+// triangle against the tile (4 frustum planes, back-face/behind-origin, 4-corner
+// barycentric range); survivors are compacted into LDS in list order with a 64-bit
+// wave ballot + popcount and a two-wave prefix; then every thread tests its A
+// subsamples against every survivor in order, keeping the nearest hit. The per-tile
+// triangle setup (edges, normal, origin offset, plane distance t) is computed once
+// by the culling thread and shared through LDS, so the per-sample loop reads it with
+// LDS broadcasts. FMAs are placed as annotated in prim_tests.h:195-229.
+//
+// Output: the compressed GBuffer (EmitGBuffer, intersect.cu:28-89) laid out per
+// wave64: slot k of lane l of wave w at gbuffer[w*64*A + k*64 + l]. If every lane of
+// the wave saw a single triangle id across its subsamples, only slot 0 is written.
+// Tiles with an empty list also write their (all-miss) GBuffer, so every GBuffer slot
+// the resolve pass can reach was written this frame.
+
+#include "device_common.h"
+#include "gpu_internal.h"
+
+namespace hvvr {
+
+struct IntersectArgs {
+    GBufferSubsample* gbuffer;
+    const DirectionalBeam* beams;
+    const uint32_t* tileCounts;
+    const uint32_t* tileOffsets;
+    const uint32_t* triIndices;
+    const SimpleRayFrustum* worldTiles;
+    const PrecomputedTriangleIntersect* tris;
+    unsigned long long* survivors;
+    uint64_t listCapacity;
+    matrix4x4 cameraToWorld;
+    vector2 offsets[32];
+};
+
+struct TriTile {
+    float t;
+    vector3 e0, e1, normal, v0ToRayOrigin;
+};
+
+template <int N>
+__device__ inline void sortKeysAscending(uint32_t* k) {
+    // bitonic network (raycaster/sort.h:40-74) over distinct keys; fully unrolled so the
+    // keys stay in registers
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N / 2; i++) {
+                int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
+                int hi = lo | stride;
+                bool ascending = (lo & size) == 0;
+                uint32_t a = k[lo], b = k[hi];
+                bool swap = ascending ? (a > b) : (a < b);
+                k[lo] = swap ? b : a;
+                k[hi] = swap ? a : b;
+            }
+        }
+    }
+}
+
+template <int A>
+__device__ inline void emitGBuffer(const uint32_t* hit, GBufferSubsample* waveBase, uint32_t lane) {
+    bool oneTri = true;
+#pragma unroll
+    for (int i = 1; i < A; i++)
+        oneTri = oneTri && (hit[i] == hit[0]);
+    const uint32_t maskAll = A < 32 ? ((1u << A) - 1u) : 0xffffffffu;
+    if (__ballot(oneTri) == ~0ull) {
+        GBufferSubsample g;
+        g.triIndex = hit[0];
+        g.sampleMask = maskAll;
+        waveBase[lane] = g;
+        return;
+    }
+    uint32_t keys[A];
+#pragma unroll
+    for (int i = 0; i < A; i++)
+        keys[i] = (hit[i] << 5) | uint32_t(i);
+    sortKeysAscending<A>(keys);
+    GBufferSubsample g;
+    g.triIndex = uint32_t(int32_t(keys[0]) >> 5);
+    g.sampleMask = 1u << (keys[0] & 31u);
+    uint32_t slot = 0;
+#pragma unroll
+    for (int i = 1; i < A; i++) {
+        uint32_t t = uint32_t(int32_t(keys[i]) >> 5);
+        uint32_t bit = 1u << (keys[i] & 31u);
+        if (t == g.triIndex) {
+            g.sampleMask |= bit;
+        } else {
+            waveBase[slot * 64 + lane] = g;
+            slot++;
+            g.triIndex = t;
+            g.sampleMask = bit;
+        }
+    }
+    waveBase[slot * 64 + lane] = g;
+}
+
+template <int A>
+__global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
+    __shared__ TriTile cache[128];
+    __shared__ uint32_t cacheIndex[128];
+    __shared__ float planes[4][4];
+    __shared__ SimpleRayFrustum frustum;
+    __shared__ uint32_t waveSurvivors[2];
+
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    uint32_t listCount = args.tileCounts[tile];
+    const uint32_t listStart = args.tileOffsets[tile];
+    // a list that did not fit the list budget is skipped (the traversal raised the
+    // overflow flag, and the frame reports an error)
+    if (uint64_t(listStart) + listCount > args.listCapacity)
+        listCount = 0;
+    const uint32_t sample = tile * TILE_SIZE + tid;
+
+    const matrix4x4& m = args.cameraToWorld;
+    const matrix3x3 rot = m.upper3x3();
+    const vector3 rayOrigin = m.m3.xyz();
+    DirectionalBeam beam = args.beams[sample];
+    const vector3 ndir = -(rot * beam.centerRay);
+    const vector3 ndu = -(rot * beam.du);
+    const vector3 ndv = -(rot * beam.dv);
+
+    if (tid == 0) {
+        SimpleRayFrustum fr = args.worldTiles[tile];
+        frustum = fr;
+        for (int n = 0; n < 4; n++) {
+            vector3 dir = cross(fr.directions[n], fr.directions[(n + 1) % 4]);
+            planes[n][0] = dir.x;
+            planes[n][1] = dir.y;
+            planes[n][2] = dir.z;
+            planes[n][3] = dot(fr.origins[n], dir);
+        }
+    }
+
+    float depth[A];
+    uint32_t hit[A];
+#pragma unroll
+    for (int i = 0; i < A; i++) {
+        depth[i] = FLT_MAX;
+        hit[i] = badTriIndex;
+    }
+    __syncthreads();
+
+    uint32_t survivorsSum = 0;
+    for (uint32_t chunk = 0; chunk < listCount; chunk += 128) {
+        // ---- cull one list entry per thread ----
+        bool keep = false;
+        uint32_t triIndex = badTriIndex;
+        TriTile tt;
+        if (chunk + tid < listCount) {
+            triIndex = args.triIndices[listStart + chunk + tid];
+            PrecomputedTriangleIntersect tri = args.tris[triIndex];
+            // FrustumPlanes::test (prim_tests.h:30-49)
+            bool inside = true;
+            for (int n = 0; n < 4; n++) {
+                vector3 dir(planes[n][0], planes[n][1], planes[n][2]);
+                float d0 = dot(dir, tri.v0);
+                float d1 = d0 + dot(dir, tri.edge0);
+                float d2 = d0 + dot(dir, tri.edge1);
+                if (d0 > planes[n][3] && d1 > planes[n][3] && d2 > planes[n][3])
+                    inside = false;
+            }
+            if (inside) {
+                // IntersectTriangleTile<false>::setup (prim_tests.h:148-163)
+                tt.e0 = tri.edge0;
+                tt.e1 = tri.edge1;
+                tt.normal = cross(tt.e0, tt.e1);
+                tt.v0ToRayOrigin = rayOrigin - tri.v0;
+                tt.t = dot(tt.v0ToRayOrigin, tt.normal);
+                if (!(tt.t < 0.0f)) {
+                    // TestTriangleFrustaUVW (prim_tests.h:89-116)
+                    float uMin = FLT_MAX, uMax = -FLT_MAX, vMin = FLT_MAX, vMax = -FLT_MAX;
+                    float wMin = FLT_MAX, wMax = -FLT_MAX;
+                    for (int n = 0; n < 4; n++) {
+                        vector3 rdir = frustum.directions[n];
+                        vector3 o = frustum.origins[n];
+                        vector3 nn = cross(tri.edge0, tri.edge1);
+                        float denom = dot(-rdir, nn);
+                        vector3 v0o = o - tri.v0;
+                        float tr = dot(v0o, nn);
+                        vector3 e = cross(-rdir, v0o);
+                        float v = dot(tri.edge1, e);
+                        float w = -dot(tri.edge0, e);
+                        float ood = 1.0f / denom;
+                        tr *= ood;
+                        v *= ood;
+                        w *= ood;
+                        float u = 1.0f - v - w;
+                        (void)tr;
+                        uMin = fminf(uMin, u);
+                        uMax = fmaxf(uMax, u);
+                        vMin = fminf(vMin, v);
+                        vMax = fmaxf(vMax, v);
+                        wMin = fminf(wMin, w);
+                        wMax = fmaxf(wMax, w);
+                    }
+                    keep = !(uMax < 0 || uMin > 1 || vMax < 0 || vMin > 1 || wMax < 0 || wMin > 1);
+                }
+            }
+        }
+        // ---- ordered compaction into LDS ----
+        uint64_t ballot = __ballot(keep);
+        uint32_t prefix = __popcll(ballot & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            waveSurvivors[wave] = uint32_t(__popcll(ballot));
+        __syncthreads();
+        uint32_t total = waveSurvivors[0] + waveSurvivors[1];
+        if (keep) {
+            uint32_t slot = (wave ? waveSurvivors[0] : 0u) + prefix;
+            cache[slot] = tt;
+            cacheIndex[slot] = triIndex;
+        }
+        __syncthreads();
+        survivorsSum += total;
+
+        // ---- every sample against every survivor, in list order ----
+        for (uint32_t j = 0; j < total; j++) {
+            const TriTile& c = cache[j];
+            const uint32_t idx = cacheIndex[j];
+            // IntersectTriangleThread ctor + GetDifferentials (prim_tests.h:118-134, 175-188)
+            float denomCenter = dot(ndir, c.normal);
+            vector3 eCenter = cross(ndir, c.v0ToRayOrigin);
+            float vCenter = dot(c.e1, eCenter);
+            float wCenter = -dot(c.e0, eCenter);
+            float dD0 = dot(ndu, c.normal);
+            float dD1 = dot(ndv, c.normal);
+            vector3 cu = cross(ndu, c.v0ToRayOrigin);
+            vector3 cv = cross(ndv, c.v0ToRayOrigin);
+            float dV0 = dot(c.e1, cu), dV1 = dot(c.e1, cv);
+            float dW0 = -dot(c.e0, cu), dW1 = -dot(c.e0, cv);
+            const float t = c.t;
+#pragma unroll
+            for (int i = 0; i < A; i++) {
+                // IntersectTriangleThread::test (prim_tests.h:192-231)
+                const vector2 a = args.offsets[i];
+                float denom = fmaf(dD1, a.y, fmaf(dD0, a.x, denomCenter));
+                float depthDelta = fmaf(depth[i], denom, -t);
+                float v = fmaf(dV1, a.y, fmaf(dV0, a.x, vCenter));
+                float w = fmaf(dW1, a.y, fmaf(dW0, a.x, wCenter));
+                float u = (denom - v) - w;
+                int32_t test = int32_t(fbits(depthDelta) | fbits(u) | fbits(v) | fbits(w));
+                if (test >= 0) {
+                    depth[i] = t * (1.0f / denom);
+                    hit[i] = idx;
+                }
+            }
+        }
+        __syncthreads(); // before the next chunk overwrites the cache
+    }
+    if (args.survivors && tid == 0)
+        atomicAdd(args.survivors, (unsigned long long)survivorsSum * 128ull);
+
+    GBufferSubsample* waveBase = args.gbuffer + size_t(sample >> 6) * 64 * A;
+    emitGBuffer<A>(hit, waveBase, lane);
+}
+
+void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+    IntersectArgs a;
+    a.gbuffer = v->gbuffer.ptr;
+    a.beams = v->beams.ptr;
+    a.tileCounts = v->tileCounts.ptr;
+    a.tileOffsets = v->tileOffsets.ptr;
+    a.triIndices = v->triIndices.ptr;
+    a.worldTiles = v->tileFrustaWorld.ptr;
+    a.tris = s->triIntersect.ptr;
+    a.survivors = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    a.listCapacity = v->listCapacity;
+    a.cameraToWorld = m;
+    for (int i = 0; i < 32; i++)
+        a.offsets[i] = v->offsets[i];
+    dim3 grid(v->tileCount), block(128);
+    switch (v->msaaRate) {
+        case 1: intersectKernel<1><<<grid, block, 0, stream>>>(a); break;
+        case 2: intersectKernel<2><<<grid, block, 0, stream>>>(a); break;
+        case 4: intersectKernel<4><<<grid, block, 0, stream>>>(a); break;
+        case 8: intersectKernel<8><<<grid, block, 0, stream>>>(a); break;
+        case 16: intersectKernel<16><<<grid, block, 0, stream>>>(a); break;
+        case 32: intersectKernel<32><<<grid, block, 0, stream>>>(a); break;
+        default: break;
+    }
+}
+
+} // namespace hvvr

@@ -1,0 +1,253 @@
+// Resolve (MSAA GGX shading, ACES tone map, sRGB8) and remap for one view.
+//
+//   resolveKernel  ResolveKernel + ShadeMSAA (raycaster/resolve.cu:161-284, 324-384) and
+//                  ClearEmptyKernel (resolve.cu:527-542) in one launch: one 128-thread
+//                  workgroup per tile, one thread per sample; tiles whose triangle list
+//                  was empty write the clear colour 0xff000000 without touching the
+//                  GBuffer. Per unique triangle of a sample: centroid of its subsample
+//                  offsets, barycentrics (calcUVW, prim_tests.h:233-245), vertex
+//                  interpolation (shading.h:19-36), GGXShade (shading.h:84-161) weighted
+//                  by the covered subsample count.
+//   remapKernel    RemapKernel (raycaster/remap.cu:35-77): sample order -> image order,
+//                  split-colour merge for 2 (pentile) or 3 channels.
+//
+// Texture lookups are out of scope (the benchmark scenes are untextured): the UV
+// derivatives the reference computes for tex2DGrad (resolve.cu:264-275) only feed
+// texture reads, so they are not computed; materials with a texture index other than
+// SimpleMaterial::badTextureIndex shade with their constant colours.
+
+#include "device_common.h"
+#include "gpu_internal.h"
+
+namespace hvvr {
+
+struct ResolveArgs {
+    uint32_t* sampleResults;
+    const GBufferSubsample* gbuffer;
+    const DirectionalBeam* beams;
+    const uint32_t* tileCounts;
+    const PrecomputedTriangleIntersect* tris;
+    const PrecomputedTriangleShade* triShade;
+    const ShadingVertex* verts;
+    const SimpleMaterial* materials;
+    matrix4x4 cameraToWorld;
+    vector2 offsets[32];
+    LightingEnvironment env;
+};
+
+__device__ inline vector3 ggxSpecular(vector3 N, vector3 L, vector3 V, float roughness, vector3 F0) {
+    // GGX_SchlickFresnel_SchlickSmithGGX (shading_helpers.h:89-112)
+    float alpha = roughness * roughness;
+    vector3 H = normalize(L + V);
+    float alpha2 = alpha * alpha;
+    float NdotH = saturate(dot(N, H));
+    float q = NdotH * NdotH * (alpha2 - 1.0f) + 1.0f;
+    float D = alpha2 / (3.141592653589f * (q * q));
+    float LdotH = saturate(dot(L, H));
+    float x = 1.0f - LdotH;
+    float x2 = x * x;
+    float p5 = x2 * x2 * x;
+    vector3 F = F0 + (vector3(1.0f) - F0) * p5;
+    float NdotL = saturate(dot(N, L));
+    float NdotV = saturate(dot(N, V));
+    float k = alpha * 0.5f;
+    float vis = (1.0f / (NdotL * (1.0f - k) + k)) * (1.0f / (NdotV * (1.0f - k) + k));
+    return ((F * D) * vis) * 0.25f;
+}
+
+__device__ inline vector3 pbrBrdf(vector3 L, vector3 N, vector3 V, vector3 base, float roughness, vector3 F0,
+                                  vector3 lightColor) {
+    // PBR_BRDF1 (shading_helpers.h:115-122)
+    vector3 spec = ggxSpecular(N, L, V, roughness, F0);
+    float NdotL = saturate(dot(N, L));
+    vector3 diffuse = ((base * lightColor) * (vector3(1.0f) - spec)) * NdotL;
+    return diffuse + lightColor * spec;
+}
+
+__device__ inline vector3 ggxShade(const SimpleMaterial& mat, vector3 pos, vector3 nrm, vector3 camPos,
+                                   const LightingEnvironment& env) {
+    // GGXShade (shading.h:84-161), untextured
+    vector3 normal = normalize(nrm);
+    uint32_t model = uint32_t(mat.textureIDsAndShadingModel & 0xffffu);
+    if (model == ShadingModelEmissive)
+        return mat.emissive.xyz();
+    vector3 base = mat.diffuse.xyz();
+    vector3 V = normalize(camPos - pos);
+    vector3 F0 = mat.specular.xyz();
+    float roughness = 1.0f - mat.glossiness;
+    vector3 rad = mat.emissive.xyz();
+    rad += base * vector3(0.15f, 0.15f, 0.165f);
+    for (int i = 0; i < MAX_DIRECTIONAL_LIGHTS; i++) {
+        if (i >= env.directionalLightCount)
+            break;
+        rad += pbrBrdf(env.directionalLights[i].Direction, normal, V, base, roughness, F0,
+                       env.directionalLights[i].Power);
+    }
+    for (int i = 0; i < MAX_POINT_LIGHTS; i++) {
+        if (i >= env.pointLightCount)
+            break;
+        const LightPoint& l = env.pointLights[i];
+        vector3 delta = l.Position - pos;
+        float r = length(delta);
+        vector3 L = normalize(delta);
+        float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
+        rad += pbrBrdf(L, normal, V, base, roughness, F0, l.Color * fdist);
+    }
+    for (int i = 0; i < MAX_SPOT_LIGHTS; i++) {
+        if (i >= env.spotLightCount)
+            break;
+        const LightSpot& l = env.spotLights[i];
+        vector3 delta = l.Position - pos;
+        float r = length(delta);
+        vector3 L = normalize(delta);
+        float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
+        vector3 lit = pbrBrdf(L, normal, V, base, roughness, F0, l.Color * fdist);
+        float cosA = dot(l.Direction, -L);
+        float fang = saturate((cosA - l.CosOuterAngle) * l.CosAngleScale);
+        rad += lit * fang;
+    }
+    return vector3(fmaxv(rad.x, 0.0f), fmaxv(rad.y, 0.0f), fmaxv(rad.z, 0.0f));
+}
+
+__device__ inline float acesFilm(float x) {
+    // ACESFilm (shading_helpers.h:158-170)
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    return saturate((x * (a * x + b)) / (x * (c * x + d) + e));
+}
+__device__ inline float linearToSRgb(float x) {
+    // util_graphics.h:26-28
+    return (x <= 0.0031308f) ? (x * 12.92f) : (powf(x, 1.0f / 2.4f) * 1.055f - 0.055f);
+}
+
+template <int A>
+__global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t sample = tile * TILE_SIZE + tid;
+    if (args.tileCounts[tile] == 0) {
+        args.sampleResults[sample] = 0xff000000u;
+        return;
+    }
+    const uint32_t lane = tid & 63u;
+    const GBufferSubsample* waveBase = args.gbuffer + size_t(sample >> 6) * 64 * A;
+    const matrix4x4& m = args.cameraToWorld;
+    const matrix3x3 rot = m.upper3x3();
+    const vector3 cam = m.m3.xyz();
+    DirectionalBeam beam = args.beams[sample];
+    const vector3 ndir = -(rot * beam.centerRay);
+    const vector3 ndu = -(rot * beam.du);
+    const vector3 ndv = -(rot * beam.dv);
+    const uint32_t maskAll = A < 32 ? ((1u << A) - 1u) : 0xffffffffu;
+
+    vector3 result(0.0f);
+    uint32_t combined = 0;
+    for (int slot = 0; slot < A; slot++) {
+        if (combined == maskAll)
+            break;
+        GBufferSubsample g = waveBase[slot * 64 + lane];
+        if (g.triIndex == badTriIndex)
+            break;
+        uint32_t mask = g.sampleMask;
+        combined |= mask;
+        float inv = 1.0f / float(__popc(mask));
+        float cx = 0.0f, cy = 0.0f;
+        for (uint32_t bits = mask; bits; bits &= bits - 1) {
+            int i = __ffs(bits) - 1;
+            vector2 o = args.offsets[i];
+            cx = cx + o.x * inv;
+            cy = cy + o.y * inv;
+        }
+        PrecomputedTriangleIntersect T = args.tris[g.triIndex];
+        vector3 normal = cross(T.edge0, T.edge1);
+        vector3 v0o = cam - T.v0;
+        float denomC = dot(ndir, normal);
+        vector3 eC = cross(ndir, v0o);
+        float vC = dot(T.edge1, eC);
+        float wC = -dot(T.edge0, eC);
+        float dD0 = dot(ndu, normal), dD1 = dot(ndv, normal);
+        vector3 cu = cross(ndu, v0o), cv = cross(ndv, v0o);
+        float dV0 = dot(T.edge1, cu), dV1 = dot(T.edge1, cv);
+        float dW0 = -dot(T.edge0, cu), dW1 = -dot(T.edge0, cv);
+        float denom = fmaf(dD1, cy, fmaf(dD0, cx, denomC));
+        float v = fmaf(dV1, cy, fmaf(dV0, cx, vC));
+        float w = fmaf(dW1, cy, fmaf(dW0, cx, wC));
+        float u = (denom - v) - w;
+        float dinv = 1.0f / denom;
+        float bx = u * dinv, by = v * dinv, bz = w * dinv;
+        PrecomputedTriangleShade S = args.triShade[g.triIndex];
+        ShadingVertex V0 = args.verts[S.indices[0]], V1 = args.verts[S.indices[1]], V2 = args.verts[S.indices[2]];
+        vector3 pos = V0.pos * bx + V1.pos * by + V2.pos * bz;
+        vector3 n0(halfToFloat(V0.normal[0]), halfToFloat(V0.normal[1]), halfToFloat(V0.normal[2]));
+        vector3 n1(halfToFloat(V1.normal[0]), halfToFloat(V1.normal[1]), halfToFloat(V1.normal[2]));
+        vector3 n2(halfToFloat(V2.normal[0]), halfToFloat(V2.normal[1]), halfToFloat(V2.normal[2]));
+        vector3 nrm = n0 * bx + n1 * by + n2 * bz;
+        vector3 c = ggxShade(args.materials[S.material], pos, nrm, cam, args.env);
+        result += c * float(__popc(mask));
+    }
+    result = result * (1.0f / float(A));
+    float r = linearToSRgb(acesFilm(result.x));
+    float gch = linearToSRgb(acesFilm(result.y));
+    float b = linearToSRgb(acesFilm(result.z));
+    uint32_t ri = uint32_t(r * 255.0f), gi = uint32_t(gch * 255.0f), bi = uint32_t(b * 255.0f);
+    args.sampleResults[sample] = ri | (gi << 8) | (bi << 16) | 0xff000000u;
+}
+
+__global__ void remapKernel(const uint32_t* __restrict__ src, const int32_t* __restrict__ remap,
+                            uint32_t* __restrict__ image, uint32_t width, uint32_t height, uint32_t split) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height)
+        return;
+    size_t pix = size_t(y) * width + x;
+    if (split == 1) {
+        int32_t o = remap[pix];
+        if (o >= 0)
+            image[pix] = src[o];
+        return;
+    }
+    uint32_t c[3] = {0, 0, 0};
+    for (uint32_t ch = 0; ch < split; ch++) {
+        int32_t o = remap[pix * split + ch];
+        if (o >= 0)
+            c[ch] = src[o];
+    }
+    if (split == 2) // mergeSplitColorsPentile (remap.cu:25-31)
+        image[pix] = (c[1] & 0xffu) | (c[0] & 0xff00u) | (c[1] & 0xff0000u) | 0xff000000u;
+    else // mergeSplitColors (remap.cu:18-23)
+        image[pix] = (c[0] & 0xffu) | (c[1] & 0xff00u) | (c[2] & 0xff0000u) | 0xff000000u;
+}
+
+void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+    ResolveArgs a;
+    a.sampleResults = v->sampleResults.ptr;
+    a.gbuffer = v->gbuffer.ptr;
+    a.beams = v->beams.ptr;
+    a.tileCounts = v->tileCounts.ptr;
+    a.tris = s->triIntersect.ptr;
+    a.triShade = s->triShade.ptr;
+    a.verts = s->vertsWorld.ptr;
+    a.materials = s->materials.ptr;
+    a.cameraToWorld = m;
+    for (int i = 0; i < 32; i++)
+        a.offsets[i] = v->offsets[i];
+    a.env = s->lighting;
+    dim3 grid(v->tileCount), block(128);
+    switch (v->msaaRate) {
+        case 1: resolveKernel<1><<<grid, block, 0, stream>>>(a); break;
+        case 2: resolveKernel<2><<<grid, block, 0, stream>>>(a); break;
+        case 4: resolveKernel<4><<<grid, block, 0, stream>>>(a); break;
+        case 8: resolveKernel<8><<<grid, block, 0, stream>>>(a); break;
+        case 16: resolveKernel<16><<<grid, block, 0, stream>>>(a); break;
+        case 32: resolveKernel<32><<<grid, block, 0, stream>>>(a); break;
+        default: break;
+    }
+}
+
+void launchRemap(hvvr_gpu_view* v, hipStream_t stream) {
+    dim3 block(16, 8);
+    dim3 grid((v->width + 15) / 16, (v->height + 7) / 8);
+    remapKernel<<<grid, block, 0, stream>>>(v->sampleResults.ptr, v->remap.ptr, v->image.ptr, v->width, v->height,
+                                            v->split);
+}
+
+} // namespace hvvr

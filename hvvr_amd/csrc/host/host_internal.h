@@ -1,0 +1,143 @@
+#pragma once
+// Host-side (CPU, one-off per scene / per camera configuration) building blocks:
+// BVH construction, scene flattening, sample hierarchy setup and the seeded scene
+// generators used by the benchmarks and tests.
+
+#include "../common/hvvr_types.h"
+#include "../common/subsample.h"
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+namespace hvvr {
+
+// raycaster/model.h:26-31
+struct MeshData {
+    std::vector<ShadingVertex> verts;
+    std::vector<PrecomputedTriangleShade> triShade;
+    std::vector<TopologyNode> nodes;
+    std::vector<SimpleMaterial> materials;
+};
+
+// SAH BVH2 -> BVH4 -> depth-first topology; reorders triangles and vertices
+// (shared/bvh.cpp:542-556). Returns false (with a message) on malformed meshes.
+bool GenerateTopology(MeshData& mesh, std::string* error = nullptr);
+
+// One model of a scene: mesh + model-to-world transform.
+struct SceneModel {
+    const MeshData* mesh;
+    matrix4x4 modelToWorld;
+};
+
+// Flattened scene in the global layout the GPU consumes (raycaster/scene_update.cpp:22-143).
+struct FlatScene {
+    std::vector<ShadingVertex> verts;         // model space
+    std::vector<uint32_t> vertexToTransform;  // model index per vertex
+    std::vector<PrecomputedTriangleShade> triShade;
+    std::vector<BVHNode> nodes;               // topology filled, bounds zero
+    std::vector<SimpleMaterial> materials;
+    std::vector<matrix4x4> transforms;        // model to world, one per model
+    std::vector<LightUnion> lights;
+};
+void buildScene(const std::vector<SceneModel>& models, FlatScene& out);
+
+// ---------------------------------------------------------------------------------
+// samples (raycaster/samples.h:17-67)
+// ---------------------------------------------------------------------------------
+struct Sample {
+    vector2ui pixelLocation;
+    vector2 position;
+    struct Extents {
+        vector2 minorAxis;
+        float majorAxisLength;
+    } extents;
+};
+static_assert(sizeof(Sample) == 28, "Sample layout");
+
+std::vector<Sample> getGridSamples(uint32_t width, uint32_t height);
+// Radial barrel distortion r' = r (1 + k1 r^2 + k2 r^4) applied to sample positions in
+// centred sample space, extents scaled by the local radial magnification.
+void applyBarrelDistortion(std::vector<Sample>& samples, float k1, float k2);
+
+// Camera-space sample hierarchy ready for rendering (raycaster/camera.h:28-47).
+struct SampleSetup {
+    uint32_t validSampleCount = 0;
+    uint32_t sampleCount = 0; // padded to a multiple of BLOCK_SIZE
+    uint32_t blockCount = 0;
+    uint32_t splitColorSamples = 1;
+    uint32_t width = 0, height = 0;
+    std::vector<DirectionalBeam> beams;         // [sampleCount]
+    std::vector<SimpleRayFrustum> tileFrusta;   // [blockCount * TILES_PER_BLOCK]
+    std::vector<SimpleRayFrustum> blockFrusta;  // [blockCount]
+    std::vector<int32_t> remap;                 // [width * height * split], -1 = no sample
+    vector2 boundsLower, boundsUpper;
+};
+// camera.cpp:315-319
+matrix3x3 sampleToCamera(vector2 viewportLower, vector2 viewportUpper);
+// camera.cpp:21-64 + sample_hierarchy.cpp:149-293 (perspective mapping)
+void buildSampleSetup(const Sample* samples, uint32_t count, uint32_t split, uint32_t width, uint32_t height,
+                      const matrix3x3& sampleToCam, ThinLens lens, SampleSetup& out);
+
+// Depth-of-field lens table: 4 tiles x 128 samples x A offsets (raycaster/gpu_camera.cu:59-88).
+void lensPositionTable(uint32_t msaaRate, std::vector<vector2>& out);
+
+// ---------------------------------------------------------------------------------
+// seeded generators
+// ---------------------------------------------------------------------------------
+struct Pcg32 {
+    uint64_t state, inc;
+    explicit Pcg32(uint64_t seed, uint64_t stream = 0x9e3779b97f4a7c15ull) {
+        state = 0;
+        inc = (stream << 1u) | 1u;
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        uint32_t xorshifted = uint32_t(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = uint32_t(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+    }
+    // uniform double in [0,1) with 53 random bits
+    double uniform() {
+        uint64_t hi = next(), lo = next();
+        return double(((hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
+    }
+    float uniformf(float lo, float hi) { return float(lo + (hi - lo) * uniform()); }
+    double normal() {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300)
+            u1 = 1e-300;
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+};
+
+struct SceneGenResult {
+    std::vector<MeshData> meshes;
+    std::vector<matrix4x4> modelToWorld;
+    std::vector<LightUnion> lights;
+};
+// 64x64-cell heightfield (8192 triangles) + 256 boxes (3072 triangles), per-triangle GGX
+// materials, one directional light.
+void generateHeightfieldBoxesScene(uint64_t seed, SceneGenResult& out);
+// meshCount seeded meshes of 256 triangles (subdivided boxes and cylinders) in a
+// 30 x 15 x 30 m room, 4 point lights + 1 directional light.
+void generateRoomScene(uint64_t seed, uint32_t meshCount, SceneGenResult& out);
+
+// Camera presets for the benchmark configurations.
+struct CameraPreset {
+    uint32_t width, height;
+    vector2 viewportLower, viewportUpper;
+    float barrelK1, barrelK2;
+    transform cameraToWorld;
+    ThinLens lens;
+};
+// config 1: pinhole 640x480, 60 degree vertical FOV looking at the heightfield
+CameraPreset heightfieldCamera();
+// config 2: one eye of a 1080x1200 stereo pair with barrel distortion, eye = 0 or 1
+CameraPreset roomEyeCamera(uint32_t eye, uint32_t width, uint32_t height);
+
+} // namespace hvvr

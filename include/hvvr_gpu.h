@@ -1,0 +1,144 @@
+#ifndef HVVR_GPU_H
+#define HVVR_GPU_H
+/*
+ * Thin C ABI over the gfx950 (MI355X) HIP implementation of HVVR's per-frame render
+ * path. Everything the reference splits between its CPU traversal and its CUDA
+ * kernels runs on the GPU behind these entry points: per-tile frustum setup, block
+ * and tile frustum-BVH traversal, triangle-list intersection, GBuffer compression,
+ * GGX resolve, remap, and the scene refit.
+ *
+ * Plain pointers and sizes only. Struct arrays passed as `const void*` use the
+ * reference's byte layouts (ShadingVertex 32 B, PrecomputedTriangleShade 16 B,
+ * BVHNode 128 B, SimpleMaterial 64 B, LightUnion 56 B, DirectionalBeam 36 B,
+ * SimpleRayFrustum 96 B, matrices column-major float[16]).
+ *
+ * Errors: every function returns HVVR_GPU_OK (0) or a negative HVVR_GPU_ERR_* code;
+ * hvvr_gpu_last_error() returns a description of the last failure on the calling
+ * thread. (The reference has no error returns: cutilSafeCall prints and exits,
+ * raycaster/cuda_util.h:18-29.)
+ *
+ * Threading: a context, and the scenes and views created from it, are used from one
+ * host thread at a time. Work is issued asynchronously on the context's HIP stream;
+ * read-back functions synchronise that stream.
+ */
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HVVR_GPU_OK = 0,
+    HVVR_GPU_ERR_INVALID = -1,   /* bad argument */
+    HVVR_GPU_ERR_HIP = -2,       /* a HIP runtime call failed */
+    HVVR_GPU_ERR_NO_DEVICE = -3, /* no GPU / no gfx950 device */
+    HVVR_GPU_ERR_OVERFLOW = -4,  /* a traversal stack or triangle-list budget was exceeded */
+    HVVR_GPU_ERR_STATE = -5      /* call out of order (e.g. render before a scene update) */
+};
+
+typedef struct hvvr_gpu_context hvvr_gpu_context;
+typedef struct hvvr_gpu_scene hvvr_gpu_scene;
+typedef struct hvvr_gpu_view hvvr_gpu_view;
+
+const char* hvvr_gpu_last_error(void);
+/* Library version string, e.g. "hvvr_gpu 0.1 gfx950". Never fails. */
+const char* hvvr_gpu_version(void);
+
+/* ---- context: replaces GPUContext::cudaInit / cleanup (raycaster/gpu_context.h:27-42) ---- */
+int hvvr_gpu_context_create(int device, hvvr_gpu_context** out);
+int hvvr_gpu_context_destroy(hvvr_gpu_context* ctx);
+int hvvr_gpu_context_synchronize(hvvr_gpu_context* ctx);
+/* 1 = record per-stage HIP events for hvvr_gpu_view_stage_times */
+int hvvr_gpu_context_set_profiling(hvvr_gpu_context* ctx, int enable);
+
+/* ---- scene: replaces GPUSceneState (raycaster/gpu_scene_state.h:31-73) ---- */
+typedef struct {
+    const void* vertices;               /* ShadingVertex[vertexCount], model space */
+    const uint32_t* vertexToTransform;  /* [vertexCount], index into transforms */
+    uint32_t vertexCount;
+    const void* triangles;              /* PrecomputedTriangleShade[triangleCount], global vertex indices */
+    uint32_t triangleCount;             /* <= 2^26 (GBuffer sort-key limit, intersect.cu:52-71) */
+    const void* nodes;                  /* BVHNode[nodeCount]: topology only, bounds are computed here */
+    uint32_t nodeCount;
+    const void* materials;              /* SimpleMaterial[materialCount] */
+    uint32_t materialCount;
+    const float* transforms;            /* [transformCount][16] model-to-world */
+    uint32_t transformCount;
+} hvvr_gpu_scene_desc;
+
+/* GPUSceneState::setGeometry + updateMaterials (gpu_scene_state.cu:469-599, :83-85).
+ * Uploads the scene and derives the refit plan; bounds are computed by the next update. */
+int hvvr_gpu_scene_create(hvvr_gpu_context* ctx, const hvvr_gpu_scene_desc* desc, hvvr_gpu_scene** out);
+int hvvr_gpu_scene_destroy(hvvr_gpu_scene* scene);
+/* GPUSceneState::animate (gpu_scene_state.cu:601-606): new model-to-world matrices. */
+int hvvr_gpu_scene_set_transforms(hvvr_gpu_scene* scene, const float* transforms, uint32_t count);
+/* Replace model-space vertices [first, first+count) (per-vertex animation; no reference
+ * counterpart, the reference only animates per-model transforms). */
+int hvvr_gpu_scene_set_vertices(hvvr_gpu_scene* scene, uint32_t first, uint32_t count, const void* vertices);
+/* GPUSceneState::updateLighting (gpu_scene_state.cu:47-81): LightUnion[count]; lights past
+ * 1 directional / 4 point / 4 spot are dropped and counted in *dropped (may be NULL). */
+int hvvr_gpu_scene_set_lights(hvvr_gpu_scene* scene, const void* lights, uint32_t count, uint32_t* dropped);
+/* GPUSceneState::update (gpu_scene_state.cu:377-451): vertex transform, triangle precompute
+ * and bottom-up BVH refit, issued on the context stream when the scene is dirty. */
+int hvvr_gpu_scene_update(hvvr_gpu_scene* scene);
+/* GPUSceneState::fetchUpdatedBVH (gpu_scene_state.cu:454-467) and friends: synchronous reads. */
+int hvvr_gpu_scene_read_nodes(hvvr_gpu_scene* scene, void* nodes);          /* BVHNode[nodeCount] */
+int hvvr_gpu_scene_read_triangles(hvvr_gpu_scene* scene, void* triangles);  /* PrecomputedTriangleIntersect[] */
+int hvvr_gpu_scene_read_vertices(hvvr_gpu_scene* scene, void* vertices);    /* world-space ShadingVertex[] */
+
+/* ---- view (one camera): replaces GPUCamera (raycaster/gpu_camera.h:100-228) ---- */
+typedef struct {
+    uint32_t msaaRate;              /* 1, 2, 4, 8, 16 or 32 (COLOR_MODE_MSAA_RATE, raycaster_common.h:27) */
+    uint32_t sampleCount;           /* padded sample count, a multiple of 8192 */
+    uint32_t validSampleCount;
+    const void* beams;              /* DirectionalBeam[sampleCount], camera space */
+    const void* tileFrusta;         /* SimpleRayFrustum[sampleCount/128], camera space */
+    const void* blockFrusta;        /* SimpleRayFrustum[sampleCount/8192], camera space */
+    const int32_t* remap;           /* [width*height*split], sample index or -1 */
+    uint32_t width, height, split;  /* output image; split = colour samples per pixel (1..3) */
+    float lensRadius, focalDistance;/* ThinLens; lensRadius must be 0 (pinhole) in this version */
+} hvvr_gpu_view_desc;
+
+/* GPUCamera::updateConfig (gpu_camera.cu:149-192). */
+int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* desc, hvvr_gpu_view** out);
+int hvvr_gpu_view_destroy(hvvr_gpu_view* view);
+/* Raycaster::intersectAndResolveBeamBatch + GPUCamera::intersectShadeResolve + remap
+ * (render.cpp:418-451, gpu_camera.cu:271-293, remap.cu:79-124): one frame of one view.
+ * cameraToWorld: float[16]; jitter: float[2] (frame jitter, used when the subsample
+ * spin is enabled, see hvvr_gpu_view_set_jitter_spin). Asynchronous. */
+int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float* cameraToWorld,
+                         const float* jitter);
+/* Several views of one scene in one submission (batched multi-view). */
+int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
+                          const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
+/* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
+int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
+
+/* Synchronous reads of the last rendered frame. */
+int hvvr_gpu_view_read_image(hvvr_gpu_view* view, uint32_t* image, uint32_t strideInPixels);
+int hvvr_gpu_view_read_sample_colors(hvvr_gpu_view* view, uint32_t* colors); /* [sampleCount] RGBA8 */
+/* GBuffer in logical order: for sample s, slots s*A .. s*A+A-1 as (triIndex, sampleMask),
+ * ascending triangle id, misses (0xffffffff) last, unused slots (0xffffffff, 0). */
+int hvvr_gpu_view_read_gbuffer(hvvr_gpu_view* view, uint32_t* triIndex, uint32_t* sampleMask);
+/* Per-tile triangle lists of the last frame in tile order: counts[tileCount]; lists
+ * concatenated into triIndices (capacity `capacity` entries); *total = list entries. */
+int hvvr_gpu_view_read_tile_lists(hvvr_gpu_view* view, uint32_t* counts, uint32_t* triIndices, uint64_t capacity,
+                                  uint64_t* total);
+typedef struct {
+    uint64_t listEntries;      /* triangle-list entries over all tiles */
+    uint64_t survivorPairs;    /* sum over tiles of 128 x triangles surviving the tile culls */
+    uint32_t occupiedTiles;
+    uint32_t tileCount;
+    uint32_t maxTileStack;     /* deepest tile traversal stack seen */
+    uint32_t overflow;         /* nonzero if a budget overflowed */
+} hvvr_gpu_view_stats;
+int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
+/* Per-stage GPU milliseconds of the last profiled frame (needs profiling enabled):
+ * [0] frustum setup, [1] block traversal, [2] tile traversal (count), [3] scan,
+ * [4] tile traversal (emit), [5] intersect, [6] resolve, [7] remap. Returns the count. */
+int hvvr_gpu_view_stage_times(hvvr_gpu_view* view, float* ms, int maxCount);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

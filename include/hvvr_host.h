@@ -1,0 +1,88 @@
+#ifndef HVVR_HOST_H
+#define HVVR_HOST_H
+/*
+ * C interface to the host-side (CPU, one-off) half of the library: seeded scene
+ * generation, BVH construction, scene flattening and camera sample setup. It has no
+ * counterpart in the reference, which is a static C++ library; it exists so test
+ * harnesses and benchmarks written in other languages can build the exact inputs the
+ * C++ API (include/hvvr/raycaster.h) builds, and hand them to the GPU C ABI
+ * (include/hvvr_gpu.h).
+ *
+ * All functions return 0 on success and a negative code on failure;
+ * hvvr_host_last_error() describes the last failure on the calling thread.
+ */
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hvvr_host_scene hvvr_host_scene;
+typedef struct hvvr_host_view hvvr_host_view;
+
+enum {
+    HVVR_SCENE_HEIGHTFIELD_BOXES = 1, /* 64x64 heightfield + 256 boxes, 11,264 triangles */
+    HVVR_SCENE_ROOM = 2               /* meshCount x 256-triangle meshes in a 30x15x30 m room */
+};
+enum {
+    HVVR_SCENE_VERTS = 0,           /* ShadingVertex[verts], model space */
+    HVVR_SCENE_VERTEX_TRANSFORM = 1,/* uint32[verts] */
+    HVVR_SCENE_TRI_SHADE = 2,       /* PrecomputedTriangleShade[tris] */
+    HVVR_SCENE_NODES = 3,           /* BVHNode[nodes], topology only */
+    HVVR_SCENE_MATERIALS = 4,       /* SimpleMaterial[materials] */
+    HVVR_SCENE_TRANSFORMS = 5,      /* float[16] column-major per model */
+    HVVR_SCENE_LIGHTS = 6           /* LightUnion[lights], 56 bytes each */
+};
+
+const char* hvvr_host_last_error(void);
+
+int hvvr_host_scene_create(int kind, uint64_t seed, uint32_t meshCount, hvvr_host_scene** out);
+/* Build a scene from raw meshes: meshes[i] uses vertices [vertexStart[i], vertexStart[i+1])
+ * and triangles [triStart[i], triStart[i+1]) with mesh-local vertex indices; material of each
+ * triangle indexes materials[] of the scene (materials are not duplicated per mesh). */
+int hvvr_host_scene_create_from_meshes(uint32_t meshCount, const uint32_t* vertexStart, const void* verts,
+                                       const uint32_t* triStart, const uint32_t* triIndices,
+                                       const uint32_t* triMaterial, const float* transforms16,
+                                       uint32_t materialCount, const void* materials, uint32_t lightCount,
+                                       const void* lights, hvvr_host_scene** out);
+void hvvr_host_scene_destroy(hvvr_host_scene* scene);
+/* counts[0..6] = verts, tris, nodes, materials, transforms, lights, models */
+int hvvr_host_scene_counts(const hvvr_host_scene* scene, uint64_t counts[7]);
+int hvvr_host_scene_copy(const hvvr_host_scene* scene, int array, void* dst, uint64_t bytes);
+
+enum {
+    HVVR_VIEW_PRESET_HEIGHTFIELD = 1, /* pinhole 640x480, 60 deg vertical FOV */
+    HVVR_VIEW_PRESET_ROOM_EYE = 2     /* one barrel-distorted eye of a stereo pair */
+};
+enum {
+    HVVR_VIEW_BEAMS = 0,        /* DirectionalBeam[sampleCount], camera space */
+    HVVR_VIEW_TILE_FRUSTA = 1,  /* SimpleRayFrustum[blockCount*64], camera space */
+    HVVR_VIEW_BLOCK_FRUSTA = 2, /* SimpleRayFrustum[blockCount], camera space */
+    HVVR_VIEW_REMAP = 3         /* int32[width*height*split] */
+};
+int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint32_t height, hvvr_host_view** out);
+/* samples: Sample[count] (28 bytes each: pixel x,y u32; position x,y f32; minor axis x,y f32;
+ * major axis length f32) */
+int hvvr_host_view_create_from_samples(const void* samples, uint32_t count, uint32_t split, uint32_t width,
+                                       uint32_t height, const float viewportLowerUpper[4], float lensRadius,
+                                       float focalDistance, hvvr_host_view** out);
+void hvvr_host_view_destroy(hvvr_host_view* view);
+/* counts[0..6] = validSampleCount, sampleCount, blockCount, width, height, split, tileCount */
+int hvvr_host_view_counts(const hvvr_host_view* view, uint64_t counts[7]);
+int hvvr_host_view_copy(const hvvr_host_view* view, int array, void* dst, uint64_t bytes);
+/* camera-to-world matrix (column-major float[16]) and lens (radius, focal distance) of a preset */
+int hvvr_host_view_camera(const hvvr_host_view* view, float cameraToWorld[16], float lens[2]);
+
+/* Grid samples (Sample[width*height]) with optional barrel distortion. */
+int hvvr_host_grid_samples(uint32_t width, uint32_t height, float k1, float k2, void* dst);
+/* Spiral subsample offsets: out[2*i], out[2*i+1] for i < msaaRate. */
+int hvvr_host_subsample_offsets(uint32_t msaaRate, float spinAngle, float* out);
+/* Depth-of-field lens table: 4*128*msaaRate float2 entries. */
+int hvvr_host_lens_table(uint32_t msaaRate, float* out);
+/* Matrix of a transform (translation xyz, quaternion xyzw, scale). */
+int hvvr_host_transform_matrix(const float translation[3], const float rotation[4], float scale, float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,126 @@
+#ifndef HVVR_ORACLE_H
+#define HVVR_ORACLE_H
+/*
+ * CPU oracle for the HVVR raycasting hot path. TEST INFRASTRUCTURE ONLY: nothing in
+ * the product (hvvr_amd/, include/) links, loads or calls this code; only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the checker.
+ *
+ * It restates, in plain C99 and scalar form, the reference algorithms the GPU
+ * kernels implement (citations are relative to the reference repository root,
+ * libraries/hvvr/...):
+ *   frustum setup      raycaster/traversal.h:38-47, raycaster/traverse_avx.cpp:22-211
+ *   child AABB tests   raycaster/traverse_avx.h:104-217
+ *   block traversal    raycaster/traverse_avx.h:293-410
+ *   tile traversal     raycaster/traverse_avx.h:412-527, raycaster/traversal.cpp:99-211
+ *   intersect          raycaster/intersect.cu:117-342, raycaster/prim_tests.h:18-245
+ *   GBuffer compress   raycaster/intersect.cu:28-89, raycaster/sort.h:40-74
+ *   resolve / shade    raycaster/resolve.cu:161-284,324-384, raycaster/shading.h:19-161,
+ *                      raycaster/shading_helpers.h:89-200, shared/util_graphics.h:26-28
+ *   clear / remap      raycaster/resolve.cu:527-542, raycaster/remap.cu:35-77
+ *   refit              raycaster/gpu_scene_state.cu:88-375
+ *
+ * Arithmetic contract (shared with the GPU kernels, so results compare bit for bit):
+ * compiled with -ffp-contract=off; fmaf() exactly where the reference fuses
+ * (_mm256_fmadd_ps and the "n FMA" annotations of prim_tests.h); correctly rounded
+ * division and sqrt; IEEE denormals. The reference's own binaries run with fast-math
+ * and FTZ/DAZ on NVIDIA hardware and cannot be run here, so agreement with them is
+ * not claimed bit for bit; each function's pinning is listed in DESIGN.md.
+ *
+ * All layouts below are byte-identical to the product's (hvvr_amd/csrc/common/hvvr_types.h),
+ * which a CPU test checks field by field.
+ */
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y; } ov2;
+typedef struct { float x, y, z; } ov3;
+
+typedef struct { ov3 pos; uint16_t normal[4]; ov2 uv; uint32_t pad; } o_vertex;          /* 32 B */
+typedef struct { ov3 v0, e0, e1; } o_tri;                                                /* 36 B */
+typedef struct { uint32_t idx[3]; uint32_t material; } o_trishade;                       /* 16 B */
+typedef struct { float b[6][4]; uint32_t leafMask; uint32_t data[5]; uint32_t pad[2]; } o_node; /* 128 B */
+typedef struct { float emissive[4], diffuse[4], specular[4]; float glossiness, opacity; uint64_t tex; } o_material;
+typedef struct { ov3 o[4]; ov3 d[4]; } o_sfrustum;                                       /* 96 B */
+typedef struct { ov3 c, du, dv; } o_beam;                                                /* 36 B */
+typedef struct {
+    float planeNeg[4][4], planePos[4][4];
+    float projLo[3], projHi[3];
+    float edge[12][3], edgeMin[12], edgeMax[12];
+    int32_t axis, degenerate;
+} o_frustum;                                                                             /* 400 B */
+typedef struct { float tMin; uint32_t nodeMask; } o_stack;                               /* 8 B */
+
+typedef struct { ov3 Direction, Power; } o_ldir;
+typedef struct { ov3 Position; float FalloffEnd; ov3 Color; float FalloffScale; } o_lpoint;
+typedef struct { ov3 Position; float FalloffEnd; ov3 Direction; float FalloffScale; ov3 Color;
+                 float CosOuterAngle, CosAngleScale; } o_lspot;
+typedef struct {
+    o_ldir dir[1]; o_lpoint point[4]; o_lspot spot[4];
+    int32_t dirCount, pointCount, spotCount;
+} o_lightenv;                                                                            /* 372 B */
+
+/* sizes of the layouts, for the layout check */
+void or_layout_sizes(uint32_t out[10]);
+
+/* ---- frustum setup ---- */
+/* in: n camera-space frusta; m: camera-to-world (column-major 4x4).
+ * out: world-space frusta and the pre-swizzled test data. */
+void or_frustum_setup(const o_sfrustum* in, uint32_t n, const float m[16], o_sfrustum* worldOut, o_frustum* preOut);
+/* AABB tests on one node: coarse mask, then refine of a previous mask */
+uint32_t or_test_children(const o_frustum* f, const o_node* node);
+uint32_t or_test_children_refine(const o_frustum* f, const o_node* node, uint32_t prev);
+
+/* ---- traversal ---- */
+/* block traversal + distance sort; returns the stack size (<= 64) */
+uint32_t or_block_traverse(const o_node* nodes, const o_frustum* f, o_stack* sorted);
+/* tile traversal from a block's sorted stack; returns the triangle count, or -1 when the
+ * 4096-entry stack or the output capacity would overflow */
+int64_t or_tile_traverse(const o_node* nodes, const o_stack* blockStack, uint32_t stackSize, const o_frustum* f,
+                         uint32_t* out, uint64_t cap);
+/* all tiles of a view in tile order: counts[tile], lists concatenated into out; returns
+ * total or -1 on overflow */
+int64_t or_build_tile_lists(const o_node* nodes, const o_frustum* blockPre, const o_frustum* tilePre,
+                            uint32_t blockCount, uint32_t* counts, uint32_t* out, uint64_t cap);
+
+/* ---- intersect ---- */
+/* Intersects one view. worldTiles: world-space tile frusta; lists as from
+ * or_build_tile_lists. Outputs per subsample (sample*A + i): triangle id (0xffffffff =
+ * miss) and depth (FLT_MAX on miss). survivorsOut: number of (tile, triangle) pairs
+ * surviving the tile culls, summed over tiles. */
+int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustum* worldTiles, uint32_t tileCount,
+                      const float m[16], const ov2* offsets, uint32_t A, const uint32_t* counts,
+                      const uint32_t* lists, const o_tri* tris, uint32_t* triOut, float* depthOut,
+                      uint64_t* survivorsOut);
+/* Compresses one sample's A subsample ids into (tri, mask) slots; returns the slot count. */
+uint32_t or_emit_gbuffer(const uint32_t* sampleTri, uint32_t A, uint32_t* slotTri, uint32_t* slotMask);
+
+/* ---- resolve ---- */
+/* Shades every sample from its subsample ids. colorOut: RGBA8 per sample;
+ * linearOut (optional): pre-tonemap linear radiance / A (rgb) per sample.
+ * precision: 0 = float (same operation order as the kernels), 1 = double. */
+int or_resolve_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                    const uint32_t* triIds, const o_tri* tris, const o_trishade* shade, const o_vertex* verts,
+                    const o_material* materials, const o_lightenv* env, int precision, uint32_t* colorOut,
+                    float* linearOut);
+/* image[y*stride + x] = results[remap[...]] (split 1, 2 or 3) */
+void or_remap(const uint32_t* results, const int32_t* remap, uint32_t width, uint32_t height, uint32_t split,
+              uint32_t stride, uint32_t* image);
+
+/* ---- refit ---- */
+/* World-space vertices, precomputed triangles and node bounds from scratch. nodes:
+ * topology (bounds ignored), nodesOut gets bounds filled. */
+int or_refit(const o_vertex* verts, uint32_t vertexCount, const uint32_t* vertexToTransform, const float* transforms,
+             const o_trishade* shade, uint32_t triCount, const o_node* nodes, uint32_t nodeCount,
+             o_vertex* worldVerts, o_tri* trisOut, o_node* nodesOut);
+
+/* lighting environment from a light list (LightUnion, 56 bytes each); returns the
+ * number of lights dropped for exceeding the per-type maxima */
+int or_lighting_env(const void* lights, uint32_t count, o_lightenv* env);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

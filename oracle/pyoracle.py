@@ -1,0 +1,207 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import c_int, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "_build", "liboracle.so")
+
+O_FRUSTUM = np.dtype([("planeNeg", "<f4", (4, 4)), ("planePos", "<f4", (4, 4)), ("projLo", "<f4", 3),
+                      ("projHi", "<f4", 3), ("edge", "<f4", (12, 3)), ("edgeMin", "<f4", 12),
+                      ("edgeMax", "<f4", 12), ("axis", "<i4"), ("degenerate", "<i4")])
+O_STACK = np.dtype([("tMin", "<f4"), ("nodeMask", "<u4")])
+O_LIGHTENV = np.dtype([("raw", "u1", 372)])
+assert O_FRUSTUM.itemsize == 400
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.or_tile_traverse.restype = ctypes.c_int64
+        L.or_build_tile_lists.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
+
+
+def layout_sizes():
+    out = np.zeros(10, np.uint32)
+    lib().or_layout_sizes(_p(out))
+    return out
+
+
+def frustum_setup(frusta, m):
+    from hvvr_amd import DTYPES
+    frusta = np.ascontiguousarray(frusta)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    world = np.zeros(len(frusta), DTYPES["sfrustum"])
+    pre = np.zeros(len(frusta), O_FRUSTUM)
+    lib().or_frustum_setup(_p(frusta), c_uint32(len(frusta)), _p(m), _p(world), _p(pre))
+    return world, pre
+
+
+def test_children(pre_one, node_one):
+    return lib().or_test_children(_p(pre_one), _p(node_one))
+
+
+def test_children_refine(pre_one, node_one, prev):
+    return lib().or_test_children_refine(_p(pre_one), _p(node_one), c_uint32(prev))
+
+
+def block_traverse(nodes, pre_one):
+    out = np.zeros(64, O_STACK)
+    n = lib().or_block_traverse(_p(nodes), _p(pre_one), _p(out))
+    return out[:n]
+
+
+def tile_traverse(nodes, stack, pre_one, cap=1 << 22):
+    out = np.zeros(cap, np.uint32)
+    stack = np.ascontiguousarray(stack, O_STACK)
+    n = lib().or_tile_traverse(_p(nodes), _p(stack), c_uint32(len(stack)), _p(pre_one), _p(out), c_uint64(cap))
+    if n < 0:
+        raise RuntimeError("oracle tile traversal overflowed")
+    return out[:n]
+
+
+def build_tile_lists(nodes, block_pre, tile_pre):
+    nodes = np.ascontiguousarray(nodes)
+    counts = np.zeros(len(tile_pre), np.uint32)
+    total = lib().or_build_tile_lists(_p(nodes), _p(block_pre), _p(tile_pre), c_uint32(len(block_pre)),
+                                      _p(counts), None, c_uint64(0))
+    if total < 0:
+        raise RuntimeError("oracle traversal overflowed")
+    lists = np.zeros(max(total, 1), np.uint32)
+    total2 = lib().or_build_tile_lists(_p(nodes), _p(block_pre), _p(tile_pre), c_uint32(len(block_pre)),
+                                       _p(counts), _p(lists), c_uint64(len(lists)))
+    assert total2 == total
+    return counts, lists[:total]
+
+
+def intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris):
+    beams = np.ascontiguousarray(beams)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    offsets = np.ascontiguousarray(offsets, np.float32)
+    tri_out = np.zeros(len(beams) * A, np.uint32)
+    depth = np.zeros(len(beams) * A, np.float32)
+    surv = np.zeros(1, np.uint64)
+    lists = np.ascontiguousarray(lists, np.uint32)
+    rc = lib().or_intersect_view(_p(beams), c_uint32(len(beams)), _p(world_tiles), c_uint32(len(world_tiles)), _p(m),
+                                 _p(offsets), c_uint32(A), _p(np.ascontiguousarray(counts, np.uint32)),
+                                 _p(lists if lists.size else np.zeros(1, np.uint32)), _p(tris), _p(tri_out),
+                                 _p(depth), _p(surv))
+    if rc:
+        raise RuntimeError("oracle intersect failed")
+    return tri_out.reshape(-1, A), depth.reshape(-1, A), int(surv[0])
+
+
+def emit_gbuffer(sample_tris):
+    A = len(sample_tris)
+    st = np.ascontiguousarray(sample_tris, np.uint32)
+    t = np.zeros(A, np.uint32)
+    m = np.zeros(A, np.uint32)
+    n = lib().or_emit_gbuffer(_p(st), c_uint32(A), _p(t), _p(m))
+    return t[:n], m[:n]
+
+
+def gbuffer_slots(tri_ids):
+    """Logical GBuffer of every sample: (tri[s, A], mask[s, A]) padded with (0xffffffff, 0)."""
+    S, A = tri_ids.shape
+    tri = np.full((S, A), 0xFFFFFFFF, np.uint32)
+    mask = np.zeros((S, A), np.uint32)
+    for s in range(S):
+        t, m = emit_gbuffer(tri_ids[s])
+        tri[s, :len(t)] = t
+        mask[s, :len(m)] = m
+    return tri, mask
+
+
+def lighting_env(lights):
+    env = np.zeros(1, O_LIGHTENV)
+    lights = np.ascontiguousarray(lights)
+    dropped = lib().or_lighting_env(_p(lights), c_uint32(len(lights)), _p(env))
+    return env, dropped
+
+
+def resolve_view(beams, m, offsets, A, tri_ids, tris, shade, verts, materials, env, precision=0):
+    beams = np.ascontiguousarray(beams)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    offsets = np.ascontiguousarray(offsets, np.float32)
+    colors = np.zeros(len(beams), np.uint32)
+    linear = np.zeros((len(beams), 3), np.float32)
+    tri_ids = np.ascontiguousarray(tri_ids, np.uint32)
+    rc = lib().or_resolve_view(_p(beams), c_uint32(len(beams)), _p(m), _p(offsets), c_uint32(A), _p(tri_ids),
+                               _p(tris), _p(shade), _p(verts), _p(materials), _p(env), c_int(precision), _p(colors),
+                               _p(linear))
+    if rc:
+        raise RuntimeError("oracle resolve failed")
+    return colors, linear
+
+
+def remap(results, remap_table, width, height, split=1):
+    img = np.zeros((height, width), np.uint32)
+    lib().or_remap(_p(np.ascontiguousarray(results, np.uint32)), _p(np.ascontiguousarray(remap_table, np.int32)),
+                   c_uint32(width), c_uint32(height), c_uint32(split), c_uint32(width), _p(img))
+    return img
+
+
+def refit(scene):
+    """World vertices, triangles and node bounds of a HostScene, from scratch."""
+    from hvvr_amd import DTYPES
+    verts = np.ascontiguousarray(scene.verts)
+    wv = np.zeros_like(verts)
+    tris = np.zeros(len(scene.triShade), DTYPES["tri_intersect"])
+    nodes = np.zeros_like(scene.nodes)
+    rc = lib().or_refit(_p(verts), c_uint32(len(verts)), _p(np.ascontiguousarray(scene.vertexToTransform)),
+                        _p(np.ascontiguousarray(scene.transforms, np.float32)), _p(scene.triShade),
+                        c_uint32(len(scene.triShade)), _p(scene.nodes), c_uint32(len(scene.nodes)), _p(wv), _p(tris),
+                        _p(nodes))
+    if rc:
+        raise RuntimeError("oracle refit failed")
+    return wv, tris, nodes
+
+
+def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=True):
+    """Full oracle frame: refit, frustum setup, traversal, intersect, resolve, remap."""
+    m = view.camera_to_world if camera_to_world is None else np.asarray(camera_to_world, np.float32)
+    wv, tris, nodes = refit(scene)
+    world_tiles, tile_pre = frustum_setup(view.tile_frusta, m)
+    _, block_pre = frustum_setup(view.block_frusta, m)
+    counts, lists = build_tile_lists(nodes, block_pre, tile_pre)
+    from hvvr_amd import subsample_offsets
+    offsets = subsample_offsets(A)
+    tri_ids, depth, surv = intersect_view(view.beams, world_tiles, m, offsets, A, counts, lists, tris)
+    out = dict(nodes=nodes, tris=tris, world_verts=wv, counts=counts, lists=lists, tri_ids=tri_ids, depth=depth,
+               survivors=surv, offsets=offsets)
+    if with_colors:
+        env, _ = lighting_env(scene.lights)
+        colors, linear = resolve_view(view.beams, m, offsets, A, tri_ids, tris, scene.triShade, wv, scene.materials,
+                                      env, precision)
+        # empty tiles: clear colour (ClearEmptyKernel, resolve.cu:527-542)
+        empty = np.repeat(counts == 0, 128)
+        colors[empty[: len(colors)]] = 0xFF000000
+        out["colors"] = colors
+        out["linear"] = linear
+        out["image"] = remap(colors, view.remap, view.width, view.height, view.split)
+    return out
