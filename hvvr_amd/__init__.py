@@ -349,6 +349,9 @@ class GpuView:
         j = np.asarray(jitter, np.float32)
         _check_gpu(self.lib.hvvr_gpu_view_render(self.handle, scene.handle, _ptr(m), _ptr(j)))
 
+    def set_traversal_stack_limit(self, entries):
+        _check_gpu(self.lib.hvvr_gpu_view_set_traversal_stack_limit(self.handle, c_uint32(entries)))
+
     def set_jitter_spin(self, enable):
         _check_gpu(self.lib.hvvr_gpu_view_set_jitter_spin(self.handle, c_int(1 if enable else 0)))
 

@@ -4,7 +4,7 @@
 // (raycaster/gpu_samples.h:23-61) with startOffset 0.5, evaluated in double and
 // rounded once so that every consumer of a table sees the same bits.
 
-#include "hvvr_types.h"
+#include "hvvr/types.h"
 
 namespace hvvr {
 

@@ -378,6 +378,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->blockStackSize.resize(v->blockCount));
     step(v->tileStackScratch.resize(size_t(TILE_STACK_SIZE) * v->tileCount));
     step(v->tileCounts.resize(v->tileCount));
+    step(v->overflowTiles.resize(v->tileCount));
     step(v->tileOffsets.resize(v->tileCount + 1));
     step(v->scanPartials.resize((v->tileCount + 1023) / 1024 + 1));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
@@ -406,6 +407,15 @@ int hvvr_gpu_view_destroy(hvvr_gpu_view* v) {
     return HVVR_GPU_OK;
 }
 
+int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* v, uint32_t entries) {
+    if (!v)
+        return fail("null view");
+    if (entries < 1 || entries > 512)
+        return fail("traversal stack limit must be in [1, 512]");
+    v->waveStackLimit = entries;
+    return HVVR_GPU_OK;
+}
+
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
     if (!v)
         return fail("null view");
@@ -429,6 +439,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
             (void)hipEventRecord(c->stageEvents[i], st);
     };
     HVVR_HIP_CHECK(hipMemsetAsync(v->counters.ptr, 0, v->counters.bytes(), st));
+    HVVR_HIP_CHECK(hipMemsetAsync(v->overflowTiles.ptr, 0, v->overflowTiles.bytes(), st));
     mark(0);
     launchFrustumSetup(v, m, st);
     mark(1);

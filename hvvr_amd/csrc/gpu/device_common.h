@@ -7,7 +7,7 @@
 // exactly where the reference uses _mm256_fmadd_ps, plain float ops elsewhere (the
 // whole library builds with -ffp-contract=off).
 
-#include "../common/hvvr_types.h"
+#include "hvvr/types.h"
 
 #include <float.h>
 

@@ -2,7 +2,7 @@
 // Internal state behind the C ABI (include/hvvr_gpu.h) and the kernel launchers.
 
 #include "../../../include/hvvr_gpu.h"
-#include "../common/hvvr_types.h"
+#include "hvvr/types.h"
 
 #include <hip/hip_runtime.h>
 
@@ -98,6 +98,7 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
     hvvr::DeviceBuffer<hvvr::StackEntry> tileStackScratch; // [TILE_STACK_SIZE][tileCount], slot-major
+    hvvr::DeviceBuffer<uint32_t> overflowTiles;         // [tileCount] 1 = LDS stack overflowed
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount]
     hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount + 1] exclusive scan, [tileCount] = total
     hvvr::DeviceBuffer<uint32_t> scanPartials;
@@ -108,6 +109,7 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> image;                 // [width*height]
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack
     uint64_t listCapacity = 0;
+    uint32_t waveStackLimit = 512; // LDS stack entries per tile traversal (<= WAVE_STACK_SIZE)
     float stageMs[hvvr::kStageCount] = {};
     bool rendered = false;
 };

@@ -3,23 +3,22 @@
 // 99-330, raycaster/traverse_avx.h:293-527); here they are kernels whose output, the
 // per-tile triangle lists, never leaves HBM.
 //
-//   frustumSetupKernel   one thread per tile or block frustum: camera->world
-//                        transform (traversal.h:38-47) + precompute (traverse_avx.cpp)
-//   blockTraverseKernel  one thread per block (<= 64-entry cull stack, then the
-//                        distance sort of BlockFrame::sort)
-//   tileTraverseKernel   one thread per tile: depth-first traversal from the block's
-//                        sorted stack; the 64 tiles of a block form one wave64, so the
-//                        lanes of a wave start from the same stack. Run twice: count
-//                        pass, then (after an exclusive scan of the counts) emit pass,
-//                        so every tile's list lands contiguously in tile order with no
-//                        host round trip and no atomics.
-//   scan kernels         3-phase exclusive scan of the per-tile counts.
-//
-// Tile stacks live in a slot-major scratch array ([slot][tile]): lanes of a wave that
-// sit at the same depth touch consecutive words.
+//   frustumSetupKernel        one thread per tile or block frustum: camera->world
+//                             transform (traversal.h:38-47) + precompute (traverse_avx.cpp)
+//   blockTraverseWaveKernel   one wave64 per block: cooperative node tests, the <= 64-entry
+//                             cull stack in LDS, then the distance sort of
+//                             BlockFrame::sort as a parallel rank computation
+//   tileTraverseWaveKernel    one wave64 per tile (wave_traverse.h). Run twice: count pass,
+//                             then (after an exclusive scan of the counts) emit pass, so
+//                             every tile's list lands contiguously in tile order with no
+//                             host round trip and no atomics.
+//   tileTraverseKernel        one thread per tile with the full 4096-entry stack in global
+//                             memory; only re-runs tiles whose LDS stack overflowed.
+//   scan kernels              3-phase exclusive scan of the per-tile counts.
 
 #include "device_common.h"
 #include "gpu_internal.h"
+#include "wave_traverse.h"
 
 namespace hvvr {
 
@@ -146,6 +145,140 @@ __global__ __launch_bounds__(64) void blockTraverseKernel(const BVHNode* __restr
     stackSize[b] = top;
 }
 
+// traverseBlocks + BlockFrame::sort, one wave64 per block
+__global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __restrict__ nodes,
+                                                              const FrustumPre* __restrict__ blockPre,
+                                                              uint32_t blockCount, StackEntry* __restrict__ stacks,
+                                                              uint32_t* __restrict__ stackSize) {
+    __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
+    const uint32_t b = blockIdx.x;
+    const uint32_t lane = threadIdx.x, c = lane & 3u, role = lane >> 2;
+    WaveFrustum w;
+    loadWaveFrustum(blockPre[b], role, w);
+    if (w.degenerate) {
+        if (lane == 0)
+            stackSize[b] = 0;
+        return;
+    }
+    const int axis = w.axis, sizeAxis = w.axis & ~1;
+    uint32_t top = 0;
+    float tMin = 0.0f;
+    uint32_t node = 0;
+    // insert e so that it lands below the run of top entries whose (DAZ) key exceeds `key`
+    auto insert = [&](float key, const CullEntry& e) {
+        int j = int(top) - 1 - int(lane);
+        bool larger = j >= 0 && daz(cull[j >= 0 ? j : 0].bits) > key;
+        uint64_t run = ~__ballot(larger);
+        uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
+        CullEntry moved;
+        if (lane < n)
+            moved = cull[j];
+        waveLdsSync();
+        if (lane < n)
+            cull[j + 1] = moved;
+        waveLdsSync();
+        if (lane == 0)
+            cull[top - n] = e;
+        waveLdsSync();
+        top++;
+    };
+    for (;;) {
+        NodeLanes nl;
+        loadNodeLanes(nodes, node, c, nl);
+        const uint32_t leafMask = nodes[node].leafMask;
+        uint32_t mask = waveTestChildren(w, nl, role);
+        mask = waveTestRefine(w, nl, role, mask);
+        if (mask) {
+            uint32_t leaf = mask & leafMask;
+            if (leaf) {
+                mask &= ~leaf;
+                CullEntry e;
+                e.bits = leaf;
+                e.tMinTemp = tMin;
+                e.node = node;
+                insert(0.0f, e);
+            }
+            for (int k = 0; k < 4; k++) {
+                if (!(mask & (1u << k)))
+                    continue;
+                float size = ftz(nodes[node].bounds[sizeAxis][k] + nodes[node].bounds[sizeAxis + 1][k]);
+                CullEntry e;
+                e.bits = fbits(size);
+                e.tMinTemp = -nodes[node].bounds[axis][k];
+                e.node = node + nodes[node].data[1 + k];
+                insert(daz(e.bits), e);
+            }
+        }
+        if (top >= BLOCK_STACK_SIZE - 4 || top == 0 || cull[top - 1].bits <= 0xfu)
+            break;
+        top--;
+        node = uniform(cull[top].node);
+        tMin = cull[top].tMinTemp;
+    }
+    // BlockFrame::sort as ranks: entry t ends above every entry with a larger distance
+    // and above earlier entries with an equal one
+    if (lane < top) {
+        CullEntry me = cull[lane];
+        uint32_t rank = 0;
+        for (uint32_t s = 0; s < top; s++) {
+            float o = cull[s].tMinTemp;
+            rank += (o > me.tMinTemp || (s < lane && o == me.tMinTemp)) ? 1u : 0u;
+        }
+        StackEntry out;
+        out.tMin = me.tMinTemp;
+        uint32_t lm = me.bits <= 0xfu ? me.bits : 0u;
+        out.nodeMask = (me.node & STACK_NODE_MASK) | (lm << 28);
+        stacks[size_t(b) * BLOCK_STACK_SIZE + rank] = out;
+    }
+    if (lane == 0)
+        stackSize[b] = top;
+}
+
+// traverseTiles, one wave64 per tile (wave_traverse.h)
+__global__ __launch_bounds__(64) void tileTraverseWaveKernel(const BVHNode* __restrict__ nodes,
+                                                             const FrustumPre* __restrict__ tilePre,
+                                                             const StackEntry* __restrict__ blockStacks,
+                                                             const uint32_t* __restrict__ blockStackSize,
+                                                             uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             uint32_t* __restrict__ triOut, uint64_t capacity,
+                                                             uint32_t* __restrict__ overflowTiles,
+                                                             unsigned long long* __restrict__ counters,
+                                                             uint32_t stackLimit, int emit) {
+    __shared__ StackEntry stack[WAVE_STACK_SIZE];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t block = tile / TILES_PER_BLOCK;
+    const uint64_t outBase = emit ? offsets[tile] : 0;
+    uint32_t count = 0;
+    auto sink = [&](uint32_t t0, uint32_t t1) {
+        uint32_t len = t1 - t0;
+        if (emit) {
+            for (uint32_t i = lane; i < len; i += 64) {
+                uint64_t at = outBase + count + i;
+                if (at < capacity)
+                    triOut[at] = t0 + i;
+            }
+        }
+        count += len;
+    };
+    uint32_t depth = 0;
+    bool ok = waveTraverseTile(nodes, tilePre[tile], blockStacks + size_t(block) * BLOCK_STACK_SIZE,
+                               blockStackSize[block], stack, stackLimit, lane, sink, &depth);
+    if (lane == 0) {
+        if (!ok) {
+            overflowTiles[tile] = 1u; // re-run by the global-stack kernel
+            count = 0;
+        } else {
+            if (emit && outBase + count > capacity)
+                atomicOr(&counters[1], 1ull);
+            atomicMax(&counters[2], (unsigned long long)depth);
+        }
+        if (!emit)
+            counts[tile] = count;
+    }
+}
+
 // traverseTiles (traverse_avx.h:412-527)
 __global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restrict__ nodes,
                                                          const FrustumPre* __restrict__ tilePre, uint32_t tileCount,
@@ -154,9 +287,12 @@ __global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restri
                                                          StackEntry* __restrict__ scratch, uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ offsets,
                                                          uint32_t* __restrict__ triOut, uint64_t capacity,
+                                                         const uint32_t* __restrict__ onlyTiles,
                                                          unsigned long long* __restrict__ counters, int emit) {
     uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
     if (tile >= tileCount)
+        return;
+    if (onlyTiles && !onlyTiles[tile])
         return;
     uint32_t block = tile / TILES_PER_BLOCK;
     uint32_t n0 = blockStackSize[block];
@@ -332,15 +468,21 @@ void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream
 }
 
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    blockTraverseKernel<<<(v->blockCount + 63) / 64, 64, 0, stream>>>(s->nodes.ptr, v->blockPre.ptr, v->blockCount,
-                                                                      v->blockStacks.ptr, v->blockStackSize.ptr);
+    blockTraverseWaveKernel<<<v->blockCount, 64, 0, stream>>>(s->nodes.ptr, v->blockPre.ptr, v->blockCount,
+                                                              v->blockStacks.ptr, v->blockStackSize.ptr);
 }
 
 void launchTileTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, bool emit, hipStream_t stream) {
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    tileTraverseWaveKernel<<<v->tileCount, 64, 0, stream>>>(
+        s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr, v->blockStackSize.ptr, v->tileCounts.ptr,
+        v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity, v->overflowTiles.ptr, counters, v->waveStackLimit,
+        emit ? 1 : 0);
+    // tiles whose stack outgrew LDS: full-depth global-memory traversal, same results
     tileTraverseKernel<<<(v->tileCount + 63) / 64, 64, 0, stream>>>(
         s->nodes.ptr, v->tilePre.ptr, v->tileCount, v->blockStacks.ptr, v->blockStackSize.ptr,
         v->tileStackScratch.ptr, v->tileCounts.ptr, v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity,
-        reinterpret_cast<unsigned long long*>(v->counters.ptr), emit ? 1 : 0);
+        v->overflowTiles.ptr, counters, emit ? 1 : 0);
 }
 
 void launchScan(hvvr_gpu_view* v, hipStream_t stream) {

@@ -379,4 +379,11 @@ bool GenerateTopology(MeshData& mesh, std::string* error) {
     return true;
 }
 
+bool GenerateTopology(MeshData& mesh) {
+    std::string err;
+    if (!GenerateTopology(mesh, &err))
+        throw RaycasterError(err);
+    return true;
+}
+
 } // namespace hvvr

@@ -3,7 +3,8 @@
 // BVH construction, scene flattening, sample hierarchy setup and the seeded scene
 // generators used by the benchmarks and tests.
 
-#include "../common/hvvr_types.h"
+#include "hvvr/raycaster.h"
+#include "hvvr/types.h"
 #include "../common/subsample.h"
 
 #include <stdint.h>
@@ -12,17 +13,9 @@
 
 namespace hvvr {
 
-// raycaster/model.h:26-31
-struct MeshData {
-    std::vector<ShadingVertex> verts;
-    std::vector<PrecomputedTriangleShade> triShade;
-    std::vector<TopologyNode> nodes;
-    std::vector<SimpleMaterial> materials;
-};
-
 // SAH BVH2 -> BVH4 -> depth-first topology; reorders triangles and vertices
 // (shared/bvh.cpp:542-556). Returns false (with a message) on malformed meshes.
-bool GenerateTopology(MeshData& mesh, std::string* error = nullptr);
+bool GenerateTopology(MeshData& mesh, std::string* error);
 
 // One model of a scene: mesh + model-to-world transform.
 struct SceneModel {
@@ -45,17 +38,8 @@ void buildScene(const std::vector<SceneModel>& models, FlatScene& out);
 // ---------------------------------------------------------------------------------
 // samples (raycaster/samples.h:17-67)
 // ---------------------------------------------------------------------------------
-struct Sample {
-    vector2ui pixelLocation;
-    vector2 position;
-    struct Extents {
-        vector2 minorAxis;
-        float majorAxisLength;
-    } extents;
-};
 static_assert(sizeof(Sample) == 28, "Sample layout");
 
-std::vector<Sample> getGridSamples(uint32_t width, uint32_t height);
 // Radial barrel distortion r' = r (1 + k1 r^2 + k2 r^4) applied to sample positions in
 // centred sample space, extents scaled by the local radial magnification.
 void applyBarrelDistortion(std::vector<Sample>& samples, float k1, float k2);

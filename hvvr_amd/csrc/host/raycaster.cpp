@@ -1,0 +1,352 @@
+// hvvr::Raycaster / Camera / Model / Light over the GPU C ABI (include/hvvr_gpu.h).
+//
+// Object model and call sequence follow the reference's public API:
+//   Raycaster ctor/dtor/reinit          raycaster/raycaster.cpp:30-76
+//   createCamera/destroyCamera          raycaster/raycaster.cpp:78-91
+//   createLight/createModel/...         raycaster/scene.cpp:43-80
+//   render(elapsedTime)                 raycaster/render.cpp:204-218, :382-395
+//   updateScene/uploadScene             raycaster/scene_update.cpp:145-174
+//   Camera setters, setSamples, jitter  raycaster/camera.cpp:70-209
+//   Model::setTransform                 raycaster/model.cpp:20-30
+//
+// Differences, all on the host side: a transform-only change re-uploads the model
+// matrices and refits on the GPU instead of rebuilding the flattened scene; lights are
+// re-uploaded whenever one is created or destroyed (the reference only uploads them with
+// the geometry); the enabled cameras of one render() go to the GPU as one batched
+// submission; a camera with a CPU render target gets its image copied back at the end of
+// render() (the reference copies only to 3D-API targets, render.cpp:259-268).
+
+#include "host_internal.h"
+
+#include "hvvr_gpu.h"
+
+#include <random>
+#include <string.h>
+
+namespace hvvr {
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc != HVVR_GPU_OK)
+        throw RaycasterError(std::string(what) + ": " + hvvr_gpu_last_error());
+}
+
+bool validMsaaRate(uint32_t a) {
+    return a == 1 || a == 2 || a == 4 || a == 8 || a == 16 || a == 32;
+}
+
+template <class T>
+void eraseOwned(std::vector<std::unique_ptr<T>>& v, T* p, const char* what) {
+    for (auto it = v.begin(); it != v.end(); ++it) {
+        if (it->get() == p) {
+            v.erase(it);
+            return;
+        }
+    }
+    throw RaycasterError(std::string(what) + ": object does not belong to this raycaster");
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------------
+Model::Model(Raycaster& raycaster, MeshData&& meshData)
+    : _raycaster(raycaster), _meshData(std::move(meshData)), _transform(transform::identity()) {}
+
+void Model::setTransform(const transform& t) {
+    _transform = t;
+    _raycaster.transformsChanged();
+}
+
+// ---------------------------------------------------------------------------------
+// Camera
+// ---------------------------------------------------------------------------------
+Camera::Camera(Raycaster& raycaster, const FloatRect& viewport, float apertureRadius)
+    : _raycaster(raycaster), _viewport(viewport), _lens{apertureRadius, 1.0f} {
+    // 16 frame jitters in [0,1)^2 from a default-seeded mt19937 (camera.cpp:74-79)
+    std::uniform_real_distribution<float> uniform(0.0f, 1.0f);
+    std::mt19937 generator;
+    for (int i = 0; i < 16; ++i) {
+        float x = uniform(generator);
+        float y = uniform(generator);
+        _frameJitters.push_back(vector2(x, y));
+    }
+}
+
+Camera::~Camera() {
+    if (_view)
+        hvvr_gpu_view_destroy(_view);
+}
+
+void Camera::setViewport(const FloatRect& viewport) {
+    _viewport = viewport;
+    _viewDirty = true;
+}
+
+void Camera::setLens(const ThinLens& lens) {
+    _lens = lens;
+    _viewDirty = true;
+}
+
+void Camera::setAperture(float radius) {
+    _lens.radius = radius;
+    _viewDirty = true;
+}
+
+void Camera::setFocalDepth(float depth) {
+    _lens.focalDistance = depth;
+    _viewDirty = true;
+}
+
+void Camera::setRenderTarget(const ImageResourceDescriptor& target) {
+    if (target.memoryType != ImageResourceDescriptor::MemoryType::CPU_MEMORY)
+        throw RaycasterError("Camera::setRenderTarget: only CPU_MEMORY render targets are supported");
+    if (target.stride < target.width)
+        throw RaycasterError("Camera::setRenderTarget: stride (in pixels) is smaller than the width");
+    bool resized = target.width != _renderTarget.width || target.height != _renderTarget.height;
+    _renderTarget = target;
+    if (resized)
+        _viewDirty = true;
+}
+
+void Camera::setSamples(const Sample* samples, uint32_t count, uint32_t splitColorSamples) {
+    if (!samples && count)
+        throw RaycasterError("Camera::setSamples: null samples");
+    if (splitColorSamples < 1 || splitColorSamples > 3)
+        throw RaycasterError("Camera::setSamples: splitColorSamples must be 1, 2 or 3");
+    _samples.assign(samples, samples + count);
+    _split = splitColorSamples;
+    _viewDirty = true;
+    // the reference builds the sample hierarchy here (camera.cpp:135-156)
+    rebuildView();
+}
+
+matrix3x3 Camera::getSampleToCamera() const {
+    return sampleToCamera(_viewport.lower, _viewport.upper);
+}
+
+void Camera::rebuildView() {
+    if (_view) {
+        hvvr_gpu_view_destroy(_view);
+        _view = nullptr;
+    }
+    _validSampleCount = 0;
+    if (_samples.empty() || _renderTarget.width == 0 || _renderTarget.height == 0) {
+        _viewDirty = true; // retried on the next render once samples and a target exist
+        return;
+    }
+    for (const Sample& s : _samples)
+        if (s.pixelLocation.x >= _renderTarget.width || s.pixelLocation.y >= _renderTarget.height)
+            throw RaycasterError("Camera::setSamples: sample pixel location outside the render target");
+
+    SampleSetup setup;
+    buildSampleSetup(_samples.data(), uint32_t(_samples.size()), _split, _renderTarget.width,
+                     _renderTarget.height, getSampleToCamera(), _lens, setup);
+
+    hvvr_gpu_view_desc d = {};
+    d.msaaRate = _raycaster._spec.msaaRate;
+    d.sampleCount = setup.sampleCount;
+    d.validSampleCount = setup.validSampleCount;
+    d.beams = setup.beams.data();
+    d.tileFrusta = setup.tileFrusta.data();
+    d.blockFrusta = setup.blockFrusta.data();
+    d.remap = setup.remap.data();
+    d.width = setup.width;
+    d.height = setup.height;
+    d.split = setup.splitColorSamples;
+    d.lensRadius = _lens.radius;
+    d.focalDistance = _lens.focalDistance;
+    check(hvvr_gpu_view_create(_raycaster._ctx, &d, &_view), "Camera: view creation failed");
+    _validSampleCount = setup.validSampleCount;
+    _viewDirty = false;
+}
+
+void Camera::extractImage() {
+    if (!_view || !_renderTarget.data)
+        return;
+    check(hvvr_gpu_view_read_image(_view, (uint32_t*)_renderTarget.data, uint32_t(_renderTarget.stride)),
+          "Camera::extractImage");
+}
+
+vector2 Camera::nextJitter() {
+    // camera.cpp:205-209
+    vector2 j = _frameJitters[_frameCount % _frameJitters.size()];
+    ++_frameCount;
+    return vector2(j.x * 0.5f + 0.5f, j.y * 0.5f + 0.5f);
+}
+
+// ---------------------------------------------------------------------------------
+// Raycaster
+// ---------------------------------------------------------------------------------
+Raycaster::Raycaster(const RayCasterSpecification& spec) : _spec(spec) {
+    if (_spec.mode != RayCasterGPUMode::GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE)
+        throw RaycasterError("Raycaster: only GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE is implemented");
+    if (_spec.outputTo3DApi)
+        throw RaycasterError("Raycaster: 3D-API output (D3D11/GL interop) is not supported; use CPU render targets");
+    if (!validMsaaRate(_spec.msaaRate))
+        throw RaycasterError("Raycaster: msaaRate must be 1, 2, 4, 8, 16 or 32");
+    check(hvvr_gpu_context_create(_spec.device, &_ctx), "Raycaster: GPU context creation failed");
+}
+
+Raycaster::~Raycaster() {
+    _cameras.clear(); // views before the context
+    if (_scene)
+        hvvr_gpu_scene_destroy(_scene);
+    if (_ctx)
+        hvvr_gpu_context_destroy(_ctx);
+}
+
+void Raycaster::reinit(RayCasterGPUMode mode) {
+    // raycaster.cpp:57-76: drops cameras and the GPU scene, keeps models and lights
+    if (mode != RayCasterGPUMode::GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE)
+        throw RaycasterError("Raycaster::reinit: only GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE is implemented");
+    if (_ctx)
+        hvvr_gpu_context_synchronize(_ctx);
+    _cameras.clear();
+    if (_scene) {
+        hvvr_gpu_scene_destroy(_scene);
+        _scene = nullptr;
+    }
+    if (_ctx) {
+        hvvr_gpu_context_destroy(_ctx);
+        _ctx = nullptr;
+    }
+    _spec.mode = mode;
+    check(hvvr_gpu_context_create(_spec.device, &_ctx), "Raycaster::reinit: GPU context creation failed");
+    _sceneDirty = true;
+    _lightsDirty = true;
+}
+
+Camera* Raycaster::createCamera(const FloatRect& viewport, float apertureRadius) {
+    _cameras.emplace_back(new Camera(*this, viewport, apertureRadius));
+    return _cameras.back().get();
+}
+
+void Raycaster::destroyCamera(Camera* camera) {
+    eraseOwned(_cameras, camera, "Raycaster::destroyCamera");
+}
+
+Light* Raycaster::createLight(const LightUnion& lightUnion) {
+    _lights.emplace_back(new Light(lightUnion));
+    _lightsDirty = true;
+    return _lights.back().get();
+}
+
+void Raycaster::destroyLight(Light* light) {
+    eraseOwned(_lights, light, "Raycaster::destroyLight");
+    _lightsDirty = true;
+}
+
+Model* Raycaster::createModel(MeshData&& meshData) {
+    if (meshData.nodes.empty())
+        throw RaycasterError("Raycaster::createModel: mesh has no BVH (call GenerateTopology first)");
+    _models.emplace_back(new Model(*this, std::move(meshData)));
+    _sceneDirty = true;
+    return _models.back().get();
+}
+
+Model* Raycaster::createModel(const MeshData& meshData) {
+    return createModel(MeshData(meshData));
+}
+
+void Raycaster::destroyModel(Model* model) {
+    eraseOwned(_models, model, "Raycaster::destroyModel");
+    _sceneDirty = true;
+}
+
+void Raycaster::setSceneDirty() {
+    _sceneDirty = true;
+}
+
+void Raycaster::updateScene() {
+    if (_sceneDirty) {
+        if (_scene) {
+            hvvr_gpu_scene_destroy(_scene);
+            _scene = nullptr;
+        }
+        if (!_models.empty()) {
+            std::vector<SceneModel> models;
+            models.reserve(_models.size());
+            for (const auto& m : _models)
+                models.push_back({&m->getMesh(), matrix4x4(m->getTransform())});
+            FlatScene flat;
+            buildScene(models, flat);
+            if (flat.triShade.empty())
+                throw RaycasterError("Raycaster: the scene's models hold no triangles");
+            hvvr_gpu_scene_desc d = {};
+            d.vertices = flat.verts.data();
+            d.vertexToTransform = flat.vertexToTransform.data();
+            d.vertexCount = uint32_t(flat.verts.size());
+            d.triangles = flat.triShade.data();
+            d.triangleCount = uint32_t(flat.triShade.size());
+            d.nodes = flat.nodes.data();
+            d.nodeCount = uint32_t(flat.nodes.size());
+            d.materials = flat.materials.data();
+            d.materialCount = uint32_t(flat.materials.size());
+            d.transforms = (const float*)flat.transforms.data();
+            d.transformCount = uint32_t(flat.transforms.size());
+            check(hvvr_gpu_scene_create(_ctx, &d, &_scene), "Raycaster: scene upload failed");
+        }
+        _sceneDirty = false;
+        _transformsDirty = false;
+        _lightsDirty = true;
+    }
+    if (!_scene)
+        return;
+    if (_transformsDirty) {
+        std::vector<matrix4x4> xf;
+        xf.reserve(_models.size());
+        for (const auto& m : _models)
+            xf.push_back(matrix4x4(m->getTransform()));
+        check(hvvr_gpu_scene_set_transforms(_scene, (const float*)xf.data(), uint32_t(xf.size())),
+              "Raycaster: transform upload failed");
+        _transformsDirty = false;
+    }
+    if (_lightsDirty) {
+        std::vector<LightUnion> lights;
+        lights.reserve(_lights.size());
+        for (const auto& l : _lights)
+            lights.push_back(l->getLightUnion());
+        check(hvvr_gpu_scene_set_lights(_scene, lights.data(), uint32_t(lights.size()), nullptr),
+              "Raycaster: light upload failed");
+        _lightsDirty = false;
+    }
+    check(hvvr_gpu_scene_update(_scene), "Raycaster: scene update failed");
+}
+
+void Raycaster::render(double elapsedTime) {
+    (void)elapsedTime;
+    updateScene();
+    if (!_scene)
+        return; // no scene geometry is loaded (render.cpp:205-207)
+
+    std::vector<Camera*> active;
+    for (auto& c : _cameras) {
+        if (!c->getEnabled())
+            continue;
+        if (c->_viewDirty)
+            c->rebuildView();
+        if (c->_view)
+            active.push_back(c.get());
+    }
+    if (active.empty())
+        return;
+
+    std::vector<hvvr_gpu_view*> views;
+    std::vector<matrix4x4> cameraToWorld;
+    std::vector<vector2> jitter;
+    for (Camera* c : active) {
+        views.push_back(c->_view);
+        cameraToWorld.push_back(c->getCameraToWorld());
+        jitter.push_back(c->nextJitter());
+    }
+    check(hvvr_gpu_render_views(_scene, uint32_t(views.size()), views.data(), (const float*)cameraToWorld.data(),
+                                (const float*)jitter.data()),
+          "Raycaster::render");
+    for (Camera* c : active)
+        c->extractImage();
+}
+
+} // namespace hvvr

@@ -17,10 +17,10 @@
 
 namespace hvvr {
 
-std::vector<Sample> getGridSamples(uint32_t width, uint32_t height) {
+std::vector<Sample> getGridSamples(size_t width, size_t height) {
     float invW = 1.0f / float(width);
     float invH = 1.0f / float(height);
-    std::vector<Sample> s(size_t(width) * height);
+    std::vector<Sample> s(width * height);
     for (uint32_t y = 0; y < height; y++) {
         for (uint32_t x = 0; x < width; x++) {
             Sample& o = s[size_t(y) * width + x];

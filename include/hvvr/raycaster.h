@@ -1,0 +1,206 @@
+#pragma once
+// hvvr::Raycaster C++ API for the MI355X render path: the same object model and entry
+// points as the reference's public API (raycaster/raycaster.h:38-115, camera.h:51-146,
+// model.h:26-57, light.h:19-77), implemented on the GPU C ABI of hvvr_gpu.h.
+//
+// Differences a caller sees:
+//  * RayCasterSpecification::msaaRate selects the MSAA rate at run time (the reference
+//    compiles it in as COLOR_MODE_MSAA_RATE = 16, raycaster_common.h:27);
+//  * errors throw hvvr::RaycasterError instead of printing and exiting;
+//  * render targets are CPU memory (ImageResourceDescriptor::MemoryType::CPU_MEMORY);
+//    D3D11/GL interop, textures and the foveated mode are outside this build.
+
+#include "types.h"
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct hvvr_gpu_context;
+struct hvvr_gpu_scene;
+struct hvvr_gpu_view;
+
+namespace hvvr {
+
+class RaycasterError : public std::runtime_error {
+public:
+    explicit RaycasterError(const std::string& what) : std::runtime_error(what) {}
+};
+
+// graphics_types.h:25-31
+struct FloatRect {
+    vector2 lower, upper;
+    FloatRect() = default;
+    FloatRect(const vector2& l, const vector2& u) : lower(l), upper(u) {}
+};
+
+// graphics_types.h:37-81 (CPU memory only)
+struct ImageResourceDescriptor {
+    enum class MemoryType { CPU_MEMORY, CUDA_MEMORY, DX_TEXTURE, OPENGL_TEXTURE };
+    MemoryType memoryType = MemoryType::CPU_MEMORY;
+    void* data = nullptr;
+    uint32_t width = 0;
+    uint32_t height = 0;
+    size_t stride = 0; // in pixels
+};
+
+// model.h:26-31
+struct MeshData {
+    std::vector<ShadingVertex> verts;
+    std::vector<PrecomputedTriangleShade> triShade;
+    std::vector<TopologyNode> nodes;
+    std::vector<SimpleMaterial> materials;
+};
+
+// samples.h:17-29
+struct Sample {
+    vector2ui pixelLocation;
+    vector2 position;
+    struct Extents {
+        vector2 minorAxis;
+        float majorAxisLength;
+    } extents;
+};
+
+// raycaster_spec.h:22-50
+enum class RaycasterOutputFormat { COLOR_RGBA8 };
+enum class RayCasterGPUMode { GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE, GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT };
+struct RayCasterSpecification {
+    RayCasterGPUMode mode = RayCasterGPUMode::GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE;
+    size_t threadCount = 0;
+    struct FoveatedSamplePattern {
+        float degreeTrackingError = 0.5f;
+        float maxFOVDegrees = 110.0f;
+        float marSlope = 0.022f;
+        float fovealMARDegrees = 1.0f / 60.0f;
+    } foveatedSamplePattern;
+    RaycasterOutputFormat outputFormat = RaycasterOutputFormat::COLOR_RGBA8;
+    bool outputTo3DApi = false;
+    uint32_t msaaRate = 16; // 1, 2, 4, 8, 16 or 32
+    int device = 0;
+};
+
+// SAH BVH4 topology for a mesh, reordering triangles and vertices (shared/bvh.h:15).
+bool GenerateTopology(MeshData& mesh);
+// Grid samples of a width x height render target (samples.cpp:21-42).
+std::vector<Sample> getGridSamples(size_t width, size_t height);
+
+class Raycaster;
+
+class Light {
+public:
+    explicit Light(const LightUnion& u) : _lightUnion(u) {}
+    const LightUnion& getLightUnion() const { return _lightUnion; }
+
+protected:
+    LightUnion _lightUnion;
+};
+
+class Model {
+public:
+    Model(Raycaster& raycaster, MeshData&& meshData);
+    Model(const Model&) = delete;
+    Model& operator=(const Model&) = delete;
+    const MeshData& getMesh() const { return _meshData; }
+    void setTransform(const transform& t);
+    const transform& getTransform() const { return _transform; }
+
+protected:
+    Raycaster& _raycaster;
+    MeshData _meshData;
+    transform _transform;
+};
+
+class Camera {
+    friend class Raycaster;
+
+public:
+    Camera(Raycaster& raycaster, const FloatRect& viewport, float apertureRadius);
+    ~Camera();
+    Camera(const Camera&) = delete;
+    Camera& operator=(const Camera&) = delete;
+
+    void setEnabled(bool enabled) { _enabled = enabled; }
+    bool getEnabled() const { return _enabled; }
+    void setEyeDir(const vector3& eyeDir) { _eyeDir = eyeDir; }
+    const vector3& getEyeDir() const { return _eyeDir; }
+    void setViewport(const FloatRect& viewport);
+    const FloatRect& getViewport() const { return _viewport; }
+    void setLens(const ThinLens& lens);
+    const ThinLens& getLens() const { return _lens; }
+    void setAperture(float radius);
+    float getAperture() const { return _lens.radius; }
+    void setFocalDepth(float depth);
+    void setRenderTarget(const ImageResourceDescriptor& target);
+    void setSamples(const Sample* samples, uint32_t count, uint32_t splitColorSamples);
+    uint32_t getSampleCount() const { return _validSampleCount; }
+    matrix3x3 getSampleToCamera() const;
+    void setCameraToWorld(const transform& cameraToWorld) { _cameraToWorld = cameraToWorld; }
+    matrix4x4 getCameraToWorld() const { return matrix4x4(_cameraToWorld); }
+    const vector3& getTranslation() const { return _cameraToWorld.translation; }
+    // Copies the last rendered image into the CPU render target (render() does this for
+    // every enabled camera that has one).
+    void extractImage();
+
+protected:
+    void rebuildView();
+    vector2 nextJitter();
+
+    Raycaster& _raycaster;
+    FloatRect _viewport;
+    ThinLens _lens = {0.0f, 1.0f};
+    bool _enabled = true;
+    vector3 _eyeDir = vector3(0.0f, 0.0f, -1.0f);
+    ImageResourceDescriptor _renderTarget;
+    transform _cameraToWorld = transform::identity();
+    std::vector<Sample> _samples;
+    uint32_t _split = 1;
+    uint32_t _validSampleCount = 0;
+    bool _viewDirty = true;
+    hvvr_gpu_view* _view = nullptr;
+    std::vector<vector2> _frameJitters;
+    uint32_t _frameCount = 0;
+};
+
+class Raycaster {
+    friend class Camera;
+    friend class Model;
+
+public:
+    explicit Raycaster(const RayCasterSpecification& spec);
+    ~Raycaster();
+    Raycaster(const Raycaster&) = delete;
+    Raycaster& operator=(const Raycaster&) = delete;
+
+    Camera* createCamera(const FloatRect& viewport, float apertureRadius);
+    void destroyCamera(Camera* camera);
+    Light* createLight(const LightUnion& lightUnion);
+    void destroyLight(Light* light);
+    Model* createModel(MeshData&& meshData);
+    Model* createModel(const MeshData& meshData);
+    void destroyModel(Model* model);
+    void setSceneDirty();
+    void render(double elapsedTime);
+    void reinit(RayCasterGPUMode mode);
+
+    // GPU handles, for callers that mix this API with the C ABI (hvvr_gpu.h)
+    hvvr_gpu_context* gpuContext() const { return _ctx; }
+    hvvr_gpu_scene* gpuScene() const { return _scene; }
+
+protected:
+    void updateScene();
+    void transformsChanged() { _transformsDirty = true; }
+
+    RayCasterSpecification _spec;
+    std::vector<std::unique_ptr<Camera>> _cameras;
+    std::vector<std::unique_ptr<Light>> _lights;
+    std::vector<std::unique_ptr<Model>> _models;
+    bool _sceneDirty = false;
+    bool _transformsDirty = false;
+    bool _lightsDirty = true;
+    hvvr_gpu_context* _ctx = nullptr;
+    hvvr_gpu_scene* _scene = nullptr;
+};
+
+} // namespace hvvr

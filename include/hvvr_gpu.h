@@ -111,6 +111,11 @@ int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float
 /* Several views of one scene in one submission (batched multi-view). */
 int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
                           const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
+/* Tile traversal keeps its distance-sorted stack in LDS, up to `entries` (1..512, default
+ * 512); tiles that need more are re-traversed with a full-depth (4096-entry, the
+ * reference's TileFrame::stackSize, traverse_avx.h:281) global-memory stack. The result
+ * is the same either way; the limit only moves work between the two paths. */
+int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entries);
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
 

@@ -26,7 +26,7 @@
  * and FTZ/DAZ on NVIDIA hardware and cannot be run here, so agreement with them is
  * not claimed bit for bit; each function's pinning is listed in DESIGN.md.
  *
- * All layouts below are byte-identical to the product's (hvvr_amd/csrc/common/hvvr_types.h),
+ * All layouts below are byte-identical to the product's (include/hvvr/types.h),
  * which a CPU test checks field by field.
  */
 #include <stdint.h>

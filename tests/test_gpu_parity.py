@@ -1,0 +1,120 @@
+"""GPU render path vs the CPU oracle on the same seeded scenes and cameras.
+
+Bar: BVH refit, tile triangle lists, per-subsample triangle ids and subsample masks
+bit-exact; colours within 1/255 per channel (libm pow/sqrt may differ by an ulp)."""
+
+import numpy as np
+import pytest
+
+from conftest import rgba_delta
+
+pytestmark = pytest.mark.gpu
+
+
+def _render(H, ctx, scene, view, A, limit=None, m=None):
+    gs = H.GpuScene(ctx, scene)
+    gv = H.GpuView(ctx, view, A)
+    if limit:
+        gv.set_traversal_stack_limit(limit)
+    gv.render(gs, m)
+    return gs, gv
+
+
+def _check_frame(O, gs, gv, scene, view, A, m=None, colors=True):
+    ref = O.render_view(scene, view, A, camera_to_world=m, with_colors=colors)
+    assert np.array_equal(gs.read_nodes().view(np.uint8), ref["nodes"].view(np.uint8)), "refit bounds differ"
+    assert np.array_equal(gs.read_triangles().view(np.uint8), ref["tris"].view(np.uint8)), "triangles differ"
+    counts, lists = gv.read_tile_lists()
+    assert np.array_equal(counts, ref["counts"]), "tile list lengths differ"
+    assert np.array_equal(lists, ref["lists"]), "tile list contents/order differ"
+    tri, mask = gv.read_gbuffer()
+    rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+    bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
+    assert len(bad) == 0, f"{len(bad)} samples differ, first {bad[:5]}"
+    st = gv.stats()
+    assert st.survivorPairs == ref["survivors"] and st.overflow == 0
+    if colors:
+        col = gv.read_sample_colors()
+        assert rgba_delta(col, ref["colors"]).max() <= 1
+        img = gv.read_image()
+        assert rgba_delta(img, ref["image"]).max() <= 1
+    return ref
+
+
+@pytest.mark.parametrize("A", [1, 4])
+def test_heightfield_640x480(H, O, ctx, scene1, view1, A):
+    gs, gv = _render(H, ctx, scene1, view1, A)
+    _check_frame(O, gs, gv, scene1, view1, A)
+
+
+@pytest.mark.parametrize("A", [2, 8, 16, 32])
+def test_heightfield_msaa_rates(H, O, ctx, scene1, view1_small, A):
+    gs, gv = _render(H, ctx, scene1, view1_small, A)
+    _check_frame(O, gs, gv, scene1, view1_small, A)
+
+
+def test_room_262k_reduced_eye(H, O, ctx, scene2, view2_small):
+    gs, gv = _render(H, ctx, scene2, view2_small, 4)
+    _check_frame(O, gs, gv, scene2, view2_small, 4)
+
+
+@pytest.mark.parametrize("limit", [1, 8, 40])
+def test_deep_stack_fallback_same_lists(H, O, ctx, scene2, view2_small, limit):
+    # a tiny LDS stack budget sends most tiles down the global-memory traversal path
+    gs, gv = _render(H, ctx, scene2, view2_small, 4, limit=limit)
+    _check_frame(O, gs, gv, scene2, view2_small, 4, colors=False)
+
+
+def test_render_is_deterministic(H, ctx, scene2, view2_small):
+    gs, gv = _render(H, ctx, scene2, view2_small, 4)
+    a = gv.read_gbuffer()
+    c = gv.read_sample_colors()
+    gv.render(gs)
+    b = gv.read_gbuffer()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(c, gv.read_sample_colors())
+
+
+def test_animated_transforms_refit(H, O, ctx, scene1, view1_small):
+    gs, gv = _render(H, ctx, scene1, view1_small, 4)
+    rng = np.random.default_rng(7)
+    xf = scene1.transforms.copy()
+    moved = rng.choice(len(xf), size=len(xf) // 10, replace=False)
+    xf[moved, 12:15] += rng.normal(0, 0.5, size=(len(moved), 3)).astype(np.float32)
+    gs.set_transforms(xf)
+    gv.render(gs)
+    animated = H.HostScene.__new__(H.HostScene)
+    animated.__dict__.update(scene1.__dict__)
+    animated.transforms = xf
+    _check_frame(O, gs, gv, animated, view1_small, 4)
+
+
+def test_vertex_displacement_refit(H, O, ctx, scene2, view2_small):
+    gs, gv = _render(H, ctx, scene2, view2_small, 2)
+    rng = np.random.default_rng(11)
+    verts = scene2.verts.copy()
+    first, n = 256 * 190, 50 * 190
+    verts["pos"][first:first + n] += rng.normal(0, 0.02, size=(n, 3)).astype(np.float32)
+    gs.set_vertices(first, verts[first:first + n])
+    gv.render(gs)
+    displaced = H.HostScene.__new__(H.HostScene)
+    displaced.__dict__.update(scene2.__dict__)
+    displaced.verts = verts
+    _check_frame(O, gs, gv, displaced, view2_small, 2, colors=False)
+
+
+def test_camera_looking_away_renders_clear(H, O, ctx, scene1, view1_small):
+    # camera below the ground looking down: every tile list is empty
+    m = view1_small.camera_to_world.copy().reshape(4, 4)
+    m[3, :3] = [8.0, -50.0, 8.0]
+    m[:3, :3] = np.diag([1.0, -1.0, -1.0]).astype(np.float32)
+    gs, gv = _render(H, ctx, scene1, view1_small, 4, m=m.reshape(16))
+    counts, _ = gv.read_tile_lists()
+    assert counts.sum() == 0
+    assert np.all(gv.read_sample_colors() == 0xFF000000)
+    _check_frame(O, gs, gv, scene1, view1_small, 4, m=m.reshape(16))
+
+
+def test_invalid_view_rejected(H, ctx, view1_small):
+    with pytest.raises(H.HvvrError, match="msaaRate"):
+        H.GpuView(ctx, view1_small, 3)
