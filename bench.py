@@ -26,8 +26,7 @@ import hvvr_amd as H  # noqa: E402
 W, HGT, A, EYES, MESHES = 1080, 1200, 4, 2, 1024
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector FLOP/s
 PEAK_HBM = 8.0e12  # MI355X HBM3E bytes/s
-STAGES = ["frustum_setup", "block_traverse", "tile_traverse_count", "scan", "tile_traverse_emit", "intersect",
-          "resolve", "remap"]
+STAGES = H.STAGES
 
 
 def log(*a):
@@ -73,6 +72,19 @@ def cpu_baseline(scene, views, budget_s=20.0):
                       f"resolve; scene refit ({t_refit:.2f} s) excluded"}
 
 
+def copy_calibration(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +92,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
+                         "workload for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU: the render path shards by view "
                                    "(eye / request) with no per-frame exchange, so N>1 is N independent "
@@ -140,7 +158,7 @@ def main():
     # latency bound, reported against HBM with 128 B per list entry written+read as a floor
     dom = int(np.argmax(stage_ms))
     flops_intersect = survivors * (80 + 16 * A)
-    rl_intersect = flops_intersect / (stage_ms[5] * 1e-3)
+    rl_intersect = flops_intersect / (stage_ms[STAGES.index("intersect")] * 1e-3)
     roof = {"kernel": "intersect", "bound": "mfma", "achieved": round(rl_intersect / 1e12, 3),
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(rl_intersect / PEAK_FP32, 4),
             "traffic": None,

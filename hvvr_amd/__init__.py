@@ -53,6 +53,9 @@ assert DTYPES["vertex"].itemsize == 32 and DTYPES["node"].itemsize == 128
 assert DTYPES["material"].itemsize == 64 and DTYPES["light"].itemsize == 56
 assert DTYPES["beam"].itemsize == 36 and DTYPES["sfrustum"].itemsize == 96 and DTYPES["sample"].itemsize == 28
 
+# per-stage timings returned by GpuView.stage_times (hvvr_gpu_view_stage_times)
+STAGES = ["frustum_setup", "block_traverse", "tile_lists", "tile_lists_fallback", "intersect", "resolve", "remap"]
+
 _gpu = None
 _host = None
 
@@ -388,8 +391,8 @@ class GpuView:
         return st
 
     def stage_times(self):
-        out = np.zeros(8, np.float32)
-        n = _check_gpu(self.lib.hvvr_gpu_view_stage_times(self.handle, _ptr(out), c_int(8)))
+        out = np.zeros(len(STAGES), np.float32)
+        n = _check_gpu(self.lib.hvvr_gpu_view_stage_times(self.handle, _ptr(out), c_int(len(STAGES))))
         return out[:n]
 
     def close(self):

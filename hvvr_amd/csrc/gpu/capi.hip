@@ -379,8 +379,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->tileStackScratch.resize(size_t(TILE_STACK_SIZE) * v->tileCount));
     step(v->tileCounts.resize(v->tileCount));
     step(v->overflowTiles.resize(v->tileCount));
-    step(v->tileOffsets.resize(v->tileCount + 1));
-    step(v->scanPartials.resize((v->tileCount + 1023) / 1024 + 1));
+    step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
     step(v->gbuffer.resize(size_t(v->sampleCount) * v->msaaRate));
@@ -410,8 +409,8 @@ int hvvr_gpu_view_destroy(hvvr_gpu_view* v) {
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* v, uint32_t entries) {
     if (!v)
         return fail("null view");
-    if (entries < 1 || entries > 512)
-        return fail("traversal stack limit must be in [1, 512]");
+    if (entries < 1 || entries > 256)
+        return fail("traversal stack limit must be in [1, 256]");
     v->waveStackLimit = entries;
     return HVVR_GPU_OK;
 }
@@ -445,18 +444,16 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     mark(1);
     launchBlockTraverse(v, s, st);
     mark(2);
-    launchTileTraverse(v, s, false, st);
+    launchTileLists(v, s, st);
     mark(3);
-    launchScan(v, st);
+    launchTileListFallback(v, s, st);
     mark(4);
-    launchTileTraverse(v, s, true, st);
-    mark(5);
     launchIntersect(v, s, m, st);
-    mark(6);
+    mark(5);
     launchResolve(v, s, m, st);
-    mark(7);
+    mark(6);
     launchRemap(v, st);
-    mark(8);
+    mark(7);
     HVVR_HIP_CHECK(hipGetLastError());
     if (prof) {
         HVVR_HIP_CHECK(hipEventSynchronize(c->stageEvents[kStageCount]));
@@ -574,16 +571,30 @@ int hvvr_gpu_view_read_tile_lists(hvvr_gpu_view* v, uint32_t* counts, uint32_t* 
     int rc = viewSync(v);
     if (rc)
         return rc;
-    uint32_t t = 0;
+    uint64_t t = 0;
     HVVR_HIP_CHECK(hipMemcpy(counts, v->tileCounts.ptr, size_t(v->tileCount) * 4, hipMemcpyDeviceToHost));
-    HVVR_HIP_CHECK(hipMemcpy(&t, v->tileOffsets.ptr + v->tileCount, 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < v->tileCount; i++)
+        t += counts[i];
     if (total)
         *total = t;
     if (tris) {
         if (capacity < t)
             return fail("tile list capacity too small", HVVR_GPU_ERR_OVERFLOW);
-        if (t)
-            HVVR_HIP_CHECK(hipMemcpy(tris, v->triIndices.ptr, size_t(t) * 4, hipMemcpyDeviceToHost));
+        // lists are stored in allocation order: gather them into tile order
+        uint64_t allocated = 0;
+        HVVR_HIP_CHECK(hipMemcpy(&allocated, v->counters.ptr + 3, 8, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> offsets(v->tileCount), all(allocated);
+        HVVR_HIP_CHECK(hipMemcpy(offsets.data(), v->tileOffsets.ptr, offsets.size() * 4, hipMemcpyDeviceToHost));
+        if (allocated)
+            HVVR_HIP_CHECK(hipMemcpy(all.data(), v->triIndices.ptr, size_t(allocated) * 4, hipMemcpyDeviceToHost));
+        uint64_t at = 0;
+        for (uint32_t i = 0; i < v->tileCount; i++) {
+            if (uint64_t(offsets[i]) + counts[i] > allocated)
+                return fail("tile list outside the allocated range", HVVR_GPU_ERR_STATE);
+            if (counts[i])
+                memcpy(tris + at, all.data() + offsets[i], size_t(counts[i]) * 4);
+            at += counts[i];
+        }
     }
     return HVVR_GPU_OK;
 }
@@ -598,8 +609,9 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
     std::vector<uint32_t> counts(v->tileCount);
     HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
-    uint32_t total = 0;
-    HVVR_HIP_CHECK(hipMemcpy(&total, v->tileOffsets.ptr + v->tileCount, 4, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (uint32_t c : counts)
+        total += c;
     memset(st, 0, sizeof(*st));
     st->listEntries = total;
     st->survivorPairs = counters[0];

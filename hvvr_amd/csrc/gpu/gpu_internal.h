@@ -49,7 +49,7 @@ struct DeviceBuffer {
     size_t bytes() const { return count * sizeof(T); }
 };
 
-enum { kStageCount = 8 };
+enum { kStageCount = 7 };
 
 struct RefitLevel {
     uint32_t start, count;
@@ -98,18 +98,18 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
     hvvr::DeviceBuffer<hvvr::StackEntry> tileStackScratch; // [TILE_STACK_SIZE][tileCount], slot-major
-    hvvr::DeviceBuffer<uint32_t> overflowTiles;         // [tileCount] 1 = LDS stack overflowed
-    hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount]
-    hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount + 1] exclusive scan, [tileCount] = total
-    hvvr::DeviceBuffer<uint32_t> scanPartials;
-    hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage
+    hvvr::DeviceBuffer<uint32_t> overflowTiles;         // [tileCount] 1 = did not fit the LDS traversal
+    hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length
+    hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount] list start in triIndices
+    hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage, tiles in allocation order
     hvvr::DeviceBuffer<hvvr::GBufferSubsample> gbuffer; // [sampleCount * A], wave-interleaved
     hvvr::DeviceBuffer<uint32_t> sampleResults;         // [sampleCount]
     hvvr::DeviceBuffer<int32_t> remap;                  // [width*height*split]
     hvvr::DeviceBuffer<uint32_t> image;                 // [width*height]
-    hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack
+    hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
+                                                        // [3] list entries allocated
     uint64_t listCapacity = 0;
-    uint32_t waveStackLimit = 512; // LDS stack entries per tile traversal (<= WAVE_STACK_SIZE)
+    uint32_t waveStackLimit = 256; // LDS stack entries per tile traversal (<= 256)
     float stageMs[hvvr::kStageCount] = {};
     bool rendered = false;
 };
@@ -120,8 +120,8 @@ namespace hvvr {
 void launchSceneRefit(hvvr_gpu_scene* s, hipStream_t stream);
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream);
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
-void launchTileTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, bool emit, hipStream_t stream);
-void launchScan(hvvr_gpu_view* v, hipStream_t stream);
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
+void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
 void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchRemap(hvvr_gpu_view* v, hipStream_t stream);

@@ -8,14 +8,12 @@
 //   blockTraverseWaveKernel   one wave64 per block: cooperative node tests, the <= 64-entry
 //                             cull stack in LDS, then the distance sort of
 //                             BlockFrame::sort as a parallel rank computation
-//   tileTraverseWaveKernel    one wave64 per tile (wave_traverse.h). Run twice: count pass,
-//                             then (after an exclusive scan of the counts) emit pass, so
-//                             every tile's list lands contiguously in tile order with no
-//                             host round trip and no atomics.
+//   tileListKernel            one wave64 per tile, in three phases (see below); writes the
+//                             tile's list at an offset claimed with one atomic add, so no
+//                             count pass and no scan are needed
 //   tileTraverseKernel        one thread per tile with the full 4096-entry stack in global
-//                             memory; only re-runs tiles whose LDS stack overflowed.
-//   scan kernels              3-phase exclusive scan of the per-tile counts.
-
+//                             memory; only runs the tiles tileListKernel could not hold in
+//                             LDS (count pass, allocation, emit pass)
 #include "device_common.h"
 #include "gpu_internal.h"
 #include "wave_traverse.h"
@@ -70,81 +68,6 @@ __device__ inline float ftz(float x) {
     return (fbits(x) & 0x7f800000u) == 0 ? bitsf(fbits(x) & 0x80000000u) : x;
 }
 
-// traverseBlocks (traverse_avx.h:316-410) + BlockFrame::sort (:293-314)
-__global__ __launch_bounds__(64) void blockTraverseKernel(const BVHNode* __restrict__ nodes,
-                                                          const FrustumPre* __restrict__ blockPre,
-                                                          uint32_t blockCount, StackEntry* __restrict__ stacks,
-                                                          uint32_t* __restrict__ stackSize) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= blockCount)
-        return;
-    const FrustumPre& f = blockPre[b];
-    if (f.degenerate) {
-        stackSize[b] = 0;
-        return;
-    }
-    CullEntry cull[BLOCK_STACK_SIZE + 4];
-    uint32_t top = 0;
-    float tMin = 0.0f;
-    uint32_t node = 0;
-    int axis = f.distanceAxis, sizeAxis = f.distanceAxis & ~1;
-    for (;;) {
-        const BVHNode& n = nodes[node];
-        uint32_t mask = testChildren(f, n);
-        mask = testChildrenRefine(f, n, mask);
-        if (mask) {
-            uint32_t leaf = mask & n.leafMask;
-            if (leaf) {
-                mask &= ~leaf;
-                uint32_t i = top;
-                while (i > 0 && daz(cull[i - 1].bits) > 0.0f) {
-                    cull[i] = cull[i - 1];
-                    i--;
-                }
-                top++;
-                cull[i].bits = leaf;
-                cull[i].tMinTemp = tMin;
-                cull[i].node = node;
-            }
-            for (int c = 0; c < 4; c++) {
-                if (!(mask & (1u << c)))
-                    continue;
-                float dist = -n.bounds[axis][c];
-                float size = ftz(n.bounds[sizeAxis][c] + n.bounds[sizeAxis + 1][c]);
-                float key = daz(fbits(size));
-                uint32_t i = top;
-                while (i > 0 && daz(cull[i - 1].bits) > key) {
-                    cull[i] = cull[i - 1];
-                    i--;
-                }
-                cull[i].bits = fbits(size);
-                cull[i].tMinTemp = dist;
-                cull[i].node = node + n.data[1 + c];
-                top++;
-            }
-        }
-        if (top >= BLOCK_STACK_SIZE - 4 || top == 0 || cull[top - 1].bits <= 0xfu)
-            break;
-        top--;
-        node = cull[top].node;
-        tMin = cull[top].tMinTemp;
-    }
-    StackEntry* out = stacks + size_t(b) * BLOCK_STACK_SIZE;
-    for (uint32_t t = 0; t < top; t++) {
-        StackEntry e;
-        e.tMin = cull[t].tMinTemp;
-        uint32_t lm = cull[t].bits <= 0xfu ? cull[t].bits : 0u;
-        e.nodeMask = (cull[t].node & STACK_NODE_MASK) | (lm << 28);
-        uint32_t slot = t;
-        while (slot > 0 && out[slot - 1].tMin < e.tMin) {
-            out[slot] = out[slot - 1];
-            slot--;
-        }
-        out[slot] = e;
-    }
-    stackSize[b] = top;
-}
-
 // traverseBlocks + BlockFrame::sort, one wave64 per block
 __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __restrict__ nodes,
                                                               const FrustumPre* __restrict__ blockPre,
@@ -153,6 +76,7 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
     __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
     const uint32_t b = blockIdx.x;
     const uint32_t lane = threadIdx.x, c = lane & 3u, role = lane >> 2;
+    const uint32_t projAxis = laneProjAxis(role);
     WaveFrustum w;
     loadWaveFrustum(blockPre[b], role, w);
     if (w.degenerate) {
@@ -184,7 +108,7 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
     };
     for (;;) {
         NodeLanes nl;
-        loadNodeLanes(nodes, node, c, nl);
+        loadNodeLanes(nodes, node, c, projAxis, nl);
         const uint32_t leafMask = nodes[node].leafMask;
         uint32_t mask = waveTestChildren(w, nl, role);
         mask = waveTestRefine(w, nl, role, mask);
@@ -234,48 +158,278 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
         stackSize[b] = top;
 }
 
-// traverseTiles, one wave64 per tile (wave_traverse.h)
-__global__ __launch_bounds__(64) void tileTraverseWaveKernel(const BVHNode* __restrict__ nodes,
-                                                             const FrustumPre* __restrict__ tilePre,
-                                                             const StackEntry* __restrict__ blockStacks,
-                                                             const uint32_t* __restrict__ blockStackSize,
-                                                             uint32_t* __restrict__ counts,
-                                                             const uint32_t* __restrict__ offsets,
-                                                             uint32_t* __restrict__ triOut, uint64_t capacity,
-                                                             uint32_t* __restrict__ overflowTiles,
-                                                             unsigned long long* __restrict__ counters,
-                                                             uint32_t stackLimit, int emit) {
-    __shared__ StackEntry stack[WAVE_STACK_SIZE];
+// ---------------------------------------------------------------------------------
+// Tile lists: traverseTiles (traverse_avx.h:412-527), one wave64 per tile.
+//
+// The reference walks a tile's BVH cut depth-first from its block's sorted stack,
+// keeping the stack sorted by distance and emitting each visited node's surviving leaf
+// triangles in visit order. A node's test depends only on the tile frustum and the
+// node, not on when the walk reaches it, so the walk is split into three phases:
+//   A. tests: breadth-first over the nodes the walk will visit, 16 nodes per step (lane
+//      = node slot x child, each lane testing one child against every frustum axis).
+//      Each visited node becomes an LDS record: its final child mask (coarse test,
+//      refined when a leaf child passes), its leaf mask, the leaf mask of its stack
+//      entry and the index of its first internal child's record; the internal children
+//      that pass are appended as new records, in child order, with their distance keys.
+//   B. order: the reference's stack walk replayed over the records, in LDS only: pops,
+//      the single-child shortcut, the sorted insertion (one ballot compares up to 64
+//      entries), and the sequence of records whose leaves are emitted.
+//   C. emit: the list length is summed, space is claimed with one atomic add, and the
+//      triangle ranges of the emitted leaves are written in order.
+// The dependent global loads per tile drop from one per visited node (~370 on the
+// 262k-triangle room scene) to one per 16 nodes. A tile whose records or stack exceed
+// the LDS capacity is flagged for the global-stack kernel instead.
+// ---------------------------------------------------------------------------------
+enum : uint32_t { TL_REC_CAP = 768, TL_STACK_CAP = 256 };
+
+struct TileListLds {
+    uint32_t recNode[TL_REC_CAP];
+    uint32_t recKey[TL_REC_CAP];   // float bits of the record's distance key; in phase C, the
+                                   // list length of emitted record i
+    uint32_t recMeta[TL_REC_CAP];  // first child record (bits 0..15), child mask (16..19), leaf
+                                   // mask (20..23), stack-entry leaf mask (24..27)
+    uint16_t order[TL_REC_CAP];    // records with emitted leaves, in emission order
+    StackEntry stack[TL_STACK_CAP]; // tMin, record index
+};
+
+__device__ inline uint32_t metaFirst(uint32_t m) { return m & 0xffffu; }
+__device__ inline uint32_t metaMask(uint32_t m) { return (m >> 16) & 0xfu; }
+__device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 20) & 0xfu; }
+__device__ inline uint32_t metaEntry(uint32_t m) { return m >> 24; }
+
+// number of triangles in the leaves `leaf` of `node`
+__device__ inline uint32_t leafListLength(const BVHNode* __restrict__ nodes, uint32_t node, uint32_t leaf) {
+    const uint32_t* d = nodes[node].data;
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t t0 = d[k], t1 = d[k + 1];
+        if ((leaf & (1u << k)) && t1 > t0)
+            n += t1 - t0;
+    }
+    return n;
+}
+
+__global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__ nodes,
+                                                     const FrustumPre* __restrict__ tilePre,
+                                                     const StackEntry* __restrict__ blockStacks,
+                                                     const uint32_t* __restrict__ blockStackSize,
+                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
+                                                     uint32_t* __restrict__ triOut, uint64_t capacity,
+                                                     uint32_t* __restrict__ overflowTiles,
+                                                     unsigned long long* __restrict__ counters, uint32_t stackLimit) {
+    __shared__ TileListLds sh;
     const uint32_t tile = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t block = tile / TILES_PER_BLOCK;
-    const uint64_t outBase = emit ? offsets[tile] : 0;
-    uint32_t count = 0;
-    auto sink = [&](uint32_t t0, uint32_t t1) {
-        uint32_t len = t1 - t0;
-        if (emit) {
-            for (uint32_t i = lane; i < len; i += 64) {
-                uint64_t at = outBase + count + i;
-                if (at < capacity)
-                    triOut[at] = t0 + i;
+    const FrustumPre* __restrict__ f = tilePre + tile;
+    const uint32_t n0 = blockStackSize[block];
+    if (n0 == 0 || f->degenerate) {
+        if (lane == 0) {
+            counts[tile] = 0u;
+            offsets[tile] = 0u;
+        }
+        return;
+    }
+    const uint32_t limit = stackLimit < TL_STACK_CAP ? stackLimit : TL_STACK_CAP;
+    bool overflow = n0 > limit;
+    uint32_t deepest = n0; // deepest stack size seen
+    uint32_t nEmit = 0;
+
+    // ---- phase A: node tests, breadth-first ----
+    if (!overflow) {
+        const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
+        if (lane < n0) {
+            StackEntry e = bs[lane];
+            sh.recNode[lane] = e.nodeMask & STACK_NODE_MASK;
+            sh.recKey[lane] = fbits(e.tMin);
+            sh.recMeta[lane] = (e.nodeMask >> 28) << 24;
+            StackEntry se;
+            se.tMin = e.tMin;
+            se.nodeMask = lane;
+            sh.stack[lane] = se;
+        }
+        waveLdsSync();
+        const uint32_t s = lane >> 2, c = lane & 3u;
+        const uint32_t axis = uint32_t(f->distanceAxis);
+        const uint64_t groupBelow = (1ull << (4 * s)) - 1ull;
+        uint32_t nrec = n0;
+        for (uint32_t head = 0; head < nrec;) {
+            // this step tests records [head, end); the children it appends start at nrec
+            const uint32_t end = head + 16 < nrec ? head + 16 : nrec;
+            const uint32_t r = head + s;
+            const bool valid = r < end;
+            uint32_t node = 0, entry = 0;
+            if (valid) {
+                node = sh.recNode[r];
+                entry = metaEntry(sh.recMeta[r]);
+            }
+            const BVHNode* __restrict__ nd = nodes + node;
+            const float* bb = &nd->bounds[0][0];
+            const float b0 = bb[c], b1 = bb[4 + c], b2 = bb[8 + c], b3 = bb[12 + c], b4 = bb[16 + c],
+                        b5 = bb[20 + c];
+            const float key = -bb[axis * 4 + c];
+            const uint32_t leafMask = nd->leafMask;
+            const uint32_t childOffset = nd->data[1 + c];
+            bool pass = valid && childCoarsePass(f, b0, b1, b2, b3, b4, b5);
+            uint32_t mask = uint32_t(__ballot(pass) >> (4 * s)) & 0xfu;
+            // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
+            const bool refine = (mask & leafMask) != 0u;
+            if (pass && refine)
+                pass = !childRefineCull(f, b0, b1, b2, b3, b4, b5);
+            mask = uint32_t(__ballot(pass) >> (4 * s)) & 0xfu;
+            // internal children are pushed only from entries without a leaf mask (traverse_avx.h:476-483)
+            const bool internal = pass && ((leafMask >> c) & 1u) == 0u && entry == 0u;
+            const uint64_t bi = __ballot(internal);
+            const uint32_t slot = nrec + lanesBelow(bi);
+            const uint32_t first = nrec + uint32_t(__builtin_popcountll(bi & groupBelow));
+            if (internal && slot < TL_REC_CAP) {
+                sh.recNode[slot] = node + childOffset;
+                sh.recKey[slot] = fbits(key);
+                sh.recMeta[slot] = 0u;
+            }
+            if (valid && c == 0)
+                sh.recMeta[r] = first | (mask << 16) | (leafMask << 20) | (entry << 24);
+            nrec += uint32_t(__builtin_popcountll(bi));
+            head = end;
+            waveLdsSync();
+            if (nrec > TL_REC_CAP) {
+                overflow = true;
+                break;
             }
         }
-        count += len;
-    };
-    uint32_t depth = 0;
-    bool ok = waveTraverseTile(nodes, tilePre[tile], blockStacks + size_t(block) * BLOCK_STACK_SIZE,
-                               blockStackSize[block], stack, stackLimit, lane, sink, &depth);
-    if (lane == 0) {
-        if (!ok) {
-            overflowTiles[tile] = 1u; // re-run by the global-stack kernel
-            count = 0;
-        } else {
-            if (emit && outBase + count > capacity)
-                atomicOr(&counters[1], 1ull);
-            atomicMax(&counters[2], (unsigned long long)depth);
+    }
+
+    // ---- phase B: the reference's stack walk over the records ----
+    if (!overflow) {
+        uint32_t top = n0 - 1;
+        uint32_t cur = uniform(sh.stack[top].nodeMask);
+        for (;;) {
+            const uint32_t meta = uniform(sh.recMeta[cur]);
+            const uint32_t mask = metaMask(meta), leafMask = metaLeaf(meta);
+            bool pop = true;
+            if (mask) {
+                if (mask & leafMask) {
+                    if (lane == 0)
+                        sh.order[nEmit] = uint16_t(cur);
+                    nEmit++;
+                }
+                const uint32_t inner = mask & ~leafMask;
+                if (inner && metaEntry(meta) == 0u) {
+                    const uint32_t first = metaFirst(meta);
+                    if ((inner & (inner - 1u)) == 0u) {
+                        // single hit child: test it next without touching the stack
+                        cur = first;
+                        continue;
+                    }
+                    const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
+                    for (uint32_t j = 0; j < nChildren; j++) {
+                        if (j)
+                            top++;
+                        if (top >= limit) {
+                            overflow = true;
+                            break;
+                        }
+                        const float dist = bitsf(uniform(sh.recKey[first + j]));
+                        // entries above the insertion point are the contiguous run from the
+                        // top whose distance is smaller than `dist`; shift them up by one
+                        uint32_t pos = top;
+                        for (;;) {
+                            const int below = int(pos) - 1 - int(lane);
+                            const bool smaller = below >= 0 && sh.stack[below >= 0 ? below : 0].tMin < dist;
+                            const uint64_t run = ~__ballot(smaller);
+                            const uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
+                            StackEntry moved;
+                            if (lane < n)
+                                moved = sh.stack[below];
+                            waveLdsSync();
+                            if (lane < n)
+                                sh.stack[below + 1] = moved;
+                            waveLdsSync();
+                            pos -= n;
+                            if (n < 64u)
+                                break;
+                        }
+                        if (lane == 0) {
+                            StackEntry ne;
+                            ne.tMin = dist;
+                            ne.nodeMask = first + j;
+                            sh.stack[pos] = ne;
+                        }
+                        waveLdsSync();
+                    }
+                    if (overflow)
+                        break;
+                    deepest = top + 1 > deepest ? top + 1 : deepest;
+                    cur = uniform(sh.stack[top].nodeMask);
+                    pop = false;
+                }
+            }
+            if (pop) {
+                if (top == 0)
+                    break;
+                top--;
+                cur = uniform(sh.stack[top].nodeMask);
+            }
         }
-        if (!emit)
-            counts[tile] = count;
+    }
+    if (overflow) {
+        // re-traversed by the global-stack kernel
+        if (lane == 0) {
+            overflowTiles[tile] = 1u;
+            counts[tile] = 0u;
+            offsets[tile] = 0u;
+        }
+        return;
+    }
+
+    // ---- phase C: emit ----
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < nEmit; base += 64) {
+        const uint32_t i = base + lane;
+        uint32_t len = 0;
+        if (i < nEmit) {
+            const uint32_t rec = sh.order[i];
+            const uint32_t meta = sh.recMeta[rec];
+            len = leafListLength(nodes, sh.recNode[rec], metaMask(meta) & metaLeaf(meta));
+            sh.recKey[i] = len;
+        }
+        total += waveSum(len);
+    }
+    unsigned long long start = 0;
+    if (lane == 0)
+        start = atomicAdd(&counters[3], (unsigned long long)total);
+    start = __shfl(start, 0);
+    const bool fits = start + total <= capacity;
+    if (lane == 0) {
+        counts[tile] = total;
+        offsets[tile] = uint32_t(start);
+        if (!fits)
+            atomicOr(&counters[1], 1ull);
+        atomicMax(&counters[2], (unsigned long long)deepest);
+    }
+    if (!fits)
+        return;
+    uint32_t run = uint32_t(start);
+    for (uint32_t base = 0; base < nEmit; base += 64) {
+        const uint32_t i = base + lane;
+        const uint32_t len = i < nEmit ? sh.recKey[i] : 0u;
+        const uint32_t incl = waveInclusiveScan(len, lane);
+        uint32_t at = run + incl - len;
+        run += __shfl(incl, 63);
+        if (len) {
+            const uint32_t rec = sh.order[i];
+            const uint32_t meta = sh.recMeta[rec];
+            const uint32_t leaf = metaMask(meta) & metaLeaf(meta);
+            const uint32_t* d = nodes[sh.recNode[rec]].data;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t t0 = d[k], t1 = d[k + 1];
+                if (leaf & (1u << k))
+                    for (uint32_t t = t0; t < t1; t++)
+                        triOut[at++] = t;
+            }
+        }
     }
 }
 
@@ -394,70 +548,17 @@ __global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restri
     atomicMax(&counters[2], (unsigned long long)(maxTop + 1));
 }
 
-// ---- exclusive scan of tile counts: offsets[i] = sum(counts[0..i)), offsets[n] = total ----
-enum { kScanThreads = 256, kScanPerThread = 4, kScanTile = kScanThreads * kScanPerThread };
-
-__device__ inline uint32_t blockInclusiveScan(uint32_t v, uint32_t* lds) {
-    uint32_t tid = threadIdx.x;
-    lds[tid] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t add = tid >= off ? lds[tid - off] : 0u;
-        __syncthreads();
-        lds[tid] += add;
-        __syncthreads();
-    }
-    return lds[tid];
-}
-
-__global__ __launch_bounds__(kScanThreads) void scanReduceKernel(const uint32_t* __restrict__ in, uint32_t n,
-                                                                 uint32_t* __restrict__ partials) {
-    __shared__ uint32_t lds[kScanThreads];
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    uint32_t s = 0;
-    for (int k = 0; k < kScanPerThread; k++)
-        s += base + k < n ? in[base + k] : 0u;
-    uint32_t inc = blockInclusiveScan(s, lds);
-    if (threadIdx.x == kScanThreads - 1)
-        partials[blockIdx.x] = inc;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scanPartialsKernel(uint32_t* partials, uint32_t count,
-                                                                   uint32_t* __restrict__ totalOut) {
-    __shared__ uint32_t lds[kScanThreads];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < count; base += kScanThreads) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < count ? partials[i] : 0u;
-        uint32_t inc = blockInclusiveScan(v, lds);
-        if (i < count)
-            partials[i] = carry + inc - v;
-        uint32_t sum = lds[kScanThreads - 1];
-        __syncthreads();
-        carry += sum;
-    }
-    if (threadIdx.x == 0)
-        *totalOut = carry;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scanApplyKernel(const uint32_t* __restrict__ in, uint32_t n,
-                                                                const uint32_t* __restrict__ partials,
-                                                                uint32_t* __restrict__ out) {
-    __shared__ uint32_t lds[kScanThreads];
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    uint32_t vals[kScanPerThread];
-    uint32_t s = 0;
-    for (int k = 0; k < kScanPerThread; k++) {
-        vals[k] = base + k < n ? in[base + k] : 0u;
-        s += vals[k];
-    }
-    uint32_t inc = blockInclusiveScan(s, lds);
-    uint32_t run = partials[blockIdx.x] + inc - s;
-    for (int k = 0; k < kScanPerThread; k++) {
-        if (base + k < n)
-            out[base + k] = run;
-        run += vals[k];
-    }
+// List space for the tiles the global-stack kernel re-traverses, after its count pass.
+__global__ void fallbackAllocKernel(const uint32_t* __restrict__ overflowTiles, const uint32_t* __restrict__ counts,
+                                    uint32_t* __restrict__ offsets, uint32_t tileCount, uint64_t capacity,
+                                    unsigned long long* __restrict__ counters) {
+    uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= tileCount || !overflowTiles[tile])
+        return;
+    unsigned long long start = atomicAdd(&counters[3], (unsigned long long)counts[tile]);
+    offsets[tile] = uint32_t(start);
+    if (start + counts[tile] > capacity)
+        atomicOr(&counters[1], 1ull);
 }
 
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream) {
@@ -472,26 +573,27 @@ void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t 
                                                               v->blockStacks.ptr, v->blockStackSize.ptr);
 }
 
-void launchTileTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, bool emit, hipStream_t stream) {
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
-    tileTraverseWaveKernel<<<v->tileCount, 64, 0, stream>>>(
-        s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr, v->blockStackSize.ptr, v->tileCounts.ptr,
-        v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity, v->overflowTiles.ptr, counters, v->waveStackLimit,
-        emit ? 1 : 0);
-    // tiles whose stack outgrew LDS: full-depth global-memory traversal, same results
-    tileTraverseKernel<<<(v->tileCount + 63) / 64, 64, 0, stream>>>(
-        s->nodes.ptr, v->tilePre.ptr, v->tileCount, v->blockStacks.ptr, v->blockStackSize.ptr,
-        v->tileStackScratch.ptr, v->tileCounts.ptr, v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity,
-        v->overflowTiles.ptr, counters, emit ? 1 : 0);
+    tileListKernel<<<v->tileCount, 64, 0, stream>>>(s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr,
+                                                    v->blockStackSize.ptr, v->tileCounts.ptr, v->tileOffsets.ptr,
+                                                    v->triIndices.ptr, v->listCapacity, v->overflowTiles.ptr, counters,
+                                                    v->waveStackLimit);
 }
 
-void launchScan(hvvr_gpu_view* v, hipStream_t stream) {
-    uint32_t n = v->tileCount;
-    uint32_t groups = (n + kScanTile - 1) / kScanTile;
-    scanReduceKernel<<<groups, kScanThreads, 0, stream>>>(v->tileCounts.ptr, n, v->scanPartials.ptr);
-    scanPartialsKernel<<<1, kScanThreads, 0, stream>>>(v->scanPartials.ptr, groups, v->tileOffsets.ptr + n);
-    scanApplyKernel<<<groups, kScanThreads, 0, stream>>>(v->tileCounts.ptr, n, v->scanPartials.ptr,
-                                                         v->tileOffsets.ptr);
+void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    const uint32_t groups = (v->tileCount + 63) / 64;
+    for (int emit = 0; emit < 2; emit++) {
+        if (emit)
+            fallbackAllocKernel<<<groups, 64, 0, stream>>>(v->overflowTiles.ptr, v->tileCounts.ptr,
+                                                           v->tileOffsets.ptr, v->tileCount, v->listCapacity,
+                                                           counters);
+        tileTraverseKernel<<<groups, 64, 0, stream>>>(
+            s->nodes.ptr, v->tilePre.ptr, v->tileCount, v->blockStacks.ptr, v->blockStackSize.ptr,
+            v->tileStackScratch.ptr, v->tileCounts.ptr, v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity,
+            v->overflowTiles.ptr, counters, emit);
+    }
 }
 
 } // namespace hvvr

@@ -1,26 +1,22 @@
 #pragma once
-// Cooperative tile traversal: one wave64 walks one tile's frustum through the BVH.
+// Wave64 building blocks shared by the traversal kernels.
 //
-// Same algorithm, same emission order as traverseTiles (raycaster/traverse_avx.h:
-// 412-527), but laid out for a 64-wide wavefront instead of one AVX2 core:
-//   * lane = (child c = lane & 3, role r = lane >> 2). A node's coarse test (4 frustum
-//     planes x 4 children in roles 0..3, 3 axis projections x 4 children in roles 4..6)
-//     and its refinement (12 edge axes x 4 children in roles 0..11) each take one pass,
-//     and a 64-bit ballot folds the per-lane verdicts into the 4-bit child mask;
-//   * every lane keeps the frustum coefficients of its role in registers for the whole
-//     tile, and reads the six bounds of its child from the node (the node index is
-//     wave-uniform, so leafMask/child offsets are scalar loads);
-//   * the distance-sorted stack lives in LDS; an insertion compares up to 64 entries in
-//     one ballot and shifts them with one LDS read + one LDS write per lane;
-//   * all control flow is wave-uniform, so the wave never diverges.
-// Stacks deeper than the LDS capacity set a per-tile overflow flag; the caller
-// re-traverses those tiles with the global-memory thread-per-tile kernel.
+// For a node test with one wave per frustum (block traversal): lane = (child c =
+// lane & 3, role r = lane >> 2). A node's coarse test (4 frustum planes x 4 children in
+// roles 0..3, 3 axis projections x 4 children in roles 4..6) and its refinement (12
+// edge axes x 4 children in roles 0..11) each take one pass, and a 64-bit ballot folds
+// the per-lane verdicts into the 4-bit child mask (testBVHNodeChildren/Refine,
+// raycaster/traverse_avx.h:104-217). Every lane keeps the frustum coefficients of its
+// role in registers, and each lane's node operands are loaded straight from the node,
+// with no per-lane indexing into private arrays (which the compiler would place in
+// scratch memory).
+//
+// For per-child tests (tile lists): one lane evaluates every plane, projection and
+// edge axis of one child, so a wave tests 16 nodes at once.
 
 #include "device_common.h"
 
 namespace hvvr {
-
-enum : uint32_t { WAVE_STACK_SIZE = 512 };
 
 struct WaveFrustum {
     // coefficients of this lane's role
@@ -64,28 +60,37 @@ __device__ inline uint32_t foldChildren(uint64_t b) {
 }
 
 struct NodeLanes {
-    float b[6]; // bounds[k][child] for k = 0..5
+    float b0, b1, b2, b3, b4, b5; // bounds[k][child]
+    float projMax, projNegMin;    // bounds[2a][child], bounds[2a+1][child] for this lane's projection axis a
 };
 
-__device__ inline void loadNodeLanes(const BVHNode* nodes, uint32_t node, uint32_t c, NodeLanes& nl) {
+// `projAxis` is the lane's projection axis (0 for lanes without one)
+__device__ inline void loadNodeLanes(const BVHNode* nodes, uint32_t node, uint32_t c, uint32_t projAxis,
+                                     NodeLanes& nl) {
     const float* base = &nodes[node].bounds[0][0];
-#pragma unroll
-    for (int k = 0; k < 6; k++)
-        nl.b[k] = base[k * 4 + c];
+    nl.b0 = base[0 * 4 + c];
+    nl.b1 = base[1 * 4 + c];
+    nl.b2 = base[2 * 4 + c];
+    nl.b3 = base[3 * 4 + c];
+    nl.b4 = base[4 * 4 + c];
+    nl.b5 = base[5 * 4 + c];
+    nl.projMax = base[(2 * projAxis) * 4 + c];
+    nl.projNegMin = base[(2 * projAxis + 1) * 4 + c];
+}
+
+__device__ inline uint32_t laneProjAxis(uint32_t role) {
+    return (role >= 4 && role < 7) ? role - 4 : 0u;
 }
 
 // testBVHNodeChildren (traverse_avx.h:104-164), one pass over 28 lanes
 __device__ inline uint32_t waveTestChildren(const WaveFrustum& w, const NodeLanes& n, uint32_t role) {
     bool cull = false;
     if (role < 4) {
-        float dmax = fmaf(w.pn0, n.b[0], fmaf(w.pn1, n.b[2], w.pn2 * n.b[4]));
-        float dneg = fmaf(w.pp0, n.b[1], fmaf(w.pp1, n.b[3], w.pp2 * n.b[5]));
+        float dmax = fmaf(w.pn0, n.b0, fmaf(w.pn1, n.b2, w.pn2 * n.b4));
+        float dneg = fmaf(w.pp0, n.b1, fmaf(w.pp1, n.b3, w.pp2 * n.b5));
         cull = !((dmax + dneg) <= w.pw);
     } else if (role < 7) {
-        uint32_t a = role - 4;
-        float mx = a == 0 ? n.b[0] : (a == 1 ? n.b[2] : n.b[4]);
-        float ng = a == 0 ? n.b[1] : (a == 1 ? n.b[3] : n.b[5]);
-        cull = !(mx >= w.projLo && ng >= w.projHi);
+        cull = !(n.projMax >= w.projLo && n.projNegMin >= w.projHi);
     }
     return 0xfu & ~foldChildren(__ballot(cull));
 }
@@ -94,12 +99,12 @@ __device__ inline uint32_t waveTestChildren(const WaveFrustum& w, const NodeLane
 __device__ inline uint32_t waveTestRefine(const WaveFrustum& w, const NodeLanes& n, uint32_t role, uint32_t prev) {
     bool cull = false;
     if (role < 12) {
-        float px = w.ex > 0.0f ? n.b[0] : -n.b[1];
-        float py = w.ey > 0.0f ? n.b[2] : -n.b[3];
-        float pz = w.ez > 0.0f ? n.b[4] : -n.b[5];
-        float nx = w.ex > 0.0f ? -n.b[1] : n.b[0];
-        float ny = w.ey > 0.0f ? -n.b[3] : n.b[2];
-        float nz = w.ez > 0.0f ? -n.b[5] : n.b[4];
+        float px = w.ex > 0.0f ? n.b0 : -n.b1;
+        float py = w.ey > 0.0f ? n.b2 : -n.b3;
+        float pz = w.ez > 0.0f ? n.b4 : -n.b5;
+        float nx = w.ex > 0.0f ? -n.b1 : n.b0;
+        float ny = w.ey > 0.0f ? -n.b3 : n.b2;
+        float nz = w.ez > 0.0f ? -n.b5 : n.b4;
         float maxBox = fmaf(w.ex, px, fmaf(w.ey, py, w.ez * pz));
         float minBox = fmaf(w.ex, nx, fmaf(w.ey, ny, w.ez * nz));
         cull = (maxBox < w.eMin) || (w.eMax < minBox);
@@ -119,112 +124,72 @@ __device__ inline void waveLdsSync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Walks one tile. `emit(t0, t1)` is called (wave-uniformly) for every leaf triangle range
-// in emission order. Returns false if the stack would exceed `stackLimit` entries
-// (results then incomplete). `stack` is the caller's LDS array of at least `stackLimit`
-// (<= WAVE_STACK_SIZE) entries.
-template <class Emit>
-__device__ inline bool waveTraverseTile(const BVHNode* __restrict__ nodes, const FrustumPre& f,
-                                        const StackEntry* __restrict__ blockStack, uint32_t blockStackSize,
-                                        StackEntry* stack, uint32_t stackLimit, uint32_t lane, Emit&& emit,
-                                        uint32_t* maxDepth) {
-    const uint32_t c = lane & 3u, role = lane >> 2;
-    WaveFrustum w;
-    loadWaveFrustum(f, role, w);
-    if (w.degenerate || blockStackSize == 0)
-        return true;
-    if (blockStackSize > stackLimit)
-        return false;
-    if (lane < blockStackSize)
-        stack[lane] = blockStack[lane];
-    waveLdsSync();
-    uint32_t top = blockStackSize - 1;
-    uint32_t deepest = top;
-    StackEntry e = stack[top];
-    uint32_t node = uniform(e.nodeMask & STACK_NODE_MASK);
-    uint32_t entryMask = uniform(e.nodeMask >> 28);
-    for (;;) {
-        NodeLanes nl;
-        loadNodeLanes(nodes, node, c, nl);
-        const uint32_t leafMask = nodes[node].leafMask;
-        uint32_t mask = waveTestChildren(w, nl, role);
-        bool pop = true;
-        if (mask) {
-            uint32_t leaf = mask & leafMask;
-            if (leaf) {
-                mask = waveTestRefine(w, nl, role, mask);
-                leaf = mask & leafMask;
-                for (int k = 0; k < 4; k++) {
-                    if (leaf & (1u << k)) {
-                        uint32_t t0 = nodes[node].data[k], t1 = nodes[node].data[k + 1];
-                        if (t1 > t0)
-                            emit(t0, t1);
-                    }
-                }
-            }
-            mask &= ~leafMask;
-            if (mask && entryMask == 0) {
-                if ((mask & (mask - 1)) == 0) {
-                    // single hit child: test it next without touching the stack
-                    node = node + nodes[node].data[1 + __builtin_ctz(mask)];
-                    continue;
-                }
-                bool first = true;
-                for (int k = 0; k < 4; k++) {
-                    if (!(mask & (1u << k)))
-                        continue;
-                    if (!first)
-                        top++;
-                    first = false;
-                    if (top >= stackLimit)
-                        return false;
-                    const float dist = -nodes[node].bounds[w.axis][k];
-                    // entries above the insertion point are the contiguous run from the top
-                    // whose distance is smaller than `dist`; shift them up by one
-                    uint32_t pos = top;
-                    for (;;) {
-                        int j = int(pos) - 1 - int(lane);
-                        bool smaller = j >= 0 && stack[j >= 0 ? j : 0].tMin < dist;
-                        uint64_t run = ~__ballot(smaller);
-                        uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
-                        StackEntry moved;
-                        if (lane < n)
-                            moved = stack[j];
-                        waveLdsSync();
-                        if (lane < n)
-                            stack[j + 1] = moved;
-                        waveLdsSync();
-                        pos -= n;
-                        if (n < 64)
-                            break;
-                    }
-                    if (lane == 0) {
-                        StackEntry ne;
-                        ne.tMin = dist;
-                        ne.nodeMask = (node + nodes[node].data[1 + k]) & STACK_NODE_MASK;
-                        stack[pos] = ne;
-                    }
-                    waveLdsSync();
-                }
-                deepest = top > deepest ? top : deepest;
-                e = stack[top];
-                node = uniform(e.nodeMask & STACK_NODE_MASK);
-                entryMask = uniform(e.nodeMask >> 28);
-                pop = false;
-            }
-        }
-        if (pop) {
-            if (top == 0)
-                break;
-            top--;
-            e = stack[top];
-            node = uniform(e.nodeMask & STACK_NODE_MASK);
-            entryMask = uniform(e.nodeMask >> 28);
-        }
+// ---------------------------------------------------------------------------------
+// per-child tests: one lane, one child, every axis of the frustum
+// ---------------------------------------------------------------------------------
+// Coarse test of one child box (testBVHNodeChildren, traverse_avx.h:104-164, for one
+// child lane): the four frustum planes against the box's n-vertex, then the three
+// world-axis projections. Same operations and rounding as testChildren().
+__device__ inline bool childCoarsePass(const FrustumPre* __restrict__ f, float b0, float b1, float b2, float b3,
+                                       float b4, float b5) {
+    bool pass = true;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        float dmax = fmaf(f->planeNeg[p][0], b0, fmaf(f->planeNeg[p][1], b2, f->planeNeg[p][2] * b4));
+        float dneg = fmaf(f->planePos[p][0], b1, fmaf(f->planePos[p][1], b3, f->planePos[p][2] * b5));
+        pass = pass && ((dmax + dneg) <= f->planeNeg[p][3]);
     }
-    if (maxDepth)
-        *maxDepth = deepest + 1;
-    return true;
+    pass = pass && (b0 >= f->projLo[0]) && (b1 >= f->projHi[0]);
+    pass = pass && (b2 >= f->projLo[1]) && (b3 >= f->projHi[1]);
+    pass = pass && (b4 >= f->projLo[2]) && (b5 >= f->projHi[2]);
+    return pass;
+}
+
+// Refinement of one child box (testBVHNodeChildrenRefine, traverse_avx.h:166-217, for
+// one child lane): true if one of the 12 frustum-edge x world-axis separating axes
+// separates the box from the frustum. Same operations as testChildrenRefine().
+__device__ inline bool childRefineCull(const FrustumPre* __restrict__ f, float b0, float b1, float b2, float b3,
+                                       float b4, float b5) {
+    bool cull = false;
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        float ex = f->edge[e][0], ey = f->edge[e][1], ez = f->edge[e][2];
+        float px = ex > 0.0f ? b0 : -b1;
+        float py = ey > 0.0f ? b2 : -b3;
+        float pz = ez > 0.0f ? b4 : -b5;
+        float nx = ex > 0.0f ? -b1 : b0;
+        float ny = ey > 0.0f ? -b3 : b2;
+        float nz = ez > 0.0f ? -b5 : b4;
+        float maxBox = fmaf(ex, px, fmaf(ey, py, ez * pz));
+        float minBox = fmaf(ex, nx, fmaf(ey, ny, ez * nz));
+        cull = cull || (maxBox < f->edgeMin[e]) || (f->edgeMax[e] < minBox);
+    }
+    return cull;
+}
+
+// ---------------------------------------------------------------------------------
+// wave-wide integer helpers
+// ---------------------------------------------------------------------------------
+// number of set bits of `b` in lanes below this one
+__device__ inline uint32_t lanesBelow(uint64_t b) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(b >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(b), 0u));
+}
+
+__device__ inline uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline uint32_t waveInclusiveScan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o);
+        if (lane >= uint32_t(o))
+            v += t;
+    }
+    return v;
 }
 
 } // namespace hvvr

@@ -111,10 +111,11 @@ int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float
 /* Several views of one scene in one submission (batched multi-view). */
 int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
                           const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
-/* Tile traversal keeps its distance-sorted stack in LDS, up to `entries` (1..512, default
- * 512); tiles that need more are re-traversed with a full-depth (4096-entry, the
- * reference's TileFrame::stackSize, traverse_avx.h:281) global-memory stack. The result
- * is the same either way; the limit only moves work between the two paths. */
+/* Tile traversal keeps its distance-sorted stack in LDS, up to `entries` (1..256, default
+ * 256); tiles that need more (or more than 768 visited nodes) are re-traversed with a
+ * full-depth (4096-entry, the reference's TileFrame::stackSize, traverse_avx.h:281)
+ * global-memory stack. The result is the same either way; the limit only moves work
+ * between the two paths. */
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entries);
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
@@ -139,8 +140,8 @@ typedef struct {
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the last profiled frame (needs profiling enabled):
- * [0] frustum setup, [1] block traversal, [2] tile traversal (count), [3] scan,
- * [4] tile traversal (emit), [5] intersect, [6] resolve, [7] remap. Returns the count. */
+ * [0] frustum setup, [1] block traversal, [2] tile lists, [3] tile lists of tiles that
+ * did not fit in LDS, [4] intersect, [5] resolve, [6] remap. Returns the count. */
 int hvvr_gpu_view_stage_times(hvvr_gpu_view* view, float* ms, int maxCount);
 
 #ifdef __cplusplus

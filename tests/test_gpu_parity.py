@@ -58,11 +58,20 @@ def test_room_262k_reduced_eye(H, O, ctx, scene2, view2_small):
     _check_frame(O, gs, gv, scene2, view2_small, 4)
 
 
-@pytest.mark.parametrize("limit", [1, 8, 40])
+@pytest.mark.parametrize("limit", [1, 40, 100, 256])
 def test_deep_stack_fallback_same_lists(H, O, ctx, scene2, view2_small, limit):
-    # a tiny LDS stack budget sends most tiles down the global-memory traversal path
+    # an LDS stack budget below a tile's need sends that tile down the global-memory
+    # traversal path: 1 and 40 send every tile (block stacks hold ~60 entries), 100 about
+    # half of them, 256 none
     gs, gv = _render(H, ctx, scene2, view2_small, 4, limit=limit)
     _check_frame(O, gs, gv, scene2, view2_small, 4, colors=False)
+
+
+def test_room_full_eye_1080x1200(H, O, ctx, scene2):
+    # the benchmark's view at full size: eye 0, 1080x1200, barrel distortion, A = 4
+    view = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 1080, 1200)
+    gs, gv = _render(H, ctx, scene2, view, 4)
+    _check_frame(O, gs, gv, scene2, view, 4)
 
 
 def test_render_is_deterministic(H, ctx, scene2, view2_small):

@@ -121,11 +121,8 @@ def test_scene_file_is_rejected_when_malformed(tmp_path):
     assert r.returncode == 1 and "HVSC" in r.stderr
 
 
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
 def test_no_device_raises_raycaster_error(H, tmp_path):
-    import torch
-
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
     models, lights = _scene(H)
     path = tmp_path / "scene.bin"
     _write_scene(path, models, lights)
