@@ -33,27 +33,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, views, budget_s=20.0):
-    """The CPU oracle (single-threaded C restatement of the reference's traversal +
-    intersect + resolve) timed on a bounded sample: the first blocks of eye 0."""
+def cpu_baseline(scene, views, budget_s=20.0, threads=None):
+    """The CPU oracle (C restatement of the reference's traversal + intersect + resolve)
+    timed on a bounded sample of the stereo frame: blocks in a seeded shuffled order, one
+    block per task on a thread pool (the reference's CPU stage runs one task per few blocks on
+    hardware_concurrency threads, render.cpp:344-364, traversal.cpp:249-254), until the
+    time budget is used. ctypes releases the GIL, so the C work runs in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
-    v = views[0]
-    m = v.camera_to_world
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     wv, tris, nodes = O.refit(scene)
     env, _ = O.lighting_env(scene.lights)
     offsets = H.subsample_offsets(A)
     t_refit = time.perf_counter() - t0
-    samples = 0
-    t_render = 0.0
-    # blocks spread over the image, processed until the time budget is used
-    order = np.linspace(0, v.block_count - 1, v.block_count).astype(int)
-    rng = np.random.default_rng(0)
-    rng.shuffle(order)
-    for b in order:
-        t0 = time.perf_counter()
+    tasks = [(e, b) for e in range(len(views)) for b in range(views[e].block_count)]
+    order = np.random.default_rng(0).permutation(len(tasks))
+
+    def block(task):
+        e, b = task
+        v = views[e]
+        m = v.camera_to_world
         tiles = slice(b * 64, (b + 1) * 64)
         world, tpre = O.frustum_setup(v.tile_frusta[tiles], m)
         _, bpre = O.frustum_setup(v.block_frusta[b:b + 1], m)
@@ -61,15 +64,28 @@ def cpu_baseline(scene, views, budget_s=20.0):
         beams = v.beams[b * 8192:(b + 1) * 8192]
         ids, _, _ = O.intersect_view(beams, world, m, offsets, A, counts, lists, tris)
         O.resolve_view(beams, m, offsets, A, ids, tris, scene.triShade, wv, scene.materials, env, 0)
-        t_render += time.perf_counter() - t0
-        samples += 8192 * A
-        if t_render > budget_s:
-            break
-    value = samples / t_render / 1e6
-    return {"value": round(value, 4), "unit": "Msamples/s", "cores": 1, "kind": "port",
-            "sample": f"{samples // (8192 * A)} of {v.block_count} blocks (8192 samples x {A} subsamples each) of "
-                      f"eye 0, single-threaded C oracle: frustum setup + block/tile traversal + intersect + "
-                      f"resolve; scene refit ({t_refit:.2f} s) excluded"}
+        return min(8192, v.valid_sample_count - b * 8192) * A  # padding samples are not counted
+
+    samples = 0
+    done = 0
+    t_start = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        # waves of `threads` blocks, passing over both eyes again if time is left
+        while time.perf_counter() - t_start < budget_s:
+            for i in range(0, len(order), threads):
+                batch = [tasks[j] for j in order[i:i + threads]]
+                samples += sum(pool.map(block, batch))
+                done += len(batch)
+                if time.perf_counter() - t_start > budget_s:
+                    break
+    wall = time.perf_counter() - t_start
+    value = samples / wall / 1e6
+    total = len(tasks)
+    return {"value": round(value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{done} block renders ({total} blocks per stereo frame, 8192 samples x {A} subsamples "
+                      f"each, seeded shuffled order over both eyes) in {wall:.1f} s on {threads} threads; C oracle: "
+                      f"frustum setup + block/tile traversal + intersect + resolve; scene refit ({t_refit:.2f} s, "
+                      f"one thread) excluded"}
 
 
 def copy_calibration(mib):
@@ -85,6 +101,45 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for a
+    plain copy of known size), or None if the summary has no entry for it."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_stereo_1080x1200_a4.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
+    if not m:
+        return None
+    entry = json.loads(m.group(1)).get(kernel)
+    return None if entry is None else int(entry["read_bytes"] + entry["write_bytes"])
+
+
+def roofline(stage_ms, survivors, entries, visits):
+    """Roofline of the dominant kernel, per launch (one launch per eye).
+    tile_lists: HBM-class bytes = 128 B per BVH node visit + 4 B per list entry written
+    (the node reads mostly hit L2/MALL, which `traffic` shows);
+    intersect: FP32 flops = N_pair (80 + 16 A), N_pair = tile-cull survivors x 128."""
+    dom = STAGES[int(np.argmax(stage_ms))]
+    if dom == "intersect":
+        flops = survivors * (80 + 16 * A)
+        ach = flops / (stage_ms[STAGES.index("intersect")] * 1e-3)
+        return {"kernel": "intersectKernel<4>", "bound": "mfma", "achieved": round(ach / 1e12, 3),
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32, 4),
+                "traffic": measured_traffic("intersectKernel<4>"),
+                "note": f"FP32 VALU work (no MFMA on this path) against the 157.3 TFLOP/s FP32 figure; "
+                        f"F = N_pair*(80+16A), N_pair={int(survivors)} per launch"}
+    bytes_ = 128 * visits + 4 * entries
+    t = stage_ms[STAGES.index(dom)] * 1e-3
+    name = {"tile_lists": "tileListKernel", "resolve": "resolveKernel<4>"}.get(dom, dom)
+    return {"kernel": name, "bound": "hbm", "achieved": round(bytes_ / t / 1e9, 3), "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s", "frac": round(bytes_ / t / PEAK_HBM, 5), "traffic": measured_traffic(name),
+            "note": f"latency-bound traversal; algorithmic bytes = 128 B x {int(visits)} node visits + 4 B x "
+                    f"{int(entries)} list entries per launch"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +147,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--pipeline", choices=["plain", "batched"], default="plain",
+                    help="plain: one view after the other on one stream; batched: both eyes in one "
+                         "submission (hvvr_gpu_render_views), overlapping on separate streams")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
                          "workload for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -115,15 +173,22 @@ def main():
     gvs = [H.GpuView(ctx, v, A) for v in views]
 
     def frame():
-        for gv in gvs:
-            gv.render(gs)
+        if args.pipeline == "batched":
+            H.render_views(gs, gvs)
+        else:
+            for gv in gvs:
+                gv.render(gs)
 
     for _ in range(args.warmup):
         frame()
     ctx.synchronize()
 
-    # timed region: K stereo frames, each synchronised so its latency is measured too
+    # timed region: K stereo frames, each synchronised so its latency is measured too; the
+    # views record per-stage HIP events on the library's stream (no extra synchronisation),
+    # read after each frame's synchronise
+    ctx.set_profiling(True)
     lat = []
+    stage_ms = np.zeros(len(STAGES))
     ctx.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -131,22 +196,15 @@ def main():
         frame()
         ctx.synchronize()
         lat.append(time.perf_counter() - f0)
+        for gv in gvs:
+            stage_ms += gv.stage_times()
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
+    ctx.set_profiling(False)
     samples_per_frame = EYES * W * HGT * A
     value = samples_per_frame * args.steps / elapsed / 1e6
     lat_ms = np.array(lat) * 1e3
-
-    # per-stage kernel times from HIP events on the library's stream (outside the timed region)
-    ctx.set_profiling(True)
-    stage_ms = np.zeros(len(STAGES))
-    prof_frames = max(5, min(args.steps, 20))
-    for _ in range(prof_frames):
-        for gv in gvs:
-            gv.render(gs)
-            stage_ms += gv.stage_times()
-    ctx.set_profiling(False)
-    stage_ms /= prof_frames * EYES  # average per launch (one launch per eye)
+    stage_ms /= args.steps * EYES  # average per launch (one launch per eye)
     stats = [gv.stats() for gv in gvs]
     survivors = sum(s.survivorPairs for s in stats) / EYES
     entries = sum(s.listEntries for s in stats) / EYES
@@ -154,24 +212,8 @@ def main():
         if s.overflow:
             raise SystemExit("traversal budget overflow")
 
-    # rooflines: intersect is FP32-VALU bound (F = N_pair (80 + 16 A)); the traversal is
-    # latency bound, reported against HBM with 128 B per list entry written+read as a floor
-    dom = int(np.argmax(stage_ms))
-    flops_intersect = survivors * (80 + 16 * A)
-    rl_intersect = flops_intersect / (stage_ms[STAGES.index("intersect")] * 1e-3)
-    roof = {"kernel": "intersect", "bound": "mfma", "achieved": round(rl_intersect / 1e12, 3),
-            "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(rl_intersect / PEAK_FP32, 4),
-            "traffic": None,
-            "note": "FP32 vector (non-MFMA) peak; F_int = N_pair*(80+16A) per launch, "
-                    f"N_pair={int(survivors)}"}
-    if STAGES[dom] != "intersect":
-        # traversal-dominated: bytes = BVH node reads are not counted by the kernel, so the
-        # algorithmic floor is the list written (count pass) + written/read (emit pass)
-        list_bytes = entries * 4
-        ach = list_bytes / (stage_ms[dom] * 1e-3)
-        roof = {"kernel": STAGES[dom], "bound": "hbm", "achieved": round(ach / 1e9, 3), "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": round(ach / PEAK_HBM, 6), "traffic": None,
-                "note": f"latency-bound traversal; algorithmic bytes = 4 B x {int(entries)} list entries"}
+    visits = sum(s.nodeVisits for s in stats) / EYES
+    roof = roofline(stage_ms, survivors, entries, visits)
 
     out = {
         "metric": "Msamples/s shaded at 2x1080x1200 MSAA4, 262k tris; p99 frame latency (ms)",
@@ -188,11 +230,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded room scene (1,024 meshes x 256 triangles), grid samples with barrel distortion",
-        "config": {"workload": "stereo 2x1080x1200, MSAA 4, 262,144 triangles, plain per-view pipeline",
-                   "subsamples_per_frame": samples_per_frame, "pipeline": "plain"},
+        "config": {"workload": f"stereo 2x1080x1200, MSAA 4, 262,144 triangles, {args.pipeline} pipeline",
+                   "subsamples_per_frame": samples_per_frame, "pipeline": args.pipeline},
         "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage_ms)},
         "list_entries_per_eye": int(entries),
         "survivor_pairs_per_eye": int(survivors),
+        "node_visits_per_eye": int(visits),
+        "node_tests_per_s": round(visits * 4 / (stage_ms[STAGES.index("tile_lists")] * 1e-3), 1),
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -25,7 +25,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 
 __all__ = [
     "HvvrError", "DTYPES", "load_gpu", "load_host", "HostScene", "HostView", "GpuContext", "GpuScene",
-    "GpuView", "subsample_offsets", "grid_samples", "transform_matrix",
+    "GpuView", "render_views", "subsample_offsets", "lens_table", "grid_samples", "transform_matrix",
 ]
 
 
@@ -219,6 +219,13 @@ def subsample_offsets(msaa_rate, spin=0.0):
     return out
 
 
+def lens_table(msaa_rate):
+    """Depth-of-field lens-position table: [4 rows][msaa_rate][128 samples] x (u, v)."""
+    out = np.zeros((4 * 128 * msaa_rate, 2), np.float32)
+    _check_host(load_host().hvvr_host_lens_table(c_uint32(msaa_rate), _ptr(out)))
+    return out
+
+
 def grid_samples(width, height, k1=0.0, k2=0.0):
     out = np.zeros(width * height, DTYPES["sample"])
     _check_host(load_host().hvvr_host_grid_samples(c_uint32(width), c_uint32(height), c_float(k1), c_float(k2),
@@ -253,7 +260,8 @@ class _ViewDesc(ctypes.Structure):
 
 class ViewStats(ctypes.Structure):
     _fields_ = [("listEntries", c_uint64), ("survivorPairs", c_uint64), ("occupiedTiles", c_uint32),
-                ("tileCount", c_uint32), ("maxTileStack", c_uint32), ("overflow", c_uint32)]
+                ("tileCount", c_uint32), ("maxTileStack", c_uint32), ("overflow", c_uint32),
+                ("nodeVisits", c_uint64)]
 
 
 class GpuContext:
@@ -331,6 +339,20 @@ class GpuScene:
         if self.handle:
             self.lib.hvvr_gpu_scene_destroy(self.handle)
             self.handle = c_void_p()
+
+
+def render_views(scene, views, cameras=None, jitters=None):
+    """Several views of one scene in one submission (hvvr_gpu_render_views): independent
+    views run concurrently on the context's view streams."""
+    n = len(views)
+    if n == 0:
+        return
+    lib = views[0].lib
+    ms = np.stack([np.ascontiguousarray(v.view.camera_to_world if cameras is None or cameras[i] is None
+                                        else cameras[i], np.float32).reshape(16) for i, v in enumerate(views)])
+    js = None if jitters is None else np.ascontiguousarray(jitters, np.float32).reshape(n, 2)
+    handles = (c_void_p * n)(*[v.handle.value for v in views])
+    _check_gpu(lib.hvvr_gpu_render_views(scene.handle, c_uint32(n), handles, _ptr(ms), _ptr(js)))
 
 
 class GpuView:

@@ -6,6 +6,10 @@
 
 #include "hvvr/types.h"
 
+#include <math.h>
+#include <random>
+#include <vector>
+
 namespace hvvr {
 
 // Mara et al. 2016 spiral-turn table (raycaster/gpu_samples.h:23-34); a published
@@ -25,6 +29,28 @@ inline void subsampleOffsets(uint32_t msaaRate, float spinAngle, vector2* out) {
         double angle = double(alpha) * double(turns) + double(spinAngle);
         double radius = sqrt(double(alpha));
         out[i] = vector2(float(cos(angle) * radius), float(sin(angle) * radius));
+    }
+}
+
+// Depth-of-field lens-position table (GPUCamera::initLookupTables, raycaster/gpu_camera.cu:59-88).
+inline void lensPositionTable(uint32_t msaaRate, std::vector<vector2>& out) {
+    // 4 rows x 128 samples x A subsamples, entry [row * 128 * A + subsample * 128 + sample]:
+    // a random rotation per sample from a default-seeded mt19937, subsample (7s+7) mod A
+    // of the rotated spiral (gpu_camera.cu:65-85). The uniform draw is mt19937's 32-bit
+    // output * 2^-32 (this library's mapping; the reference's
+    // uniform_real_distribution<float> bits are implementation-defined).
+    const uint32_t tiles = 4;
+    out.assign(size_t(tiles) * TILE_SIZE * msaaRate, vector2());
+    std::mt19937 gen;
+    std::vector<vector2> spiral(msaaRate);
+    for (uint32_t tile = 0; tile < tiles; tile++) {
+        for (uint32_t sample = 0; sample < TILE_SIZE; sample++) {
+            float rotation = float(double(gen()) * (1.0 / 4294967296.0)) * 6.283185307180f;
+            subsampleOffsets(msaaRate, rotation, spiral.data());
+            for (uint32_t sub = 0; sub < msaaRate; sub++)
+                out[size_t(tile) * TILE_SIZE * msaaRate + size_t(sub) * TILE_SIZE + sample] =
+                    spiral[(sub * 7 + 7) % msaaRate];
+        }
     }
 }
 

@@ -135,12 +135,16 @@ int hvvr_gpu_context_create(int device, hvvr_gpu_context** out) {
     HVVR_HIP_CHECK(hipSetDevice(device));
     hvvr_gpu_context* c = new hvvr_gpu_context();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return fail("hipStreamCreate failed", HVVR_GPU_ERR_HIP);
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    for (auto& vs : c->viewStreams)
+        ok = ok && hipStreamCreateWithFlags(&vs, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+    for (auto& j : c->joins)
+        ok = ok && hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        hvvr_gpu_context_destroy(c);
+        return fail("stream or event creation failed", HVVR_GPU_ERR_HIP);
     }
-    for (auto& e : c->stageEvents)
-        (void)hipEventCreate(&e);
     *out = c;
     return HVVR_GPU_OK;
 }
@@ -149,11 +153,20 @@ int hvvr_gpu_context_destroy(hvvr_gpu_context* c) {
     if (!c)
         return HVVR_GPU_OK;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    for (auto& e : c->stageEvents)
-        if (e)
-            (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
+    for (auto& vs : c->viewStreams)
+        if (vs) {
+            (void)hipStreamSynchronize(vs);
+            (void)hipStreamDestroy(vs);
+        }
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    if (c->fork)
+        (void)hipEventDestroy(c->fork);
+    for (auto& j : c->joins)
+        if (j)
+            (void)hipEventDestroy(j);
     delete c;
     return HVVR_GPU_OK;
 }
@@ -385,7 +398,9 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->gbuffer.resize(size_t(v->sampleCount) * v->msaaRate));
     step(v->sampleResults.resize(v->sampleCount));
     step(v->image.resize(size_t(v->width) * v->height));
-    step(v->counters.resize(4));
+    step(v->counters.resize(8));
+    for (auto& ev : v->stageEvents)
+        step(hipEventCreate(&ev));
     if (e == hipSuccess)
         e = hipMemsetAsync(v->image.ptr, 0, v->image.bytes(), st);
     if (e == hipSuccess)
@@ -424,9 +439,8 @@ int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
     return HVVR_GPU_OK;
 }
 
-static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter) {
+static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter, hipStream_t st) {
     hvvr_gpu_context* c = v->ctx;
-    hipStream_t st = c->stream;
     matrix4x4 m;
     memcpy(&m, m16, sizeof(m));
     v->lastCameraToWorld = m;
@@ -435,7 +449,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     bool prof = c->profiling;
     auto mark = [&](int i) {
         if (prof)
-            (void)hipEventRecord(c->stageEvents[i], st);
+            (void)hipEventRecord(v->stageEvents[i], st);
     };
     HVVR_HIP_CHECK(hipMemsetAsync(v->counters.ptr, 0, v->counters.bytes(), st));
     HVVR_HIP_CHECK(hipMemsetAsync(v->overflowTiles.ptr, 0, v->overflowTiles.bytes(), st));
@@ -455,11 +469,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     launchRemap(v, st);
     mark(7);
     HVVR_HIP_CHECK(hipGetLastError());
-    if (prof) {
-        HVVR_HIP_CHECK(hipEventSynchronize(c->stageEvents[kStageCount]));
-        for (int i = 0; i < kStageCount; i++)
-            (void)hipEventElapsedTime(&v->stageMs[i], c->stageEvents[i], c->stageEvents[i + 1]);
-    }
+    v->stagesRecorded = prof;
     v->rendered = true;
     return HVVR_GPU_OK;
 }
@@ -474,7 +484,7 @@ int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, 
         if (rc)
             return rc;
     }
-    return renderOne(v, s, m16, jitter);
+    return renderOne(v, s, m16, jitter, v->ctx->stream);
 }
 
 int hvvr_gpu_render_views(hvvr_gpu_scene* s, uint32_t count, hvvr_gpu_view* const* views, const float* m16,
@@ -482,9 +492,44 @@ int hvvr_gpu_render_views(hvvr_gpu_scene* s, uint32_t count, hvvr_gpu_view* cons
     if (!s || (count && (!views || !m16)))
         return fail("null argument");
     for (uint32_t i = 0; i < count; i++) {
-        int rc = hvvr_gpu_view_render(views[i], s, m16 + 16 * size_t(i), jitter ? jitter + 2 * size_t(i) : nullptr);
+        if (!views[i])
+            return fail("null view");
+        if (views[i]->ctx != s->ctx)
+            return fail("view and scene belong to different contexts");
+    }
+    if (count == 0)
+        return HVVR_GPU_OK;
+    hvvr_gpu_context* c = s->ctx;
+    if (!s->everUpdated || s->dirty) {
+        int rc = hvvr_gpu_scene_update(s);
         if (rc)
             return rc;
+    }
+    if (count == 1)
+        return renderOne(views[0], s, m16, jitter, c->stream);
+    // Independent views run on separate streams so one view's latency-bound traversal
+    // overlaps another's compute-bound intersection; a view listed twice stays on one
+    // stream. Everything forks after the scene update and joins back before later work
+    // on the context stream.
+    const uint32_t nStreams = count < uint32_t(kViewStreams) ? count : uint32_t(kViewStreams);
+    HVVR_HIP_CHECK(hipEventRecord(c->fork, c->stream));
+    for (uint32_t k = 0; k < nStreams; k++)
+        HVVR_HIP_CHECK(hipStreamWaitEvent(c->viewStreams[k], c->fork, 0));
+    for (uint32_t i = 0; i < count; i++) {
+        uint32_t first = i;
+        for (uint32_t j = 0; j < i; j++)
+            if (views[j] == views[i]) {
+                first = j;
+                break;
+            }
+        int rc = renderOne(views[i], s, m16 + 16 * size_t(i), jitter ? jitter + 2 * size_t(i) : nullptr,
+                           c->viewStreams[first % nStreams]);
+        if (rc)
+            return rc;
+    }
+    for (uint32_t k = 0; k < nStreams; k++) {
+        HVVR_HIP_CHECK(hipEventRecord(c->joins[k], c->viewStreams[k]));
+        HVVR_HIP_CHECK(hipStreamWaitEvent(c->stream, c->joins[k], 0));
     }
     return HVVR_GPU_OK;
 }
@@ -605,7 +650,7 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     int rc = viewSync(v);
     if (rc && rc != HVVR_GPU_ERR_OVERFLOW)
         return rc;
-    uint64_t counters[4];
+    uint64_t counters[8];
     HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
     std::vector<uint32_t> counts(v->tileCount);
     HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
@@ -617,6 +662,7 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     st->survivorPairs = counters[0];
     st->overflow = counters[1] ? 1u : 0u;
     st->maxTileStack = uint32_t(counters[2]);
+    st->nodeVisits = counters[4];
     st->tileCount = v->tileCount;
     for (uint32_t c : counts)
         st->occupiedTiles += c ? 1u : 0u;
@@ -626,9 +672,12 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
 int hvvr_gpu_view_stage_times(hvvr_gpu_view* v, float* ms, int maxCount) {
     if (!v || !ms)
         return fail("null argument");
+    if (!v->stagesRecorded)
+        return fail("no profiled frame: enable profiling on the context before rendering", HVVR_GPU_ERR_STATE);
+    HVVR_HIP_CHECK(hipEventSynchronize(v->stageEvents[kStageCount]));
     int n = std::min(maxCount, int(kStageCount));
     for (int i = 0; i < n; i++)
-        ms[i] = v->stageMs[i];
+        HVVR_HIP_CHECK(hipEventElapsedTime(&ms[i], v->stageEvents[i], v->stageEvents[i + 1]));
     return n;
 }
 

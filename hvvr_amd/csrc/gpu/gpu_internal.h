@@ -50,6 +50,8 @@ struct DeviceBuffer {
 };
 
 enum { kStageCount = 7 };
+// streams a batched render spreads independent views over (all at default priority)
+enum { kViewStreams = 4 };
 
 struct RefitLevel {
     uint32_t start, count;
@@ -60,8 +62,11 @@ struct RefitLevel {
 struct hvvr_gpu_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool profiling = false;
-    hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
+    bool profiling = false; // views record per-stage HIP events on the stream
+    // batched renders: views fan out from `stream` onto these and join back
+    hipStream_t viewStreams[hvvr::kViewStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t joins[hvvr::kViewStreams] = {};
 };
 
 struct hvvr_gpu_scene {
@@ -107,11 +112,17 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<int32_t> remap;                  // [width*height*split]
     hvvr::DeviceBuffer<uint32_t> image;                 // [width*height]
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
-                                                        // [3] list entries allocated
+                                                        // [3] list entries allocated, [4] node visits
     uint64_t listCapacity = 0;
     uint32_t waveStackLimit = 256; // LDS stack entries per tile traversal (<= 256)
-    float stageMs[hvvr::kStageCount] = {};
+    hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
+    bool stagesRecorded = false;
     bool rendered = false;
+    ~hvvr_gpu_view() {
+        for (auto& ev : stageEvents)
+            if (ev)
+                (void)hipEventDestroy(ev);
+    }
 };
 
 namespace hvvr {

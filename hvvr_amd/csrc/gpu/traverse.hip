@@ -16,6 +16,7 @@
 //                             LDS (count pass, allocation, emit pass)
 #include "device_common.h"
 #include "gpu_internal.h"
+#include "reg_stack.h"
 #include "wave_traverse.h"
 
 namespace hvvr {
@@ -189,13 +190,36 @@ struct TileListLds {
     uint32_t recMeta[TL_REC_CAP];  // first child record (bits 0..15), child mask (16..19), leaf
                                    // mask (20..23), stack-entry leaf mask (24..27)
     uint16_t order[TL_REC_CAP];    // records with emitted leaves, in emission order
-    StackEntry stack[TL_STACK_CAP]; // tMin, record index
 };
 
 __device__ inline uint32_t metaFirst(uint32_t m) { return m & 0xffffu; }
 __device__ inline uint32_t metaMask(uint32_t m) { return (m >> 16) & 0xfu; }
 __device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 20) & 0xfu; }
 __device__ inline uint32_t metaEntry(uint32_t m) { return m >> 24; }
+
+// One child box of a node, as one lane of a 4-lane node group loads it.
+struct ChildBox {
+    float b0, b1, b2, b3, b4, b5; // bounds[k][child]
+    float key;                    // distance key: -bounds[distanceAxis][child]
+    uint32_t leafMask;            // the node's leaf mask
+    uint32_t childOffset;         // data[1 + child]
+};
+
+__device__ inline ChildBox loadChildBox(const BVHNode* __restrict__ nodes, uint32_t node, uint32_t c, uint32_t axis) {
+    const BVHNode* __restrict__ nd = nodes + node;
+    const float* bb = &nd->bounds[0][0];
+    ChildBox x;
+    x.b0 = bb[c];
+    x.b1 = bb[4 + c];
+    x.b2 = bb[8 + c];
+    x.b3 = bb[12 + c];
+    x.b4 = bb[16 + c];
+    x.b5 = bb[20 + c];
+    x.key = -bb[axis * 4 + c];
+    x.leafMask = nd->leafMask;
+    x.childOffset = nd->data[1 + c];
+    return x;
+}
 
 // number of triangles in the leaves `leaf` of `node`
 __device__ inline uint32_t leafListLength(const BVHNode* __restrict__ nodes, uint32_t node, uint32_t leaf) {
@@ -235,6 +259,7 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
     bool overflow = n0 > limit;
     uint32_t deepest = n0; // deepest stack size seen
     uint32_t nEmit = 0;
+    uint32_t nrec = n0; // records
 
     // ---- phase A: node tests, breadth-first ----
     if (!overflow) {
@@ -244,53 +269,66 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
             sh.recNode[lane] = e.nodeMask & STACK_NODE_MASK;
             sh.recKey[lane] = fbits(e.tMin);
             sh.recMeta[lane] = (e.nodeMask >> 28) << 24;
-            StackEntry se;
-            se.tMin = e.tMin;
-            se.nodeMask = lane;
-            sh.stack[lane] = se;
         }
         waveLdsSync();
         const uint32_t s = lane >> 2, c = lane & 3u;
         const uint32_t axis = uint32_t(f->distanceAxis);
         const uint64_t groupBelow = (1ull << (4 * s)) - 1ull;
-        uint32_t nrec = n0;
         for (uint32_t head = 0; head < nrec;) {
-            // this step tests records [head, end); the children it appends start at nrec
-            const uint32_t end = head + 16 < nrec ? head + 16 : nrec;
-            const uint32_t r = head + s;
-            const bool valid = r < end;
-            uint32_t node = 0, entry = 0;
-            if (valid) {
-                node = sh.recNode[r];
-                entry = metaEntry(sh.recMeta[r]);
+            // this step tests records [head, end), two per lane group (so their node loads
+            // overlap); the children it appends start at nrec
+            const uint32_t end = head + 32 < nrec ? head + 32 : nrec;
+            const uint32_t rA = head + s, rB = head + 16 + s;
+            const bool validA = rA < end, validB = rB < end;
+            uint32_t nodeA = 0, entryA = 0, nodeB = 0, entryB = 0;
+            if (validA) {
+                nodeA = sh.recNode[rA];
+                entryA = metaEntry(sh.recMeta[rA]);
             }
-            const BVHNode* __restrict__ nd = nodes + node;
-            const float* bb = &nd->bounds[0][0];
-            const float b0 = bb[c], b1 = bb[4 + c], b2 = bb[8 + c], b3 = bb[12 + c], b4 = bb[16 + c],
-                        b5 = bb[20 + c];
-            const float key = -bb[axis * 4 + c];
-            const uint32_t leafMask = nd->leafMask;
-            const uint32_t childOffset = nd->data[1 + c];
-            bool pass = valid && childCoarsePass(f, b0, b1, b2, b3, b4, b5);
-            uint32_t mask = uint32_t(__ballot(pass) >> (4 * s)) & 0xfu;
+            if (validB) {
+                nodeB = sh.recNode[rB];
+                entryB = metaEntry(sh.recMeta[rB]);
+            }
+            const ChildBox A = loadChildBox(nodes, nodeA, c, axis);
+            const ChildBox B = loadChildBox(nodes, nodeB, c, axis);
+            bool passA = validA && childCoarsePass(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
+            bool passB = validB && childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+            uint32_t maskA = uint32_t(__ballot(passA) >> (4 * s)) & 0xfu;
+            uint32_t maskB = uint32_t(__ballot(passB) >> (4 * s)) & 0xfu;
             // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
-            const bool refine = (mask & leafMask) != 0u;
-            if (pass && refine)
-                pass = !childRefineCull(f, b0, b1, b2, b3, b4, b5);
-            mask = uint32_t(__ballot(pass) >> (4 * s)) & 0xfu;
-            // internal children are pushed only from entries without a leaf mask (traverse_avx.h:476-483)
-            const bool internal = pass && ((leafMask >> c) & 1u) == 0u && entry == 0u;
-            const uint64_t bi = __ballot(internal);
-            const uint32_t slot = nrec + lanesBelow(bi);
-            const uint32_t first = nrec + uint32_t(__builtin_popcountll(bi & groupBelow));
-            if (internal && slot < TL_REC_CAP) {
-                sh.recNode[slot] = node + childOffset;
-                sh.recKey[slot] = fbits(key);
-                sh.recMeta[slot] = 0u;
+            if (passA && (maskA & A.leafMask) != 0u)
+                passA = !childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
+            if (passB && (maskB & B.leafMask) != 0u)
+                passB = !childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+            maskA = uint32_t(__ballot(passA) >> (4 * s)) & 0xfu;
+            maskB = uint32_t(__ballot(passB) >> (4 * s)) & 0xfu;
+            // internal children are pushed only from entries without a leaf mask
+            // (traverse_avx.h:476-483)
+            const bool innerA = passA && ((A.leafMask >> c) & 1u) == 0u && entryA == 0u;
+            const bool innerB = passB && ((B.leafMask >> c) & 1u) == 0u && entryB == 0u;
+            const uint64_t biA = __ballot(innerA), biB = __ballot(innerB);
+            const uint32_t cntA = uint32_t(__builtin_popcountll(biA));
+            const uint32_t slotA = nrec + lanesBelow(biA);
+            const uint32_t slotB = nrec + cntA + lanesBelow(biB);
+            if (innerA && slotA < TL_REC_CAP) {
+                sh.recNode[slotA] = nodeA + A.childOffset;
+                sh.recKey[slotA] = fbits(A.key);
+                sh.recMeta[slotA] = 0u;
             }
-            if (valid && c == 0)
-                sh.recMeta[r] = first | (mask << 16) | (leafMask << 20) | (entry << 24);
-            nrec += uint32_t(__builtin_popcountll(bi));
+            if (innerB && slotB < TL_REC_CAP) {
+                sh.recNode[slotB] = nodeB + B.childOffset;
+                sh.recKey[slotB] = fbits(B.key);
+                sh.recMeta[slotB] = 0u;
+            }
+            if (validA && c == 0) {
+                const uint32_t first = nrec + uint32_t(__builtin_popcountll(biA & groupBelow));
+                sh.recMeta[rA] = first | (maskA << 16) | (A.leafMask << 20) | (entryA << 24);
+            }
+            if (validB && c == 0) {
+                const uint32_t first = nrec + cntA + uint32_t(__builtin_popcountll(biB & groupBelow));
+                sh.recMeta[rB] = first | (maskB << 16) | (B.leafMask << 20) | (entryB << 24);
+            }
+            nrec += cntA + uint32_t(__builtin_popcountll(biB));
             head = end;
             waveLdsSync();
             if (nrec > TL_REC_CAP) {
@@ -300,11 +338,25 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
         }
     }
 
-    // ---- phase B: the reference's stack walk over the records ----
+    // ---- phase B: the reference's stack walk over the records, stack in VGPRs ----
     if (!overflow) {
-        uint32_t top = n0 - 1;
-        uint32_t cur = uniform(sh.stack[top].nodeMask);
+        RegStack st;
+        // block stack entry n0-1 is the top (depth 0)
+        const uint32_t rootRec = n0 - 1 - lane;
+        st.t0 = lane < n0 ? bitsf(sh.recKey[lane < n0 ? rootRec : 0]) : 0.0f;
+        st.s0 = lane < n0 ? rootRec : 0u;
+        st.t1 = st.t2 = st.t3 = 0.0f;
+        st.s1 = st.s2 = st.s3 = 0u;
+        st.size = n0;
+        uint32_t cur = stackPop(st);
+        uint32_t steps = 0;
         for (;;) {
+            // every record is reached exactly once; the bound keeps the loop finite whatever
+            // the records hold
+            if (++steps > nrec) {
+                overflow = true;
+                break;
+            }
             const uint32_t meta = uniform(sh.recMeta[cur]);
             const uint32_t mask = metaMask(meta), leafMask = metaLeaf(meta);
             bool pop = true;
@@ -322,54 +374,28 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
                         cur = first;
                         continue;
                     }
+                    // children in child order, each inserted below the run of closer entries
+                    // (traverse_avx.h:499-520)
                     const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
+                    const uint32_t keys = lane < nChildren ? sh.recKey[first + lane] : 0u;
                     for (uint32_t j = 0; j < nChildren; j++) {
-                        if (j)
-                            top++;
-                        if (top >= limit) {
+                        if (st.size >= limit) {
                             overflow = true;
                             break;
                         }
-                        const float dist = bitsf(uniform(sh.recKey[first + j]));
-                        // entries above the insertion point are the contiguous run from the
-                        // top whose distance is smaller than `dist`; shift them up by one
-                        uint32_t pos = top;
-                        for (;;) {
-                            const int below = int(pos) - 1 - int(lane);
-                            const bool smaller = below >= 0 && sh.stack[below >= 0 ? below : 0].tMin < dist;
-                            const uint64_t run = ~__ballot(smaller);
-                            const uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
-                            StackEntry moved;
-                            if (lane < n)
-                                moved = sh.stack[below];
-                            waveLdsSync();
-                            if (lane < n)
-                                sh.stack[below + 1] = moved;
-                            waveLdsSync();
-                            pos -= n;
-                            if (n < 64u)
-                                break;
-                        }
-                        if (lane == 0) {
-                            StackEntry ne;
-                            ne.tMin = dist;
-                            ne.nodeMask = first + j;
-                            sh.stack[pos] = ne;
-                        }
-                        waveLdsSync();
+                        stackInsert(st, bitsf(readLane(keys, j)), first + j, lane);
                     }
                     if (overflow)
                         break;
-                    deepest = top + 1 > deepest ? top + 1 : deepest;
-                    cur = uniform(sh.stack[top].nodeMask);
+                    deepest = st.size > deepest ? st.size : deepest;
+                    cur = stackPop(st);
                     pop = false;
                 }
             }
             if (pop) {
-                if (top == 0)
+                if (st.size == 0)
                     break;
-                top--;
-                cur = uniform(sh.stack[top].nodeMask);
+                cur = stackPop(st);
             }
         }
     }
@@ -407,6 +433,7 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
         if (!fits)
             atomicOr(&counters[1], 1ull);
         atomicMax(&counters[2], (unsigned long long)deepest);
+        atomicAdd(&counters[4], (unsigned long long)nrec); // node visits
     }
     if (!fits)
         return;
@@ -468,8 +495,10 @@ __global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restri
     uint32_t entryMask = S(top).nodeMask >> 28;
     bool overflow = false;
     int axis = f.distanceAxis;
+    uint32_t visits = 0;
     for (;;) {
         const BVHNode& n = nodes[node];
+        visits++;
         uint32_t mask = testChildren(f, n);
         bool pop = true;
         if (mask) {
@@ -546,6 +575,8 @@ __global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restri
     if (overflow)
         atomicOr(&counters[1], 1ull);
     atomicMax(&counters[2], (unsigned long long)(maxTop + 1));
+    if (!emit)
+        atomicAdd(&counters[4], (unsigned long long)visits); // node visits
 }
 
 // List space for the tiles the global-stack kernel re-traverses, after its count pass.

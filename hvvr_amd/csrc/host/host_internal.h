@@ -63,9 +63,6 @@ matrix3x3 sampleToCamera(vector2 viewportLower, vector2 viewportUpper);
 void buildSampleSetup(const Sample* samples, uint32_t count, uint32_t split, uint32_t width, uint32_t height,
                       const matrix3x3& sampleToCam, ThinLens lens, SampleSetup& out);
 
-// Depth-of-field lens table: 4 tiles x 128 samples x A offsets (raycaster/gpu_camera.cu:59-88).
-void lensPositionTable(uint32_t msaaRate, std::vector<vector2>& out);
-
 // ---------------------------------------------------------------------------------
 // seeded generators
 // ---------------------------------------------------------------------------------
@@ -111,7 +108,7 @@ void generateHeightfieldBoxesScene(uint64_t seed, SceneGenResult& out);
 // 30 x 15 x 30 m room, 4 point lights + 1 directional light.
 void generateRoomScene(uint64_t seed, uint32_t meshCount, SceneGenResult& out);
 
-// Camera presets for the benchmark configurations.
+// Camera presets for the benchmark scenes.
 struct CameraPreset {
     uint32_t width, height;
     vector2 viewportLower, viewportUpper;
@@ -119,9 +116,9 @@ struct CameraPreset {
     transform cameraToWorld;
     ThinLens lens;
 };
-// config 1: pinhole 640x480, 60 degree vertical FOV looking at the heightfield
+// pinhole 640x480, 60 degree vertical FOV looking at the heightfield
 CameraPreset heightfieldCamera();
-// config 2: one eye of a 1080x1200 stereo pair with barrel distortion, eye = 0 or 1
+// one eye of a 1080x1200 stereo pair with barrel distortion (k1 0.22, k2 0.24), eye = 0 or 1
 CameraPreset roomEyeCamera(uint32_t eye, uint32_t width, uint32_t height);
 
 } // namespace hvvr

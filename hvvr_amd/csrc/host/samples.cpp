@@ -247,24 +247,4 @@ void buildSampleSetup(const Sample* raw, uint32_t count, uint32_t split, uint32_
     }
 }
 
-void lensPositionTable(uint32_t msaaRate, std::vector<vector2>& out) {
-    // 4 tiles x 128 samples x A subsamples; random rotation per sample from a default-seeded
-    // mt19937, subsample (7s+7) mod A of the spiral (gpu_camera.cu:65-85). The
-    // uniform draw is mt19937's 32-bit output * 2^-32 (this library's mapping; the
-    // reference's uniform_real_distribution<float> bits are implementation-defined).
-    const uint32_t tiles = 4;
-    out.assign(size_t(tiles) * TILE_SIZE * msaaRate, vector2());
-    std::mt19937 gen;
-    std::vector<vector2> spiral(msaaRate);
-    for (uint32_t tile = 0; tile < tiles; tile++) {
-        for (uint32_t sample = 0; sample < TILE_SIZE; sample++) {
-            float rotation = float(double(gen()) * (1.0 / 4294967296.0)) * 6.283185307180f;
-            subsampleOffsets(msaaRate, rotation, spiral.data());
-            for (uint32_t sub = 0; sub < msaaRate; sub++)
-                out[size_t(tile) * TILE_SIZE * msaaRate + size_t(sub) * TILE_SIZE + sample] =
-                    spiral[(sub * 7 + 7) % msaaRate];
-        }
-    }
-}
-
 } // namespace hvvr

@@ -48,7 +48,8 @@ const char* hvvr_gpu_version(void);
 int hvvr_gpu_context_create(int device, hvvr_gpu_context** out);
 int hvvr_gpu_context_destroy(hvvr_gpu_context* ctx);
 int hvvr_gpu_context_synchronize(hvvr_gpu_context* ctx);
-/* 1 = record per-stage HIP events for hvvr_gpu_view_stage_times */
+/* 1 = every render records per-stage HIP events on the stream (no synchronisation), read
+ * back with hvvr_gpu_view_stage_times */
 int hvvr_gpu_context_set_profiling(hvvr_gpu_context* ctx, int enable);
 
 /* ---- scene: replaces GPUSceneState (raycaster/gpu_scene_state.h:31-73) ---- */
@@ -137,9 +138,11 @@ typedef struct {
     uint32_t tileCount;
     uint32_t maxTileStack;     /* deepest tile traversal stack seen */
     uint32_t overflow;         /* nonzero if a budget overflowed */
+    uint64_t nodeVisits;       /* BVH nodes tested by the tile traversal (4 child boxes each) */
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
-/* Per-stage GPU milliseconds of the last profiled frame (needs profiling enabled):
+/* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled
+ * (waits for that frame's last event):
  * [0] frustum setup, [1] block traversal, [2] tile lists, [3] tile lists of tiles that
  * did not fit in LDS, [4] intersect, [5] resolve, [6] remap. Returns the count. */
 int hvvr_gpu_view_stage_times(hvvr_gpu_view* view, float* ms, int maxCount);

@@ -85,6 +85,17 @@ int64_t or_tile_traverse(const o_node* nodes, const o_stack* blockStack, uint32_
 int64_t or_build_tile_lists(const o_node* nodes, const o_frustum* blockPre, const o_frustum* tilePre,
                             uint32_t blockCount, uint32_t* counts, uint32_t* out, uint64_t cap);
 
+/* ---- depth of field ---- */
+/* Thin lens (ThinLens, raycaster/camera.h) and the lens-position table of
+ * GPUCamera::initLookupTables (raycaster/gpu_camera.cu:59-88): 4 rows x 128 samples x A
+ * subsamples, entry [row * 128 * A + subsample * 128 + sample]. A tile uses row
+ * (rank of the tile among the tiles with a non-empty list) % 4, as the reference's
+ * kernels index it with their compacted-tile block index. */
+typedef struct {
+    float radius, focalDistance;
+    const ov2* lensTable;
+} o_dof;
+
 /* ---- intersect ---- */
 /* Intersects one view. worldTiles: world-space tile frusta; lists as from
  * or_build_tile_lists. Outputs per subsample (sample*A + i): triangle id (0xffffffff =
@@ -94,6 +105,19 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
                       const float m[16], const ov2* offsets, uint32_t A, const uint32_t* counts,
                       const uint32_t* lists, const o_tri* tris, uint32_t* triOut, float* depthOut,
                       uint64_t* survivorsOut);
+/* or_intersect_view with depth of field: IntersectTriangleTile<true>, IntersectTriangleThreadDoF,
+ * TileData<true>, GetSampleUVsDoF (prim_tests.h:247-392, tile_data.h:37-60,
+ * gpu_samples.h:84-103). dof == NULL is the pinhole path. */
+int or_intersect_view_dof(const o_beam* beams, uint32_t sampleCount, const o_sfrustum* worldTiles,
+                          uint32_t tileCount, const float m[16], const ov2* offsets, uint32_t A,
+                          const uint32_t* counts, const uint32_t* lists, const o_tri* tris, const o_dof* dof,
+                          uint32_t* triOut, float* depthOut, uint64_t* survivorsOut);
+/* The world-space depth-of-field rays the intersect tests, per subsample (sample*A + i):
+ * origin = lens centre + lensUV.x*lensU + lensUV.y*lensV, direction = lensCenterToFocalCenter
+ * + dirUV.x*lensU + dirUV.y*lensV (prim_tests.h:330-333), for tiles with a non-empty
+ * list (zero elsewhere). For tests of the DoF path against brute force. */
+int or_dof_rays(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                const o_dof* dof, const uint32_t* counts, ov3* originsOut, ov3* dirsOut);
 /* Compresses one sample's A subsample ids into (tri, mask) slots; returns the slot count. */
 uint32_t or_emit_gbuffer(const uint32_t* sampleTri, uint32_t A, uint32_t* slotTri, uint32_t* slotMask);
 
@@ -105,6 +129,12 @@ int or_resolve_view(const o_beam* beams, uint32_t sampleCount, const float m[16]
                     const uint32_t* triIds, const o_tri* tris, const o_trishade* shade, const o_vertex* verts,
                     const o_material* materials, const o_lightenv* env, int precision, uint32_t* colorOut,
                     float* linearOut);
+/* or_resolve_view with depth of field (the DoF branch of ShadeMSAA, resolve.cu:161-284,
+ * 324-384); counts: tile list lengths, for the lens-table row of each tile. */
+int or_resolve_view_dof(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets,
+                        uint32_t A, const uint32_t* triIds, const o_tri* tris, const o_trishade* shade,
+                        const o_vertex* verts, const o_material* materials, const o_lightenv* env, int precision,
+                        const o_dof* dof, const uint32_t* counts, uint32_t* colorOut, float* linearOut);
 /* image[y*stride + x] = results[remap[...]] (split 1, 2 or 3) */
 void or_remap(const uint32_t* results, const int32_t* remap, uint32_t width, uint32_t height, uint32_t split,
               uint32_t stride, uint32_t* image);

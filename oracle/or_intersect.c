@@ -14,8 +14,10 @@
  * tests every survivor in order. Chunking does not change the order in which a sample
  * meets the survivors, so this restatement culls the whole list, then tests it in order.
  * FMAs are placed where prim_tests.h:195-229 annotates them ("2 FMA", "1 FMA").
+ * With depth of field (prim_tests.h:247-392, or_dof.h) the tile cull keeps partially
+ * behind-the-lens triangles without the UVW test (intersect.cu:211-236).
  */
-#include "or_math.h"
+#include "or_dof.h"
 #include <stdlib.h>
 
 #define BAD_TRI 0xffffffffu
@@ -135,12 +137,22 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
                       const float m[16], const ov2* offsets, uint32_t A, const uint32_t* counts,
                       const uint32_t* lists, const o_tri* tris, uint32_t* triOut, float* depthOut,
                       uint64_t* survivorsOut) {
+    return or_intersect_view_dof(beams, sampleCount, worldTiles, tileCount, m, offsets, A, counts, lists, tris, NULL,
+                                 triOut, depthOut, survivorsOut);
+}
+
+int or_intersect_view_dof(const o_beam* beams, uint32_t sampleCount, const o_sfrustum* worldTiles,
+                          uint32_t tileCount, const float m[16], const ov2* offsets, uint32_t A,
+                          const uint32_t* counts, const uint32_t* lists, const o_tri* tris, const o_dof* dof,
+                          uint32_t* triOut, float* depthOut, uint64_t* survivorsOut) {
     if (A < 1 || A > 32 || sampleCount < (uint64_t)tileCount * 128) return -1;
+    if (dof && (!(dof->radius > 0.0f) || !dof->lensTable)) return -1;
     ov3 rayOrigin = v3(m[12], m[13], m[14]);
     uint64_t survivors = 0;
     uint64_t listOffset = 0;
     uint32_t* surv = (uint32_t*)malloc(sizeof(uint32_t) * 1);
     uint64_t survCap = 1;
+    uint32_t rank = 0, rankTile = 0;
     for (uint32_t tile = 0; tile < tileCount; tile++) {
         uint32_t n = counts[tile];
         const uint32_t* list = lists + listOffset;
@@ -151,6 +163,13 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
                 depthOut[((uint64_t)tile * 128 + s) * A + i] = FLT_MAX;
             }
         if (n == 0) continue;
+        uint32_t row = dof ? dof_row(counts, tile, &rank, &rankTile) : 0;
+        dof_tile dt;
+        memset(&dt, 0, sizeof(dt));
+        if (dof) {
+            const o_beam* mid = &beams[(uint64_t)tile * 128 + 64];
+            dof_tile_load(m, m3mul(m, mid->du), m3mul(m, mid->dv), dof->radius, dof->focalDistance, &dt);
+        }
         float planes[4][4];
         frustum_planes(&worldTiles[tile], planes);
         if (n > survCap) {
@@ -161,9 +180,16 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
         for (uint32_t k = 0; k < n; k++) {
             const o_tri* tri = &tris[list[k]];
             if (!planes_test(planes, tri)) continue;
-            tri_tile tt;
-            if (!tile_setup(tri, rayOrigin, &tt)) continue;
-            if (!frusta_uvw_test(&worldTiles[tile], tri)) continue;
+            if (dof) {
+                dof_tri d;
+                int r = dof_tri_setup(tri, rayOrigin, dt.lensU, dt.lensV, &d);
+                if (r == 0) continue;
+                if (r == 2 && !frusta_uvw_test(&worldTiles[tile], tri)) continue;
+            } else {
+                tri_tile tt;
+                if (!tile_setup(tri, rayOrigin, &tt)) continue;
+                if (!frusta_uvw_test(&worldTiles[tile], tri)) continue;
+            }
             surv[ns++] = list[k];
         }
         survivors += (uint64_t)ns * 128;
@@ -178,13 +204,27 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
                 depth[i] = FLT_MAX;
                 hit[i] = BAD_TRI;
             }
+            ov3 lcfc = dof ? dof_lens_to_focal(m, dir, dof->focalDistance) : v3(0, 0, 0);
             for (uint32_t k = 0; k < ns; k++) {
-                tri_tile tt;
-                tile_setup(&tris[surv[k]], rayOrigin, &tt);
-                tri_thread th;
-                thread_setup(&tt, dir, du, dv, &th);
-                for (uint32_t i = 0; i < A; i++)
-                    if (thread_test(&th, tt.t, offsets[i], &depth[i])) hit[i] = surv[k];
+                if (dof) {
+                    dof_tri d;
+                    dof_tri_setup(&tris[surv[k]], rayOrigin, dt.lensU, dt.lensV, &d);
+                    ov3 normal = cross3(d.e0, d.e1);
+                    float denomCenter = dot3(neg3(lcfc), normal);
+                    for (uint32_t i = 0; i < A; i++) {
+                        ov2 lensUV, dirUV;
+                        dof_sample_uvs(dof->lensTable, row, A, i, s, offsets[i], dt.focalToLensScale, &lensUV, &dirUV);
+                        if (dof_test(&d, denomCenter, lcfc, dt.lensU, dt.lensV, lensUV, dirUV, &depth[i]))
+                            hit[i] = surv[k];
+                    }
+                } else {
+                    tri_tile tt;
+                    tile_setup(&tris[surv[k]], rayOrigin, &tt);
+                    tri_thread th;
+                    thread_setup(&tt, dir, du, dv, &th);
+                    for (uint32_t i = 0; i < A; i++)
+                        if (thread_test(&th, tt.t, offsets[i], &depth[i])) hit[i] = surv[k];
+                }
             }
             for (uint32_t i = 0; i < A; i++) {
                 triOut[sample * A + i] = hit[i];
@@ -194,6 +234,39 @@ int or_intersect_view(const o_beam* beams, uint32_t sampleCount, const o_sfrustu
     }
     free(surv);
     if (survivorsOut) *survivorsOut = survivors;
+    return 0;
+}
+
+int or_dof_rays(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                const o_dof* dof, const uint32_t* counts, ov3* originsOut, ov3* dirsOut) {
+    if (A < 1 || A > 32 || !dof || !(dof->radius > 0.0f) || !dof->lensTable) return -1;
+    ov3 lensCenter = v3(m[12], m[13], m[14]);
+    uint32_t rank = 0, rankTile = 0;
+    for (uint32_t tile = 0; tile < sampleCount / 128; tile++) {
+        int occupied = counts[tile] != 0;
+        uint32_t row = occupied ? dof_row(counts, tile, &rank, &rankTile) : 0;
+        dof_tile dt;
+        memset(&dt, 0, sizeof(dt));
+        if (occupied) {
+            const o_beam* mid = &beams[(uint64_t)tile * 128 + 64];
+            dof_tile_load(m, m3mul(m, mid->du), m3mul(m, mid->dv), dof->radius, dof->focalDistance, &dt);
+        }
+        for (uint32_t s = 0; s < 128; s++) {
+            uint64_t sample = (uint64_t)tile * 128 + s;
+            ov3 lcfc = dof_lens_to_focal(m, m3mul(m, beams[sample].c), dof->focalDistance);
+            for (uint32_t i = 0; i < A; i++) {
+                ov3 o = v3(0, 0, 0), d = v3(0, 0, 0);
+                if (occupied) {
+                    ov2 lensUV, dirUV;
+                    dof_sample_uvs(dof->lensTable, row, A, i, s, offsets[i], dt.focalToLensScale, &lensUV, &dirUV);
+                    o = axpy2(lensCenter, lensUV.x, dt.lensU, lensUV.y, dt.lensV);
+                    d = axpy2(lcfc, dirUV.x, dt.lensU, dirUV.y, dt.lensV);
+                }
+                originsOut[sample * A + i] = o;
+                dirsOut[sample * A + i] = d;
+            }
+        }
+    }
     return 0;
 }
 

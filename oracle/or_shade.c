@@ -1,6 +1,6 @@
 /* Oracle: resolve (float and double), remap, lighting environment (test infrastructure
  * only). See or_shade_impl.h for the shading body and its citations. */
-#include "or_math.h"
+#include "or_dof.h"
 #include <stdlib.h>
 
 #define R float
@@ -57,21 +57,46 @@ int or_resolve_view(const o_beam* beams, uint32_t sampleCount, const float m[16]
                     const uint32_t* triIds, const o_tri* tris, const o_trishade* shade, const o_vertex* verts,
                     const o_material* materials, const o_lightenv* env, int precision, uint32_t* colorOut,
                     float* linearOut) {
+    return or_resolve_view_dof(beams, sampleCount, m, offsets, A, triIds, tris, shade, verts, materials, env,
+                               precision, NULL, NULL, colorOut, linearOut);
+}
+
+int or_resolve_view_dof(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets,
+                        uint32_t A, const uint32_t* triIds, const o_tri* tris, const o_trishade* shade,
+                        const o_vertex* verts, const o_material* materials, const o_lightenv* env, int precision,
+                        const o_dof* dof, const uint32_t* counts, uint32_t* colorOut, float* linearOut) {
     if (A < 1 || A > 32) return -1;
+    if (dof && (!(dof->radius > 0.0f) || !dof->lensTable || !counts)) return -1;
     ov3 cam = v3(m[12], m[13], m[14]);
+    uint32_t rank = 0, rankTile = 0;
+    dof_sample_ctx dc;
+    memset(&dc, 0, sizeof(dc));
     for (uint32_t s = 0; s < sampleCount; s++) {
         uint32_t slotTri[32], slotMask[32];
         uint32_t n = or_emit_gbuffer(triIds + (uint64_t)s * A, A, slotTri, slotMask);
         ov3 dir = m3mul(m, beams[s].c), du = m3mul(m, beams[s].du), dv = m3mul(m, beams[s].dv);
+        const dof_sample_ctx* pdc = NULL;
+        if (dof) {
+            uint32_t tile = s / 128;
+            if (s % 128 == 0) {
+                const o_beam* mid = &beams[(uint64_t)tile * 128 + 64];
+                dof_tile_load(m, m3mul(m, mid->du), m3mul(m, mid->dv), dof->radius, dof->focalDistance, &dc.tile);
+                dc.row = dof_row(counts, tile, &rank, &rankTile);
+                dc.table = dof->lensTable;
+            }
+            dc.sample = s % 128;
+            dc.lensToFocal = dof_lens_to_focal(m, dir, dof->focalDistance);
+            pdc = &dc;
+        }
         if (precision == 0) {
             float lin[3];
             colorOut[s] = shade_sample_f(dir, du, dv, cam, offsets, A, slotTri, slotMask, n, tris, shade, verts,
-                                         materials, env, lin);
+                                         materials, env, pdc, lin);
             if (linearOut) memcpy(linearOut + 3 * (uint64_t)s, lin, sizeof(lin));
         } else {
             double lin[3];
             colorOut[s] = shade_sample_d(dir, du, dv, cam, offsets, A, slotTri, slotMask, n, tris, shade, verts,
-                                         materials, env, lin);
+                                         materials, env, pdc, lin);
             if (linearOut)
                 for (int c = 0; c < 3; c++) linearOut[3 * (uint64_t)s + c] = (float)lin[c];
         }

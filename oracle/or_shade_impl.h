@@ -2,8 +2,9 @@
  * GPU kernels) and double (the FP64 shading oracle). Included by or_shade.c with
  * R = float|double and SFX = _f|_d. Test infrastructure only.
  *
- *   ShadeMSAA (non-DoF path)           raycaster/resolve.cu:161-284
+ *   ShadeMSAA                          raycaster/resolve.cu:161-284
  *   IntersectTriangleThread::calcUVW   raycaster/prim_tests.h:233-245
+ *   IntersectTriangleThreadDoF::calcUVW raycaster/prim_tests.h:366-392 (FMAs as in its test())
  *   interpolate                        raycaster/shading.h:19-36
  *   GGXShade                           raycaster/shading.h:84-161
  *   GGX_SchlickFresnel_SchlickSmithGGX raycaster/shading_helpers.h:89-112
@@ -121,7 +122,8 @@ static R FN(srgb)(R x) {
 static uint32_t FN(shade_sample)(ov3 dirC, ov3 duC, ov3 dvC, ov3 camPos, const ov2* offsets, uint32_t A,
                                  const uint32_t* slotTri, const uint32_t* slotMask, uint32_t slots,
                                  const o_tri* tris, const o_trishade* shade, const o_vertex* verts,
-                                 const o_material* materials, const o_lightenv* env, R outLinear[3]) {
+                                 const o_material* materials, const o_lightenv* env, const dof_sample_ctx* dc,
+                                 R outLinear[3]) {
     uint32_t maskAll = A < 32 ? ((1u << A) - 1u) : 0xffffffffu;
     RV3 result = FN(mk)(0, 0, 0);
     uint32_t combined = 0;
@@ -134,30 +136,68 @@ static uint32_t FN(shade_sample)(ov3 dirC, ov3 duC, ov3 dvC, ov3 camPos, const o
         combined |= mask;
         R inv = (R)1 / (R)__builtin_popcount(mask);
         R cx = 0, cy = 0;
+        R lx = 0, ly = 0, dx = 0, dy = 0; /* depth of field: centroid lens and direction offsets */
         for (uint32_t i = 0; i < A; i++) {
             if (!(mask & (1u << i))) continue;
             cx = cx + (R)offsets[i].x * inv;
             cy = cy + (R)offsets[i].y * inv;
+            if (dc) {
+                ov2 lensUV, dirUV;
+                dof_sample_uvs(dc->table, dc->row, A, i, dc->sample, offsets[i], dc->tile.focalToLensScale, &lensUV,
+                               &dirUV);
+                lx = lx + (R)lensUV.x * inv;
+                ly = ly + (R)lensUV.y * inv;
+                dx = dx + (R)dirUV.x * inv;
+                dy = dy + (R)dirUV.y * inv;
+            }
         }
         const o_tri* T = &tris[tri];
-        RV3 e0 = FN(up)(T->e0), e1 = FN(up)(T->e1);
-        RV3 v0ToRayOrigin = FN(sub)(cam, FN(up)(T->v0));
-        RV3 normal = FN(crs)(e0, e1);
-        RV3 ndir = FN(scl)(dir, (R)-1), ndu = FN(scl)(du, (R)-1), ndv = FN(scl)(dv, (R)-1);
-        R denomC = FN(dot)(ndir, normal);
-        RV3 eC = FN(crs)(ndir, v0ToRayOrigin);
-        R vC = FN(dot)(e1, eC);
-        R wC = -FN(dot)(e0, eC);
-        R dD0 = FN(dot)(ndu, normal), dD1 = FN(dot)(ndv, normal);
-        RV3 cu = FN(crs)(ndu, v0ToRayOrigin), cv = FN(crs)(ndv, v0ToRayOrigin);
-        R dV0 = FN(dot)(e1, cu), dV1 = FN(dot)(e1, cv);
-        R dW0 = -FN(dot)(e0, cu), dW1 = -FN(dot)(e0, cv);
-        R denom = FN(fm)(dD1, cy, FN(fm)(dD0, cx, denomC));
-        R v = FN(fm)(dV1, cy, FN(fm)(dV0, cx, vC));
-        R w = FN(fm)(dW1, cy, FN(fm)(dW0, cx, wC));
-        R u = (denom - v) - w;
-        R dinv = (R)1 / denom;
-        R bx = u * dinv, by = v * dinv, bz = w * dinv;
+        R bx, by, bz;
+        if (dc) {
+            /* IntersectTriangleTile<true>::setup + IntersectTriangleThreadDoF ctor + calcUVW */
+            RV3 e0 = FN(up)(T->e0), e1 = FN(up)(T->e1);
+            RV3 normal = FN(crs)(e0, e1);
+            RV3 lu = FN(up)(dc->tile.lensU), lv = FN(up)(dc->tile.lensV), lcfc = FN(up)(dc->lensToFocal);
+            RV3 v0ToLens = FN(sub)(FN(up)(camPos), FN(up)(T->v0));
+            R dDA0 = FN(dot)(FN(scl)(lu, (R)-1), normal), dDA1 = FN(dot)(FN(scl)(lv, (R)-1), normal);
+            R denomC = FN(dot)(FN(scl)(lcfc, (R)-1), normal);
+            R denom = FN(fm)(dDA1, dy, FN(fm)(dDA0, dx, denomC));
+            RV3 o = FN(mk)(FN(fm)(ly, lv.x, FN(fm)(lx, lu.x, v0ToLens.x)), FN(fm)(ly, lv.y, FN(fm)(lx, lu.y, v0ToLens.y)),
+                           FN(fm)(ly, lv.z, FN(fm)(lx, lu.z, v0ToLens.z)));
+            RV3 rd = FN(mk)(FN(fm)(dy, lv.x, FN(fm)(dx, lu.x, lcfc.x)), FN(fm)(dy, lv.y, FN(fm)(dx, lu.y, lcfc.y)),
+                            FN(fm)(dy, lv.z, FN(fm)(dx, lu.z, lcfc.z)));
+            RV3 nr = FN(scl)(rd, (R)-1);
+            RV3 e = FN(mk)(FN(fm)(nr.y, o.z, -(nr.z * o.y)), FN(fm)(nr.z, o.x, -(nr.x * o.z)),
+                           FN(fm)(nr.x, o.y, -(nr.y * o.x)));
+            R v = FN(fm)(e1.z, e.z, FN(fm)(e1.y, e.y, e1.x * e.x));
+            R w = -FN(fm)(e0.z, e.z, FN(fm)(e0.y, e.y, e0.x * e.x));
+            R u = (denom - v) - w;
+            R dinv = (R)1 / denom;
+            bx = u * dinv;
+            by = v * dinv;
+            bz = w * dinv;
+        } else {
+            RV3 e0 = FN(up)(T->e0), e1 = FN(up)(T->e1);
+            RV3 v0ToRayOrigin = FN(sub)(cam, FN(up)(T->v0));
+            RV3 normal = FN(crs)(e0, e1);
+            RV3 ndir = FN(scl)(dir, (R)-1), ndu = FN(scl)(du, (R)-1), ndv = FN(scl)(dv, (R)-1);
+            R denomC = FN(dot)(ndir, normal);
+            RV3 eC = FN(crs)(ndir, v0ToRayOrigin);
+            R vC = FN(dot)(e1, eC);
+            R wC = -FN(dot)(e0, eC);
+            R dD0 = FN(dot)(ndu, normal), dD1 = FN(dot)(ndv, normal);
+            RV3 cu = FN(crs)(ndu, v0ToRayOrigin), cv = FN(crs)(ndv, v0ToRayOrigin);
+            R dV0 = FN(dot)(e1, cu), dV1 = FN(dot)(e1, cv);
+            R dW0 = -FN(dot)(e0, cu), dW1 = -FN(dot)(e0, cv);
+            R denom = FN(fm)(dD1, cy, FN(fm)(dD0, cx, denomC));
+            R v = FN(fm)(dV1, cy, FN(fm)(dV0, cx, vC));
+            R w = FN(fm)(dW1, cy, FN(fm)(dW0, cx, wC));
+            R u = (denom - v) - w;
+            R dinv = (R)1 / denom;
+            bx = u * dinv;
+            by = v * dinv;
+            bz = w * dinv;
+        }
         const o_trishade* S = &shade[tri];
         const o_vertex* V0 = &verts[S->idx[0]];
         const o_vertex* V1 = &verts[S->idx[1]];

@@ -99,7 +99,19 @@ def build_tile_lists(nodes, block_pre, tile_pre):
     return counts, lists[:total]
 
 
-def intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris):
+class _Dof(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_float), ("focalDistance", ctypes.c_float), ("lensTable", c_void_p)]
+
+
+def _dof(dof):
+    """dof = (radius, focal_distance, lens_table) or None"""
+    if dof is None:
+        return None, None
+    table = np.ascontiguousarray(dof[2], np.float32)
+    return _Dof(float(dof[0]), float(dof[1]), table.ctypes.data_as(c_void_p)), table
+
+
+def intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris, dof=None):
     beams = np.ascontiguousarray(beams)
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     offsets = np.ascontiguousarray(offsets, np.float32)
@@ -107,13 +119,29 @@ def intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris):
     depth = np.zeros(len(beams) * A, np.float32)
     surv = np.zeros(1, np.uint64)
     lists = np.ascontiguousarray(lists, np.uint32)
-    rc = lib().or_intersect_view(_p(beams), c_uint32(len(beams)), _p(world_tiles), c_uint32(len(world_tiles)), _p(m),
-                                 _p(offsets), c_uint32(A), _p(np.ascontiguousarray(counts, np.uint32)),
-                                 _p(lists if lists.size else np.zeros(1, np.uint32)), _p(tris), _p(tri_out),
-                                 _p(depth), _p(surv))
+    d, keep = _dof(dof)
+    rc = lib().or_intersect_view_dof(_p(beams), c_uint32(len(beams)), _p(world_tiles), c_uint32(len(world_tiles)),
+                                     _p(m), _p(offsets), c_uint32(A), _p(np.ascontiguousarray(counts, np.uint32)),
+                                     _p(lists if lists.size else np.zeros(1, np.uint32)), _p(tris),
+                                     ctypes.byref(d) if d else None, _p(tri_out), _p(depth), _p(surv))
     if rc:
         raise RuntimeError("oracle intersect failed")
     return tri_out.reshape(-1, A), depth.reshape(-1, A), int(surv[0])
+
+
+def dof_rays(beams, m, offsets, A, dof, counts):
+    """World-space depth-of-field subsample rays (origins, directions), [samples*A, 3] each."""
+    beams = np.ascontiguousarray(beams)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    offsets = np.ascontiguousarray(offsets, np.float32)
+    o = np.zeros((len(beams) * A, 3), np.float32)
+    d = np.zeros((len(beams) * A, 3), np.float32)
+    dd, keep = _dof(dof)
+    rc = lib().or_dof_rays(_p(beams), c_uint32(len(beams)), _p(m), _p(offsets), c_uint32(A), ctypes.byref(dd),
+                           _p(np.ascontiguousarray(counts, np.uint32)), _p(o), _p(d))
+    if rc:
+        raise RuntimeError("oracle dof rays failed")
+    return o, d
 
 
 def emit_gbuffer(sample_tris):
@@ -144,16 +172,19 @@ def lighting_env(lights):
     return env, dropped
 
 
-def resolve_view(beams, m, offsets, A, tri_ids, tris, shade, verts, materials, env, precision=0):
+def resolve_view(beams, m, offsets, A, tri_ids, tris, shade, verts, materials, env, precision=0, dof=None,
+                 counts=None):
     beams = np.ascontiguousarray(beams)
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     offsets = np.ascontiguousarray(offsets, np.float32)
     colors = np.zeros(len(beams), np.uint32)
     linear = np.zeros((len(beams), 3), np.float32)
     tri_ids = np.ascontiguousarray(tri_ids, np.uint32)
-    rc = lib().or_resolve_view(_p(beams), c_uint32(len(beams)), _p(m), _p(offsets), c_uint32(A), _p(tri_ids),
-                               _p(tris), _p(shade), _p(verts), _p(materials), _p(env), c_int(precision), _p(colors),
-                               _p(linear))
+    d, keep = _dof(dof)
+    cnt = None if counts is None else np.ascontiguousarray(counts, np.uint32)
+    rc = lib().or_resolve_view_dof(_p(beams), c_uint32(len(beams)), _p(m), _p(offsets), c_uint32(A), _p(tri_ids),
+                                   _p(tris), _p(shade), _p(verts), _p(materials), _p(env), c_int(precision),
+                                   ctypes.byref(d) if d else None, _p(cnt), _p(colors), _p(linear))
     if rc:
         raise RuntimeError("oracle resolve failed")
     return colors, linear
@@ -189,15 +220,17 @@ def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=T
     world_tiles, tile_pre = frustum_setup(view.tile_frusta, m)
     _, block_pre = frustum_setup(view.block_frusta, m)
     counts, lists = build_tile_lists(nodes, block_pre, tile_pre)
-    from hvvr_amd import subsample_offsets
+    from hvvr_amd import lens_table, subsample_offsets
     offsets = subsample_offsets(A)
-    tri_ids, depth, surv = intersect_view(view.beams, world_tiles, m, offsets, A, counts, lists, tris)
+    radius, focal = float(view.lens[0]), float(view.lens[1])
+    dof = (radius, focal, lens_table(A)) if radius > 0 else None
+    tri_ids, depth, surv = intersect_view(view.beams, world_tiles, m, offsets, A, counts, lists, tris, dof)
     out = dict(nodes=nodes, tris=tris, world_verts=wv, counts=counts, lists=lists, tri_ids=tri_ids, depth=depth,
                survivors=surv, offsets=offsets)
     if with_colors:
         env, _ = lighting_env(scene.lights)
         colors, linear = resolve_view(view.beams, m, offsets, A, tri_ids, tris, scene.triShade, wv, scene.materials,
-                                      env, precision)
+                                      env, precision, dof, counts)
         # empty tiles: clear colour (ClearEmptyKernel, resolve.cu:527-542)
         empty = np.repeat(counts == 0, 128)
         colors[empty[: len(colors)]] = 0xFF000000

@@ -127,3 +127,22 @@ def test_camera_looking_away_renders_clear(H, O, ctx, scene1, view1_small):
 def test_invalid_view_rejected(H, ctx, view1_small):
     with pytest.raises(H.HvvrError, match="msaaRate"):
         H.GpuView(ctx, view1_small, 3)
+
+
+def test_batched_views_match_solo_renders(H, ctx, scene1, scene2, view1_small, view2_small):
+    # several views of one scene in one submission run on separate streams; each result
+    # must equal the same view rendered alone (a view listed twice shares a stream)
+    eye1 = H.HostView(H.VIEW_PRESET_ROOM_EYE, 1, 270, 300)
+    views = [H.GpuView(ctx, view2_small, 4), H.GpuView(ctx, eye1, 4), H.GpuView(ctx, view2_small, 2)]
+    gs = H.GpuScene(ctx, scene2)
+    solo = []
+    for gv in views:
+        gv.render(gs)
+        solo.append((gv.read_gbuffer(), gv.read_sample_colors(), gv.read_image()))
+    m = view2_small.camera_to_world
+    H.render_views(gs, views + [views[0]], cameras=[m, None, m, m])
+    for gv, (g, c, im) in zip(views, solo):
+        g2 = gv.read_gbuffer()
+        assert np.array_equal(g[0], g2[0]) and np.array_equal(g[1], g2[1])
+        assert np.array_equal(c, gv.read_sample_colors())
+        assert np.array_equal(im, gv.read_image())
