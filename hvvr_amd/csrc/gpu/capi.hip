@@ -5,7 +5,9 @@
 #include "gpu_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <string.h>
+#include <vector>
 
 namespace hvvr {
 
@@ -355,8 +357,10 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         return fail("view description has null arrays");
     if (d->split < 1 || d->split > 3 || d->width == 0 || d->height == 0)
         return fail("invalid image size or split");
-    if (d->lensRadius != 0.0f)
-        return fail("depth of field (lensRadius > 0) is not supported by this build");
+    if (!(d->lensRadius >= 0.0f) || !std::isfinite(d->lensRadius))
+        return fail("lensRadius must be finite and >= 0");
+    if (d->lensRadius > 0.0f && (!(d->focalDistance > 0.0f) || !std::isfinite(d->focalDistance)))
+        return fail("depth of field needs a finite focalDistance > 0");
     uint64_t remapCount = uint64_t(d->width) * d->height * d->split;
     for (uint64_t i = 0; i < remapCount; i++)
         if (d->remap[i] >= int32_t(d->sampleCount))
@@ -373,6 +377,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     v->height = d->height;
     v->split = d->split;
     v->lens = {d->lensRadius, d->focalDistance};
+    v->dof = d->lensRadius > 0.0f;
     subsampleOffsets(v->msaaRate, 0.0f, v->offsets);
     hipStream_t st = ctx->stream;
     hipError_t e = hipSuccess;
@@ -398,6 +403,12 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->gbuffer.resize(size_t(v->sampleCount) * v->msaaRate));
     step(v->sampleResults.resize(v->sampleCount));
     step(v->image.resize(size_t(v->width) * v->height));
+    std::vector<vector2> table; // lives until the stream synchronisation below
+    if (v->dof) {
+        lensPositionTable(v->msaaRate, table);
+        step(upload(v->lensTable, table.data(), table.size(), st));
+        step(v->occRank.resize(v->tileCount));
+    }
     step(v->counters.resize(8));
     for (auto& ev : v->stageEvents)
         step(hipEventCreate(&ev));
@@ -424,8 +435,8 @@ int hvvr_gpu_view_destroy(hvvr_gpu_view* v) {
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* v, uint32_t entries) {
     if (!v)
         return fail("null view");
-    if (entries < 1 || entries > 256)
-        return fail("traversal stack limit must be in [1, 256]");
+    if (entries < 1 || entries > 64)
+        return fail("traversal stack limit must be in [1, 64]");
     v->waveStackLimit = entries;
     return HVVR_GPU_OK;
 }
@@ -461,6 +472,8 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     launchTileLists(v, s, st);
     mark(3);
     launchTileListFallback(v, s, st);
+    if (v->dof)
+        launchOccupiedRank(v, st);
     mark(4);
     launchIntersect(v, s, m, st);
     mark(5);

@@ -93,6 +93,7 @@ struct hvvr_gpu_view {
     uint32_t sampleCount = 0, validSampleCount = 0, tileCount = 0, blockCount = 0;
     uint32_t width = 0, height = 0, split = 1;
     hvvr::ThinLens lens = {0.0f, 1.0f};
+    bool dof = false;                                   // lens.radius > 0
     bool jitterSpin = false;
     hvvr::vector2 offsets[32];
     hvvr::matrix4x4 lastCameraToWorld = hvvr::matrix4x4::identity();
@@ -111,10 +112,12 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> sampleResults;         // [sampleCount]
     hvvr::DeviceBuffer<int32_t> remap;                  // [width*height*split]
     hvvr::DeviceBuffer<uint32_t> image;                 // [width*height]
+    hvvr::DeviceBuffer<hvvr::vector2> lensTable;        // depth of field: [4][A][128] lens positions
+    hvvr::DeviceBuffer<uint32_t> occRank;               // depth of field: rank among occupied tiles
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
                                                         // [3] list entries allocated, [4] node visits
     uint64_t listCapacity = 0;
-    uint32_t waveStackLimit = 256; // LDS stack entries per tile traversal (<= 256)
+    uint32_t waveStackLimit = 64; // register stack entries per tile traversal (<= 64)
     hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
     bool stagesRecorded = false;
     bool rendered = false;
@@ -133,6 +136,7 @@ void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
+void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream);
 void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchRemap(hvvr_gpu_view* v, hipStream_t stream);

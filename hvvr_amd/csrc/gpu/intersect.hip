@@ -18,7 +18,10 @@
 // the resolve pass can reach was written this frame.
 
 #include "device_common.h"
+#include "dof.h"
 #include "gpu_internal.h"
+
+#include <type_traits>
 
 namespace hvvr {
 
@@ -34,6 +37,11 @@ struct IntersectArgs {
     uint64_t listCapacity;
     matrix4x4 cameraToWorld;
     vector2 offsets[32];
+    // depth of field: lens-position table [4][A][128], rank of each tile among the
+    // tiles with a non-empty list (its table row is rank % 4), thin lens
+    const vector2* lensTable;
+    const uint32_t* occRank;
+    float lensRadius, focalDistance;
 };
 
 struct TriTile {
@@ -102,9 +110,42 @@ __device__ inline void emitGBuffer(const uint32_t* hit, GBufferSubsample* waveBa
     waveBase[slot * 64 + lane] = g;
 }
 
-template <int A>
+// TestTriangleFrustaUVW (prim_tests.h:89-116): false when all four corner rays of the tile
+// miss the triangle on the same side of one barycentric coordinate
+__device__ inline bool frustaUVWPass(const SimpleRayFrustum& frustum, const PrecomputedTriangleIntersect& tri) {
+    float uMin = FLT_MAX, uMax = -FLT_MAX, vMin = FLT_MAX, vMax = -FLT_MAX;
+    float wMin = FLT_MAX, wMax = -FLT_MAX;
+    for (int n = 0; n < 4; n++) {
+        vector3 rdir = frustum.directions[n];
+        vector3 o = frustum.origins[n];
+        vector3 nn = cross(tri.edge0, tri.edge1);
+        float denom = dot(-rdir, nn);
+        vector3 v0o = o - tri.v0;
+        float tr = dot(v0o, nn);
+        vector3 e = cross(-rdir, v0o);
+        float v = dot(tri.edge1, e);
+        float w = -dot(tri.edge0, e);
+        float ood = 1.0f / denom;
+        tr *= ood;
+        v *= ood;
+        w *= ood;
+        float u = 1.0f - v - w;
+        (void)tr;
+        uMin = fminf(uMin, u);
+        uMax = fmaxf(uMax, u);
+        vMin = fminf(vMin, v);
+        vMax = fmaxf(vMax, v);
+        wMin = fminf(wMin, w);
+        wMax = fmaxf(wMax, w);
+    }
+    return !(uMax < 0 || uMin > 1 || vMax < 0 || vMin > 1 || wMax < 0 || wMin > 1);
+}
+
+template <int A, bool DoF>
 __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
-    __shared__ TriTile cache[128];
+    using Cached = typename std::conditional<DoF, TriTileDoF, TriTile>::type;
+    __shared__ Cached cache[128];
+    __shared__ DofTile dofTile;
     __shared__ uint32_t cacheIndex[128];
     __shared__ float planes[4][4];
     __shared__ SimpleRayFrustum frustum;
@@ -128,6 +169,17 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
     const vector3 ndir = -(rot * beam.centerRay);
     const vector3 ndu = -(rot * beam.du);
     const vector3 ndv = -(rot * beam.dv);
+
+    // depth of field: tile lens frame from sample 64 (IntersectTile, intersect.cu:182-189),
+    // per-sample focal offset, the lens-table row of this tile
+    vector3 lcfc(0.0f);
+    uint32_t lensBase = 0;
+    if constexpr (DoF) {
+        if (tid == 64)
+            dofTile = dofTileLoad(rot, beam, args.lensRadius, args.focalDistance);
+        lcfc = dofLensToFocal(rot, rot * beam.centerRay, args.focalDistance);
+        lensBase = (args.occRank[tile] % DOF_LENS_TABLE_ROWS) * TILE_SIZE * A + tid;
+    }
 
     if (tid == 0) {
         SimpleRayFrustum fr = args.worldTiles[tile];
@@ -155,7 +207,7 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
         // ---- cull one list entry per thread ----
         bool keep = false;
         uint32_t triIndex = badTriIndex;
-        TriTile tt;
+        Cached tt;
         if (chunk + tid < listCount) {
             triIndex = args.triIndices[listStart + chunk + tid];
             PrecomputedTriangleIntersect tri = args.tris[triIndex];
@@ -170,40 +222,20 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
                     inside = false;
             }
             if (inside) {
-                // IntersectTriangleTile<false>::setup (prim_tests.h:148-163)
-                tt.e0 = tri.edge0;
-                tt.e1 = tri.edge1;
-                tt.normal = cross(tt.e0, tt.e1);
-                tt.v0ToRayOrigin = rayOrigin - tri.v0;
-                tt.t = dot(tt.v0ToRayOrigin, tt.normal);
-                if (!(tt.t < 0.0f)) {
-                    // TestTriangleFrustaUVW (prim_tests.h:89-116)
-                    float uMin = FLT_MAX, uMax = -FLT_MAX, vMin = FLT_MAX, vMax = -FLT_MAX;
-                    float wMin = FLT_MAX, wMax = -FLT_MAX;
-                    for (int n = 0; n < 4; n++) {
-                        vector3 rdir = frustum.directions[n];
-                        vector3 o = frustum.origins[n];
-                        vector3 nn = cross(tri.edge0, tri.edge1);
-                        float denom = dot(-rdir, nn);
-                        vector3 v0o = o - tri.v0;
-                        float tr = dot(v0o, nn);
-                        vector3 e = cross(-rdir, v0o);
-                        float v = dot(tri.edge1, e);
-                        float w = -dot(tri.edge0, e);
-                        float ood = 1.0f / denom;
-                        tr *= ood;
-                        v *= ood;
-                        w *= ood;
-                        float u = 1.0f - v - w;
-                        (void)tr;
-                        uMin = fminf(uMin, u);
-                        uMax = fmaxf(uMax, u);
-                        vMin = fminf(vMin, v);
-                        vMax = fmaxf(vMax, v);
-                        wMin = fminf(wMin, w);
-                        wMax = fmaxf(wMax, w);
-                    }
-                    keep = !(uMax < 0 || uMin > 1 || vMax < 0 || vMin > 1 || wMax < 0 || wMin > 1);
+                if constexpr (DoF) {
+                    // IntersectTriangleTile<true>::setup; the corner-ray UVW test only when
+                    // every lens position sees the triangle's front (intersect.cu:211-236)
+                    int r = dofTriSetup(tri, rayOrigin, dofTile.lensU, dofTile.lensV, tt);
+                    keep = r == 1 || (r == 2 && frustaUVWPass(frustum, tri));
+                } else {
+                    // IntersectTriangleTile<false>::setup (prim_tests.h:148-163)
+                    tt.e0 = tri.edge0;
+                    tt.e1 = tri.edge1;
+                    tt.normal = cross(tt.e0, tt.e1);
+                    tt.v0ToRayOrigin = rayOrigin - tri.v0;
+                    tt.t = dot(tt.v0ToRayOrigin, tt.normal);
+                    if (!(tt.t < 0.0f))
+                        keep = frustaUVWPass(frustum, tri);
                 }
             }
         }
@@ -223,6 +255,21 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
         survivorsSum += total;
 
         // ---- every sample against every survivor, in list order ----
+        if constexpr (DoF) {
+            for (uint32_t j = 0; j < total; j++) {
+                const TriTileDoF& c = cache[j];
+                const uint32_t idx = cacheIndex[j];
+                // IntersectTriangleThreadDoF (prim_tests.h:295-355)
+                const float denomCenter = dot(-lcfc, c.normal);
+#pragma unroll
+                for (int i = 0; i < A; i++) {
+                    const vector2 lensUV = args.lensTable[lensBase + i * TILE_SIZE];
+                    const vector2 dirUV = dofDirUV(args.offsets[i], dofTile.focalToLensScale, lensUV);
+                    if (dofTest(c, denomCenter, lcfc, dofTile.lensU, dofTile.lensV, lensUV, dirUV, depth[i]))
+                        hit[i] = idx;
+                }
+            }
+        } else
         for (uint32_t j = 0; j < total; j++) {
             const TriTile& c = cache[j];
             const uint32_t idx = cacheIndex[j];
@@ -277,14 +324,27 @@ void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4&
     a.cameraToWorld = m;
     for (int i = 0; i < 32; i++)
         a.offsets[i] = v->offsets[i];
+    a.lensTable = v->lensTable.ptr;
+    a.occRank = v->occRank.ptr;
+    a.lensRadius = v->lens.radius;
+    a.focalDistance = v->lens.focalDistance;
     dim3 grid(v->tileCount), block(128);
+    const bool dof = v->dof;
     switch (v->msaaRate) {
-        case 1: intersectKernel<1><<<grid, block, 0, stream>>>(a); break;
-        case 2: intersectKernel<2><<<grid, block, 0, stream>>>(a); break;
-        case 4: intersectKernel<4><<<grid, block, 0, stream>>>(a); break;
-        case 8: intersectKernel<8><<<grid, block, 0, stream>>>(a); break;
-        case 16: intersectKernel<16><<<grid, block, 0, stream>>>(a); break;
-        case 32: intersectKernel<32><<<grid, block, 0, stream>>>(a); break;
+#define HVVR_INTERSECT_CASE(A)                                                        \
+    case A:                                                                           \
+        if (dof)                                                                      \
+            intersectKernel<A, true><<<grid, block, 0, stream>>>(a);                  \
+        else                                                                          \
+            intersectKernel<A, false><<<grid, block, 0, stream>>>(a);                 \
+        break;
+        HVVR_INTERSECT_CASE(1)
+        HVVR_INTERSECT_CASE(2)
+        HVVR_INTERSECT_CASE(4)
+        HVVR_INTERSECT_CASE(8)
+        HVVR_INTERSECT_CASE(16)
+        HVVR_INTERSECT_CASE(32)
+#undef HVVR_INTERSECT_CASE
         default: break;
     }
 }

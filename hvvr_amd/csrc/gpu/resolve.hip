@@ -17,6 +17,7 @@
 // SimpleMaterial::badTextureIndex shade with their constant colours.
 
 #include "device_common.h"
+#include "dof.h"
 #include "gpu_internal.h"
 
 namespace hvvr {
@@ -33,6 +34,10 @@ struct ResolveArgs {
     matrix4x4 cameraToWorld;
     vector2 offsets[32];
     LightingEnvironment env;
+    // depth of field (see IntersectArgs)
+    const vector2* lensTable;
+    const uint32_t* occRank;
+    float lensRadius, focalDistance;
 };
 
 __device__ inline vector3 ggxSpecular(vector3 N, vector3 L, vector3 V, float roughness, vector3 F0) {
@@ -119,8 +124,9 @@ __device__ inline float linearToSRgb(float x) {
     return (x <= 0.0031308f) ? (x * 12.92f) : (powf(x, 1.0f / 2.4f) * 1.055f - 0.055f);
 }
 
-template <int A>
+template <int A, bool DoF>
 __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
+    __shared__ DofTile dofTile;
     const uint32_t tile = blockIdx.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t sample = tile * TILE_SIZE + tid;
@@ -138,6 +144,17 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
     const vector3 ndu = -(rot * beam.du);
     const vector3 ndv = -(rot * beam.dv);
     const uint32_t maskAll = A < 32 ? ((1u << A) - 1u) : 0xffffffffu;
+    // depth of field: tile lens frame from sample 64 (ResolveKernel, resolve.cu:358-366),
+    // per-sample focal offset, the lens-table row of this tile
+    vector3 lcfc(0.0f);
+    uint32_t lensBase = 0;
+    if constexpr (DoF) {
+        if (tid == 64)
+            dofTile = dofTileLoad(rot, beam, args.lensRadius, args.focalDistance);
+        lcfc = dofLensToFocal(rot, rot * beam.centerRay, args.focalDistance);
+        lensBase = (args.occRank[tile] % DOF_LENS_TABLE_ROWS) * TILE_SIZE * A + tid;
+        __syncthreads();
+    }
 
     vector3 result(0.0f);
     uint32_t combined = 0;
@@ -151,29 +168,63 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
         combined |= mask;
         float inv = 1.0f / float(__popc(mask));
         float cx = 0.0f, cy = 0.0f;
+        float lx = 0.0f, ly = 0.0f, dx = 0.0f, dy = 0.0f;
         for (uint32_t bits = mask; bits; bits &= bits - 1) {
             int i = __ffs(bits) - 1;
             vector2 o = args.offsets[i];
             cx = cx + o.x * inv;
             cy = cy + o.y * inv;
+            if constexpr (DoF) {
+                // centroid lens position and direction offset (resolve.cu:212-222)
+                const vector2 lensUV = args.lensTable[lensBase + i * TILE_SIZE];
+                const vector2 dirUV = dofDirUV(o, dofTile.focalToLensScale, lensUV);
+                lx = lx + lensUV.x * inv;
+                ly = ly + lensUV.y * inv;
+                dx = dx + dirUV.x * inv;
+                dy = dy + dirUV.y * inv;
+            }
         }
         PrecomputedTriangleIntersect T = args.tris[g.triIndex];
-        vector3 normal = cross(T.edge0, T.edge1);
-        vector3 v0o = cam - T.v0;
-        float denomC = dot(ndir, normal);
-        vector3 eC = cross(ndir, v0o);
-        float vC = dot(T.edge1, eC);
-        float wC = -dot(T.edge0, eC);
-        float dD0 = dot(ndu, normal), dD1 = dot(ndv, normal);
-        vector3 cu = cross(ndu, v0o), cv = cross(ndv, v0o);
-        float dV0 = dot(T.edge1, cu), dV1 = dot(T.edge1, cv);
-        float dW0 = -dot(T.edge0, cu), dW1 = -dot(T.edge0, cv);
-        float denom = fmaf(dD1, cy, fmaf(dD0, cx, denomC));
-        float v = fmaf(dV1, cy, fmaf(dV0, cx, vC));
-        float w = fmaf(dW1, cy, fmaf(dW0, cx, wC));
-        float u = (denom - v) - w;
-        float dinv = 1.0f / denom;
-        float bx = u * dinv, by = v * dinv, bz = w * dinv;
+        float bx, by, bz;
+        if constexpr (DoF) {
+            // IntersectTriangleTile<true>::setup + IntersectTriangleThreadDoF::calcUVW
+            // (prim_tests.h:263-291, 366-392) at the centroid
+            const vector3 lu = dofTile.lensU, lv = dofTile.lensV;
+            const vector3 normal = cross(T.edge0, T.edge1);
+            const vector3 v0ToLens = cam - T.v0;
+            const float dDA0 = dot(-lu, normal), dDA1 = dot(-lv, normal);
+            const float denomC = dot(-lcfc, normal);
+            const float denom = fmaf(dDA1, dy, fmaf(dDA0, dx, denomC));
+            const vector3 o = axpy2(v0ToLens, lx, lu, ly, lv);
+            const vector3 rd = axpy2(lcfc, dx, lu, dy, lv);
+            const vector3 e = crossFma(-rd, o);
+            const float v = dotFma(T.edge1, e);
+            const float w = -dotFma(T.edge0, e);
+            const float u = (denom - v) - w;
+            const float dinv = 1.0f / denom;
+            bx = u * dinv;
+            by = v * dinv;
+            bz = w * dinv;
+        } else {
+            vector3 normal = cross(T.edge0, T.edge1);
+            vector3 v0o = cam - T.v0;
+            float denomC = dot(ndir, normal);
+            vector3 eC = cross(ndir, v0o);
+            float vC = dot(T.edge1, eC);
+            float wC = -dot(T.edge0, eC);
+            float dD0 = dot(ndu, normal), dD1 = dot(ndv, normal);
+            vector3 cu = cross(ndu, v0o), cv = cross(ndv, v0o);
+            float dV0 = dot(T.edge1, cu), dV1 = dot(T.edge1, cv);
+            float dW0 = -dot(T.edge0, cu), dW1 = -dot(T.edge0, cv);
+            float denom = fmaf(dD1, cy, fmaf(dD0, cx, denomC));
+            float v = fmaf(dV1, cy, fmaf(dV0, cx, vC));
+            float w = fmaf(dW1, cy, fmaf(dW0, cx, wC));
+            float u = (denom - v) - w;
+            float dinv = 1.0f / denom;
+            bx = u * dinv;
+            by = v * dinv;
+            bz = w * dinv;
+        }
         PrecomputedTriangleShade S = args.triShade[g.triIndex];
         ShadingVertex V0 = args.verts[S.indices[0]], V1 = args.verts[S.indices[1]], V2 = args.verts[S.indices[2]];
         vector3 pos = V0.pos * bx + V1.pos * by + V2.pos * bz;
@@ -231,14 +282,27 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
     for (int i = 0; i < 32; i++)
         a.offsets[i] = v->offsets[i];
     a.env = s->lighting;
+    a.lensTable = v->lensTable.ptr;
+    a.occRank = v->occRank.ptr;
+    a.lensRadius = v->lens.radius;
+    a.focalDistance = v->lens.focalDistance;
     dim3 grid(v->tileCount), block(128);
+    const bool dof = v->dof;
     switch (v->msaaRate) {
-        case 1: resolveKernel<1><<<grid, block, 0, stream>>>(a); break;
-        case 2: resolveKernel<2><<<grid, block, 0, stream>>>(a); break;
-        case 4: resolveKernel<4><<<grid, block, 0, stream>>>(a); break;
-        case 8: resolveKernel<8><<<grid, block, 0, stream>>>(a); break;
-        case 16: resolveKernel<16><<<grid, block, 0, stream>>>(a); break;
-        case 32: resolveKernel<32><<<grid, block, 0, stream>>>(a); break;
+#define HVVR_RESOLVE_CASE(A)                                                          \
+    case A:                                                                           \
+        if (dof)                                                                      \
+            resolveKernel<A, true><<<grid, block, 0, stream>>>(a);                    \
+        else                                                                          \
+            resolveKernel<A, false><<<grid, block, 0, stream>>>(a);                   \
+        break;
+        HVVR_RESOLVE_CASE(1)
+        HVVR_RESOLVE_CASE(2)
+        HVVR_RESOLVE_CASE(4)
+        HVVR_RESOLVE_CASE(8)
+        HVVR_RESOLVE_CASE(16)
+        HVVR_RESOLVE_CASE(32)
+#undef HVVR_RESOLVE_CASE
         default: break;
     }
 }

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
     auto insert = [&](float key, const CullEntry& e) {
         int j = int(top) - 1 - int(lane);
         bool larger = j >= 0 && daz(cull[j >= 0 ? j : 0].bits) > key;
-        uint64_t run = ~__ballot(larger);
+        uint64_t run = ~ballot(larger);
         uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
         CullEntry moved;
         if (lane < n)
@@ -166,36 +166,51 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
 // keeping the stack sorted by distance and emitting each visited node's surviving leaf
 // triangles in visit order. A node's test depends only on the tile frustum and the
 // node, not on when the walk reaches it, so the walk is split into three phases:
-//   A. tests: breadth-first over the nodes the walk will visit, 16 nodes per step (lane
-//      = node slot x child, each lane testing one child against every frustum axis).
-//      Each visited node becomes an LDS record: its final child mask (coarse test,
-//      refined when a leaf child passes), its leaf mask, the leaf mask of its stack
-//      entry and the index of its first internal child's record; the internal children
-//      that pass are appended as new records, in child order, with their distance keys.
-//   B. order: the reference's stack walk replayed over the records, in LDS only: pops,
-//      the single-child shortcut, the sorted insertion (one ballot compares up to 64
-//      entries), and the sequence of records whose leaves are emitted.
+//   A. tests: breadth-first over the nodes the walk will visit, 32 nodes per step (lane
+//      = node slot x child, each lane testing one child against every frustum axis,
+//      two node slots per lane group). Each visited node becomes an LDS record: its
+//      final child mask (coarse test, refined when a leaf child passes), its leaf mask,
+//      the leaf mask of its stack entry and the index of its first internal child's
+//      record; the internal children that pass are appended as new records, in child
+//      order, with their distance keys.
+//   B. order: the reference's stack walk replayed over the records, with the stack in
+//      one VGPR pair: pops, the single-child shortcut, the sorted insertion (one ballot
+//      compares every entry), and the sequence of records whose leaves are emitted.
 //   C. emit: the list length is summed, space is claimed with one atomic add, and the
 //      triangle ranges of the emitted leaves are written in order.
 // The dependent global loads per tile drop from one per visited node (~370 on the
-// 262k-triangle room scene) to one per 16 nodes. A tile whose records or stack exceed
-// the LDS capacity is flagged for the global-stack kernel instead.
+// 262k-triangle room scene) to one per 32 nodes.
+//
+// Phase B leaves out dead records, those with no passing child: the reference pops them
+// and does nothing else. Its stack stays sorted by key, ties newest on top (the block
+// stack arrives sorted and every insert lands at its sorted position), so a pop always
+// takes the least (key, newest) entry; an entry that changes nothing when popped can be
+// dropped without moving any other pop. A NaN key breaks that order (it inserts on top
+// of anything), so a tile with one, or with an unsorted block stack, goes to the
+// global-stack kernel, as does a tile whose records or stack exceed the capacity.
 // ---------------------------------------------------------------------------------
-enum : uint32_t { TL_REC_CAP = 768, TL_STACK_CAP = 256 };
+enum : uint32_t { TL_REC_CAP = 768, TL_STACK_CAP = REG_STACK_CAP };
 
 struct TileListLds {
     uint32_t recNode[TL_REC_CAP];
     uint32_t recKey[TL_REC_CAP];   // float bits of the record's distance key; in phase C, the
                                    // list length of emitted record i
-    uint32_t recMeta[TL_REC_CAP];  // first child record (bits 0..15), child mask (16..19), leaf
-                                   // mask (20..23), stack-entry leaf mask (24..27)
+    uint32_t recMeta[TL_REC_CAP];  // packMeta(first child record, child mask, leaf mask,
+                                   // stack-entry leaf mask)
     uint16_t order[TL_REC_CAP];    // records with emitted leaves, in emission order
 };
 
-__device__ inline uint32_t metaFirst(uint32_t m) { return m & 0xffffu; }
-__device__ inline uint32_t metaMask(uint32_t m) { return (m >> 16) & 0xfu; }
-__device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 20) & 0xfu; }
-__device__ inline uint32_t metaEntry(uint32_t m) { return m >> 24; }
+// Record meta: first child record (bits 0..9), child mask (20..23), leaf mask (24..27),
+// stack-entry leaf mask (28..31). On the traversal stack bits 10..19 carry the record
+// index itself, so a popped entry needs no LDS read.
+__device__ inline uint32_t metaFirst(uint32_t m) { return m & 0x3ffu; }
+__device__ inline uint32_t metaRec(uint32_t m) { return (m >> 10) & 0x3ffu; }
+__device__ inline uint32_t metaMask(uint32_t m) { return (m >> 20) & 0xfu; }
+__device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 24) & 0xfu; }
+__device__ inline uint32_t metaEntry(uint32_t m) { return m >> 28; }
+__device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf, uint32_t entry) {
+    return first | (mask << 20) | (leaf << 24) | (entry << 28);
+}
 
 // One child box of a node, as one lane of a 4-lane node group loads it.
 struct ChildBox {
@@ -264,17 +279,20 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
     // ---- phase A: node tests, breadth-first ----
     if (!overflow) {
         const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
+        bool nanKey = false;
         if (lane < n0) {
             StackEntry e = bs[lane];
             sh.recNode[lane] = e.nodeMask & STACK_NODE_MASK;
             sh.recKey[lane] = fbits(e.tMin);
-            sh.recMeta[lane] = (e.nodeMask >> 28) << 24;
+            sh.recMeta[lane] = packMeta(0u, 0u, 0u, e.nodeMask >> 28);
+            nanKey = e.tMin != e.tMin;
         }
+        overflow = ballot(nanKey) != 0ull;
         waveLdsSync();
         const uint32_t s = lane >> 2, c = lane & 3u;
         const uint32_t axis = uint32_t(f->distanceAxis);
         const uint64_t groupBelow = (1ull << (4 * s)) - 1ull;
-        for (uint32_t head = 0; head < nrec;) {
+        for (uint32_t head = 0; head < nrec && !overflow;) {
             // this step tests records [head, end), two per lane group (so their node loads
             // overlap); the children it appends start at nrec
             const uint32_t end = head + 32 < nrec ? head + 32 : nrec;
@@ -293,20 +311,20 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
             const ChildBox B = loadChildBox(nodes, nodeB, c, axis);
             bool passA = validA && childCoarsePass(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
             bool passB = validB && childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
-            uint32_t maskA = uint32_t(__ballot(passA) >> (4 * s)) & 0xfu;
-            uint32_t maskB = uint32_t(__ballot(passB) >> (4 * s)) & 0xfu;
+            uint32_t maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
+            uint32_t maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
             // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
             if (passA && (maskA & A.leafMask) != 0u)
                 passA = !childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
             if (passB && (maskB & B.leafMask) != 0u)
                 passB = !childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
-            maskA = uint32_t(__ballot(passA) >> (4 * s)) & 0xfu;
-            maskB = uint32_t(__ballot(passB) >> (4 * s)) & 0xfu;
+            maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
+            maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
             // internal children are pushed only from entries without a leaf mask
             // (traverse_avx.h:476-483)
             const bool innerA = passA && ((A.leafMask >> c) & 1u) == 0u && entryA == 0u;
             const bool innerB = passB && ((B.leafMask >> c) & 1u) == 0u && entryB == 0u;
-            const uint64_t biA = __ballot(innerA), biB = __ballot(innerB);
+            const uint64_t biA = ballot(innerA), biB = ballot(innerB);
             const uint32_t cntA = uint32_t(__builtin_popcountll(biA));
             const uint32_t slotA = nrec + lanesBelow(biA);
             const uint32_t slotB = nrec + cntA + lanesBelow(biB);
@@ -322,82 +340,116 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
             }
             if (validA && c == 0) {
                 const uint32_t first = nrec + uint32_t(__builtin_popcountll(biA & groupBelow));
-                sh.recMeta[rA] = first | (maskA << 16) | (A.leafMask << 20) | (entryA << 24);
+                sh.recMeta[rA] = packMeta(first, maskA, A.leafMask, entryA);
             }
             if (validB && c == 0) {
                 const uint32_t first = nrec + cntA + uint32_t(__builtin_popcountll(biB & groupBelow));
-                sh.recMeta[rB] = first | (maskB << 16) | (B.leafMask << 20) | (entryB << 24);
+                sh.recMeta[rB] = packMeta(first, maskB, B.leafMask, entryB);
             }
             nrec += cntA + uint32_t(__builtin_popcountll(biB));
             head = end;
             waveLdsSync();
-            if (nrec > TL_REC_CAP) {
+            if (nrec > TL_REC_CAP || ballot((innerA && A.key != A.key) || (innerB && B.key != B.key)) != 0ull) {
                 overflow = true;
                 break;
             }
         }
     }
 
-    // ---- phase B: the reference's stack walk over the records, stack in VGPRs ----
+    // ---- phase B: the reference's stack walk over the live records, stack in VGPRs ----
     if (!overflow) {
         RegStack st;
-        // block stack entry n0-1 is the top (depth 0)
-        const uint32_t rootRec = n0 - 1 - lane;
-        st.t0 = lane < n0 ? bitsf(sh.recKey[lane < n0 ? rootRec : 0]) : 0.0f;
-        st.s0 = lane < n0 ? rootRec : 0u;
-        st.t1 = st.t2 = st.t3 = 0.0f;
-        st.s1 = st.s2 = st.s3 = 0u;
-        st.size = n0;
-        uint32_t cur = stackPop(st);
+        {
+            // the block stack's live entries, compacted in order (entry q at position q)
+            const bool valid = lane < n0;
+            const uint32_t key = valid ? sh.recKey[lane] : 0u;
+            const uint32_t meta = valid ? (sh.recMeta[lane] | (lane << 10)) : 0u;
+            const uint64_t live = ballot(valid && metaMask(meta) != 0u);
+            st.size = uint32_t(__builtin_popcountll(live));
+            const uint32_t dest = ((live >> lane) & 1u) ? lanesBelow(live) : st.size + lanesBelow(~live);
+            st.t = bitsf(uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(key))));
+            st.m = uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(meta)));
+            const float above = __shfl_down(st.t, 1);
+            if (ballot(lane + 1u < st.size && st.t < above) != 0ull)
+                overflow = true;
+            deepest = st.size;
+        }
+        uint32_t emitBuf = 0; // emission order, 64 records at a time
         uint32_t steps = 0;
-        for (;;) {
-            // every record is reached exactly once; the bound keeps the loop finite whatever
+        uint32_t cur = 0;
+        bool walking = !overflow && st.size != 0u;
+        if (walking)
+            cur = stackPop(st);
+        while (walking) {
+            // every record is reached at most once; the bound keeps the loop finite whatever
             // the records hold
             if (++steps > nrec) {
                 overflow = true;
                 break;
             }
-            const uint32_t meta = uniform(sh.recMeta[cur]);
-            const uint32_t mask = metaMask(meta), leafMask = metaLeaf(meta);
-            bool pop = true;
-            if (mask) {
-                if (mask & leafMask) {
-                    if (lane == 0)
-                        sh.order[nEmit] = uint16_t(cur);
-                    nEmit++;
-                }
-                const uint32_t inner = mask & ~leafMask;
-                if (inner && metaEntry(meta) == 0u) {
-                    const uint32_t first = metaFirst(meta);
-                    if ((inner & (inner - 1u)) == 0u) {
-                        // single hit child: test it next without touching the stack
-                        cur = first;
+            const uint32_t mask = metaMask(cur), leafMask = metaLeaf(cur);
+            if (mask & leafMask) {
+                emitBuf = writeLane(emitBuf, metaRec(cur), nEmit & 63u, lane);
+                nEmit++;
+                if ((nEmit & 63u) == 0u)
+                    sh.order[nEmit - 64u + lane] = uint16_t(emitBuf);
+            }
+            const uint32_t inner = mask & ~leafMask;
+            if (inner != 0u && metaEntry(cur) == 0u) {
+                const uint32_t first = metaFirst(cur);
+                if ((inner & (inner - 1u)) == 0u) {
+                    // single hit child: test it next without touching the stack
+                    const uint32_t child = uniform(sh.recMeta[first]);
+                    if (metaMask(child) != 0u) {
+                        cur = child | (first << 10);
                         continue;
                     }
+                } else {
                     // children in child order, each inserted below the run of closer entries
-                    // (traverse_avx.h:499-520)
+                    // (traverse_avx.h:499-520), dead ones left out
                     const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
-                    const uint32_t keys = lane < nChildren ? sh.recKey[first + lane] : 0u;
-                    for (uint32_t j = 0; j < nChildren; j++) {
-                        if (st.size >= limit) {
+                    const bool isChild = lane < nChildren;
+                    const uint32_t childKey = isChild ? sh.recKey[first + lane] : 0u;
+                    const uint32_t childMeta = isChild ? (sh.recMeta[first + lane] | ((first + lane) << 10)) : 0u;
+                    const uint32_t liveBits = uint32_t(ballot(isChild && metaMask(childMeta) != 0u));
+                    if (liveBits != 0u) {
+                        const uint32_t nLive = uint32_t(__builtin_popcount(liveBits));
+                        if (st.size + nLive - 1u >= limit) {
                             overflow = true;
                             break;
                         }
-                        stackInsert(st, bitsf(readLane(keys, j)), first + j, lane);
+                        // The walk pops the top right after these inserts. When the closest
+                        // child would land on top (ties go to the newest insert), visit it
+                        // directly and insert only its siblings: the stack ends the same.
+                        // (the newest of the closest live children; the tile has no NaN keys)
+                        const uint32_t last = 31u - uint32_t(__builtin_clz(liveBits));
+                        uint32_t best = last;
+                        float bestKey = bitsf(readLane(childKey, last));
+                        for (uint32_t j = last; j-- > 0u;) {
+                            const float kj = bitsf(readLane(childKey, j));
+                            if (((liveBits >> j) & 1u) && kj < bestKey) {
+                                bestKey = kj;
+                                best = j;
+                            }
+                        }
+                        const bool take = st.size == 0u || bestKey <= stackTopKey(st);
+                        for (uint32_t j = 0; j < nChildren; j++)
+                            if (((liveBits >> j) & 1u) && !(take && j == best))
+                                stackInsert(st, bitsf(readLane(childKey, j)), readLane(childMeta, j), lane);
+                        const uint32_t depth = st.size + (take ? 1u : 0u);
+                        deepest = depth > deepest ? depth : deepest;
+                        cur = take ? readLane(childMeta, best) : stackPop(st);
+                        continue;
                     }
-                    if (overflow)
-                        break;
-                    deepest = st.size > deepest ? st.size : deepest;
-                    cur = stackPop(st);
-                    pop = false;
                 }
             }
-            if (pop) {
-                if (st.size == 0)
-                    break;
-                cur = stackPop(st);
-            }
+            if (st.size == 0u)
+                break;
+            cur = stackPop(st);
         }
+        if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
+            sh.order[(nEmit & ~63u) + lane] = uint16_t(emitBuf);
+        waveLdsSync();
     }
     if (overflow) {
         // re-traversed by the global-stack kernel
@@ -602,6 +654,37 @@ void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
     blockTraverseWaveKernel<<<v->blockCount, 64, 0, stream>>>(s->nodes.ptr, v->blockPre.ptr, v->blockCount,
                                                               v->blockStacks.ptr, v->blockStackSize.ptr);
+}
+
+// Rank of every tile among the tiles with a non-empty list (the reference's compacted
+// tile index, traversal.cpp:281-292): one workgroup scans the list lengths in tile order.
+enum : uint32_t { kRankThreads = 1024 };
+__global__ __launch_bounds__(kRankThreads) void occupiedRankKernel(const uint32_t* __restrict__ counts, uint32_t n,
+                                                                   uint32_t* __restrict__ rank) {
+    __shared__ uint32_t waveTotals[kRankThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kRankThreads) {
+        const uint32_t i = base + tid;
+        const bool occupied = i < n && counts[i] != 0u;
+        const uint64_t b = ballot(occupied);
+        if (lane == 0)
+            waveTotals[wave] = uint32_t(__builtin_popcountll(b));
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < kRankThreads / 64; w++) {
+            before += w < wave ? waveTotals[w] : 0u;
+            total += waveTotals[w];
+        }
+        if (i < n)
+            rank[i] = carry + before + lanesBelow(b);
+        carry += total;
+        __syncthreads();
+    }
+}
+
+void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
+    occupiedRankKernel<<<1, kRankThreads, 0, stream>>>(v->tileCounts.ptr, v->tileCount, v->occRank.ptr);
 }
 
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {

@@ -50,6 +50,11 @@ __device__ inline void loadWaveFrustum(const FrustumPre& f, uint32_t role, WaveF
     w.degenerate = f.degenerate;
 }
 
+// lanes where `b` holds (HIP's __ballot takes an int, which costs a select and a compare)
+__device__ inline uint64_t ballot(bool b) {
+    return __builtin_amdgcn_ballot_w64(b);
+}
+
 // fold a ballot of per-(role, child) "cull" bits into a 4-bit mask of culled children
 __device__ inline uint32_t foldChildren(uint64_t b) {
     uint64_t x = b | (b >> 32);
@@ -92,7 +97,7 @@ __device__ inline uint32_t waveTestChildren(const WaveFrustum& w, const NodeLane
     } else if (role < 7) {
         cull = !(n.projMax >= w.projLo && n.projNegMin >= w.projHi);
     }
-    return 0xfu & ~foldChildren(__ballot(cull));
+    return 0xfu & ~foldChildren(ballot(cull));
 }
 
 // testBVHNodeChildrenRefine (traverse_avx.h:166-217), one pass over 48 lanes
@@ -109,7 +114,7 @@ __device__ inline uint32_t waveTestRefine(const WaveFrustum& w, const NodeLanes&
         float minBox = fmaf(w.ex, nx, fmaf(w.ey, ny, w.ez * nz));
         cull = (maxBox < w.eMin) || (w.eMax < minBox);
     }
-    return prev & ~foldChildren(__ballot(cull));
+    return prev & ~foldChildren(ballot(cull));
 }
 
 __device__ inline uint32_t uniform(uint32_t v) {

@@ -8,7 +8,8 @@
 //    compiles it in as COLOR_MODE_MSAA_RATE = 16, raycaster_common.h:27);
 //  * errors throw hvvr::RaycasterError instead of printing and exiting;
 //  * render targets are CPU memory (ImageResourceDescriptor::MemoryType::CPU_MEMORY);
-//    D3D11/GL interop, textures and the foveated mode are outside this build.
+//    D3D11/GL interop, textures and the foveated mode are outside this build;
+//  * depth of field follows the camera's lens (setLens / setAperture / setFocalDepth).
 
 #include "types.h"
 

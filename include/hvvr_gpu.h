@@ -97,7 +97,8 @@ typedef struct {
     const void* blockFrusta;        /* SimpleRayFrustum[sampleCount/8192], camera space */
     const int32_t* remap;           /* [width*height*split], sample index or -1 */
     uint32_t width, height, split;  /* output image; split = colour samples per pixel (1..3) */
-    float lensRadius, focalDistance;/* ThinLens; lensRadius must be 0 (pinhole) in this version */
+    float lensRadius, focalDistance;/* ThinLens: radius 0 = pinhole; radius > 0 = depth of field
+                                       (IntersectTriangleThreadDoF, prim_tests.h:247-392), focal > 0 */
 } hvvr_gpu_view_desc;
 
 /* GPUCamera::updateConfig (gpu_camera.cu:149-192). */
@@ -112,11 +113,12 @@ int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float
 /* Several views of one scene in one submission (batched multi-view). */
 int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
                           const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
-/* Tile traversal keeps its distance-sorted stack in LDS, up to `entries` (1..256, default
- * 256); tiles that need more (or more than 768 visited nodes) are re-traversed with a
- * full-depth (4096-entry, the reference's TileFrame::stackSize, traverse_avx.h:281)
- * global-memory stack. The result is the same either way; the limit only moves work
- * between the two paths. */
+/* Tile traversal keeps its distance-sorted stack in registers, up to `entries` (1..64,
+ * default 64; it holds only nodes that still have work). Tiles that need more (or more
+ * than 768 visited nodes, or that meet a NaN distance) are re-traversed with a full-depth
+ * (4096-entry, the reference's TileFrame::stackSize, traverse_avx.h:281) global-memory
+ * stack. The result is the same either way; the limit only moves work between the two
+ * paths. */
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entries);
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);

@@ -8,15 +8,16 @@
 
 namespace {
 
-// ops[i]: 0 = pop (popped[i] = record), 1 = insert (keys[i], record i)
+// ops[i]: 0 = pop (popped[i] = record), 1 = insert (keys[i], record i as the entry's meta).
+// finalT/finalS[q] = key/meta at stack position q (0 = bottom).
 __global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const float* keys, uint32_t n,
                                                    uint32_t* popped, float* finalT, uint32_t* finalS,
                                                    uint32_t* finalSize) {
     using namespace hvvr;
     const uint32_t lane = threadIdx.x;
     RegStack st;
-    st.t0 = st.t1 = st.t2 = st.t3 = 0.0f;
-    st.s0 = st.s1 = st.s2 = st.s3 = 0u;
+    st.t = 0.0f;
+    st.m = 0u;
     st.size = 0;
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t op = uniform(ops[i]);
@@ -26,18 +27,12 @@ __global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const f
             const uint32_t rec = stackPop(st);
             if (lane == 0)
                 popped[i] = rec;
-        } else if (st.size < 256u) {
+        } else if (st.size < REG_STACK_CAP) {
             stackInsert(st, bitsf(uniform(fbits(keys[i]))), i, lane);
         }
     }
-    finalT[lane] = st.t0;
-    finalT[64 + lane] = st.t1;
-    finalT[128 + lane] = st.t2;
-    finalT[192 + lane] = st.t3;
-    finalS[lane] = st.s0;
-    finalS[64 + lane] = st.s1;
-    finalS[128 + lane] = st.s2;
-    finalS[192 + lane] = st.s3;
+    finalT[lane] = st.t;
+    finalS[lane] = st.m;
     if (lane == 0)
         *finalSize = st.size;
 }
@@ -50,7 +45,7 @@ extern "C" int reg_stack_probe_run(const uint32_t* ops, const float* keys, uint3
     float *dKeys = nullptr, *dT = nullptr;
     int rc = 0;
     if (hipMalloc(&dOps, n * 4) || hipMalloc(&dKeys, n * 4) || hipMalloc(&dPopped, n * 4) ||
-        hipMalloc(&dT, 256 * 4) || hipMalloc(&dS, 256 * 4) || hipMalloc(&dSize, 4))
+        hipMalloc(&dT, 64 * 4) || hipMalloc(&dS, 64 * 4) || hipMalloc(&dSize, 4))
         rc = -1;
     if (!rc && (hipMemcpy(dOps, ops, n * 4, hipMemcpyHostToDevice) ||
                 hipMemcpy(dKeys, keys, n * 4, hipMemcpyHostToDevice) ||
@@ -62,8 +57,8 @@ extern "C" int reg_stack_probe_run(const uint32_t* ops, const float* keys, uint3
             rc = -3;
     }
     if (!rc && (hipMemcpy(popped, dPopped, n * 4, hipMemcpyDeviceToHost) ||
-                hipMemcpy(finalT, dT, 256 * 4, hipMemcpyDeviceToHost) ||
-                hipMemcpy(finalS, dS, 256 * 4, hipMemcpyDeviceToHost) ||
+                hipMemcpy(finalT, dT, 64 * 4, hipMemcpyDeviceToHost) ||
+                hipMemcpy(finalS, dS, 64 * 4, hipMemcpyDeviceToHost) ||
                 hipMemcpy(finalSize, dSize, 4, hipMemcpyDeviceToHost)))
         rc = -4;
     (void)hipFree(dOps);

@@ -141,10 +141,6 @@ def test_vanishing_lens_gives_pinhole_ids(H, O, scene1):
 # ---------------------------------------------------------------------------------------
 # GPU parity
 # ---------------------------------------------------------------------------------------
-GPU_DOF_PENDING = pytest.mark.skip(reason="GPU depth-of-field kernels not built yet")
-
-
-@GPU_DOF_PENDING
 @pytest.mark.gpu
 @pytest.mark.parametrize("A,radius,focal", [(4, 0.1, 12.0), (8, 0.05, 6.0), (1, 0.2, 14.0)])
 def test_dof_heightfield_gpu_matches_oracle(H, O, ctx, scene1, A, radius, focal):
@@ -164,7 +160,6 @@ def test_dof_heightfield_gpu_matches_oracle(H, O, ctx, scene1, A, radius, focal)
     assert rgba_delta(gv.read_image(), ref["image"]).max() <= 1
 
 
-@GPU_DOF_PENDING
 @pytest.mark.gpu
 def test_dof_room_gpu_matches_oracle(H, O, ctx, scene2):
     # the foveated/DoF workload's lens: radius 1 cm, focal distance inside the room
