@@ -199,6 +199,10 @@ class HostView:
         self.camera_to_world = np.zeros(16, np.float32)
         self.lens = np.zeros(2, np.float32)
         _check_host(lib.hvvr_host_view_camera(h, _ptr(self.camera_to_world), _ptr(self.lens)))
+        # sample space: sampleToCamera (column-major 3x3) and the sample-position bounds
+        self.sample_to_camera = np.zeros(9, np.float32)
+        self.sample_bounds = np.zeros(4, np.float32)
+        _check_host(lib.hvvr_host_view_sample_space(h, _ptr(self.sample_to_camera), _ptr(self.sample_bounds)))
         lib.hvvr_host_view_destroy(h)
 
     @classmethod
@@ -211,6 +215,60 @@ class HostView:
             _ptr(samples), c_uint32(len(samples)), c_uint32(split), c_uint32(width), c_uint32(height), _ptr(vp),
             c_float(lens_radius), c_float(focal_distance), byref(h)))
         return cls(_handle=h)
+
+
+class HostFoveated:
+    """Eye-space polar foveated samples of a FoveatedSamplePattern (raycaster_spec.h:29-47):
+    clustered beams, eye-space tile/block frusta and each sample's polar texel."""
+
+    # (degreeTrackingError, maxFOVDegrees, marSlope, fovealMARDegrees)
+    DEFAULT = (0.5, 110.0, 0.022, 1.0 / 60.0)
+    FEB2017 = (7.0, 90.0, 0.015, 1.0 / 20.0)
+
+    def __init__(self, pattern=DEFAULT):
+        lib = load_host()
+        h = c_void_p()
+        pat = np.asarray(pattern, np.float32)
+        _check_host(lib.hvvr_host_foveated_create(_ptr(pat), byref(h)))
+        try:
+            counts = np.zeros(6, np.uint64)
+            _check_host(lib.hvvr_host_foveated_counts(h, _ptr(counts)))
+            (self.valid_sample_count, self.sample_count, self.block_count, self.samples_per_ring, self.ring_count,
+             self.tile_count) = (int(c) for c in counts)
+            self.beams = np.zeros(self.sample_count, DTYPES["beam"])
+            self.tile_frusta = np.zeros(self.tile_count, DTYPES["sfrustum"])
+            self.block_frusta = np.zeros(self.block_count, DTYPES["sfrustum"])
+            self.polar_remap = np.zeros((self.valid_sample_count, 2), np.uint32)
+            for idx, arr in enumerate([self.beams, self.tile_frusta, self.block_frusta, self.polar_remap]):
+                _check_host(lib.hvvr_host_foveated_copy(h, c_int(idx), _ptr(arr), c_uint64(arr.nbytes)))
+            self.emap = np.zeros(6, np.float32)
+            mx = c_float(0.0)
+            _check_host(lib.hvvr_host_foveated_params(h, _ptr(self.emap), byref(mx)))
+            self.max_eccentricity = float(mx.value)
+        finally:
+            lib.hvvr_host_foveated_destroy(h)
+
+
+def foveated_history():
+    """Previous-frame state of a foveated camera: worldToEyePrevious (zero) and
+    eyePreviousToSamplePrevious (identity), camera.h:121-122."""
+    h = np.zeros(25, np.float32)
+    h[16] = h[20] = h[24] = 1.0
+    return h
+
+
+def foveated_frame(camera_to_world, eye_dir, sample_to_camera, sample_bounds, history):
+    """One frame's foveated transforms (render.cpp:454-488); advances `history` in place."""
+    lib = load_host()
+    m = np.ascontiguousarray(camera_to_world, np.float32).reshape(16)
+    e = np.asarray(eye_dir, np.float32)
+    s2c = np.ascontiguousarray(sample_to_camera, np.float32).reshape(9)
+    b = np.ascontiguousarray(sample_bounds, np.float32).reshape(4)
+    out = np.zeros(57, np.float32)
+    assert history.dtype == np.float32 and history.size == 25 and history.flags.c_contiguous
+    _check_host(lib.hvvr_host_foveated_frame(_ptr(m), _ptr(e), _ptr(s2c), _ptr(b), _ptr(history), _ptr(out)))
+    return dict(eye_to_world=out[:16].copy(), cull_planes=out[16:32].copy(), sample_to_eye=out[32:41].copy(),
+                eye_to_sample_previous=out[41:57].copy())
 
 
 def subsample_offsets(msaa_rate, spin=0.0):
@@ -355,6 +413,21 @@ def render_views(scene, views, cameras=None, jitters=None):
     _check_gpu(lib.hvvr_gpu_render_views(scene.handle, c_uint32(n), handles, _ptr(ms), _ptr(js)))
 
 
+class _FovDesc(ctypes.Structure):
+    _fields_ = [("polarRemap", c_void_p), ("samplesPerRing", c_uint32), ("ringCount", c_uint32),
+                ("eccentricityMap", c_float * 6), ("maxEccentricity", c_float), ("taaAlpha", c_float),
+                ("taaStddevMultiplier", c_float), ("contrastEnhancement", c_int), ("contrastFe", c_float)]
+
+
+class _FovFrame(ctypes.Structure):
+    _fields_ = [("eyeToWorld", c_float * 16), ("cullPlanes", c_float * 16), ("sampleToEye", c_float * 9),
+                ("eyeToSamplePrevious", c_float * 16)]
+
+
+FOVEATED_BUFFERS = {"polar_color": 0, "polar_depth": 1, "moment1": 2, "moment2": 3, "filtered": 4, "previous": 5,
+                    "blur1": 6, "blur2": 7, "tmax": 8}
+
+
 class GpuView:
     def __init__(self, ctx: GpuContext, view: HostView, msaa_rate=1):
         self.ctx = ctx
@@ -421,3 +494,53 @@ class GpuView:
         if self.handle:
             self.lib.hvvr_gpu_view_destroy(self.handle)
             self.handle = c_void_p()
+
+
+class _FoveatedViewShape:
+    def __init__(self, fov, width, height, lens):
+        self.sample_count, self.valid_sample_count = fov.sample_count, fov.valid_sample_count
+        self.tile_count, self.block_count = fov.tile_count, fov.block_count
+        self.width, self.height, self.split = width, height, 1
+        self.beams, self.tile_frusta, self.block_frusta = fov.beams, fov.tile_frusta, fov.block_frusta
+        self.camera_to_world = np.eye(4, dtype=np.float32).reshape(16)
+        self.lens = np.asarray(lens, np.float32)
+
+
+class GpuFoveatedView(GpuView):
+    """A polar foveated view: eye-space polar samples traced each frame and reconstructed
+    into a width x height screen image (hvvr_gpu_view_set_foveated)."""
+
+    def __init__(self, ctx: GpuContext, fov: HostFoveated, width, height, msaa_rate=1, lens_radius=0.0,
+                 focal_distance=1.0, taa_alpha=0.05, taa_stddev=1.0, contrast=True, contrast_fe=1.0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.fov = fov
+        self.view = _FoveatedViewShape(fov, width, height, (lens_radius, focal_distance))
+        self.msaa_rate = msaa_rate
+        self._keep = [np.ascontiguousarray(a) for a in (fov.beams, fov.tile_frusta, fov.block_frusta)]
+        b, tf, bf = self._keep
+        d = _ViewDesc(msaa_rate, fov.sample_count, fov.valid_sample_count, _ptr(b), _ptr(tf), _ptr(bf), None, width,
+                      height, 1, float(lens_radius), float(focal_distance))
+        self.handle = c_void_p()
+        _check_gpu(self.lib.hvvr_gpu_view_create(ctx.handle, byref(d), byref(self.handle)))
+        remap = np.ascontiguousarray(fov.polar_remap, np.uint32)
+        fd = _FovDesc(_ptr(remap), fov.samples_per_ring, fov.ring_count, (c_float * 6)(*fov.emap),
+                      fov.max_eccentricity, taa_alpha, taa_stddev, 1 if contrast else 0, contrast_fe)
+        _check_gpu(self.lib.hvvr_gpu_view_set_foveated(self.handle, byref(fd)))
+
+    def render_foveated(self, scene: GpuScene, frame, jitter=(0.0, 0.0)):
+        fr = _FovFrame((c_float * 16)(*frame["eye_to_world"]), (c_float * 16)(*frame["cull_planes"]),
+                       (c_float * 9)(*frame["sample_to_eye"]), (c_float * 16)(*frame["eye_to_sample_previous"]))
+        j = np.asarray(jitter, np.float32)
+        _check_gpu(self.lib.hvvr_gpu_view_render_foveated(self.handle, scene.handle, byref(fr), _ptr(j)))
+
+    def read_foveated(self, name):
+        f, v = self.fov, self.view
+        polar = (f.ring_count, f.samples_per_ring)
+        shape, dt = {"polar_color": (polar, np.uint32), "polar_depth": (polar, np.float32),
+                     "moment1": (polar, np.uint64), "moment2": (polar, np.uint64),
+                     "tmax": ((f.sample_count,), np.float32)}.get(name, ((v.height, v.width), np.uint32))
+        out = np.zeros(shape, dt)
+        _check_gpu(self.lib.hvvr_gpu_view_read_foveated(self.handle, c_int(FOVEATED_BUFFERS[name]), _ptr(out),
+                                                        c_uint64(out.nbytes)))
+        return out

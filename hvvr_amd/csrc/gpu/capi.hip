@@ -353,7 +353,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         return fail("sampleCount must be a positive multiple of 8192");
     if (d->validSampleCount > d->sampleCount)
         return fail("validSampleCount exceeds sampleCount");
-    if (!d->beams || !d->tileFrusta || !d->blockFrusta || !d->remap)
+    if (!d->beams || !d->tileFrusta || !d->blockFrusta)
         return fail("view description has null arrays");
     if (d->split < 1 || d->split > 3 || d->width == 0 || d->height == 0)
         return fail("invalid image size or split");
@@ -361,7 +361,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         return fail("lensRadius must be finite and >= 0");
     if (d->lensRadius > 0.0f && (!(d->focalDistance > 0.0f) || !std::isfinite(d->focalDistance)))
         return fail("depth of field needs a finite focalDistance > 0");
-    uint64_t remapCount = uint64_t(d->width) * d->height * d->split;
+    uint64_t remapCount = d->remap ? uint64_t(d->width) * d->height * d->split : 0;
     for (uint64_t i = 0; i < remapCount; i++)
         if (d->remap[i] >= int32_t(d->sampleCount))
             return fail("remap entry out of range");
@@ -450,10 +450,18 @@ int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
     return HVVR_GPU_OK;
 }
 
-static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter, hipStream_t st) {
+// one frame of one view; `fov` (a foveated frame) switches on the screen cull planes, the
+// t-max output and the polar reconstruction in place of the remap
+static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter, hipStream_t st,
+                     const hvvr_gpu_foveated_frame* fov = nullptr) {
     hvvr_gpu_context* c = v->ctx;
     matrix4x4 m;
     memcpy(&m, m16, sizeof(m));
+    v->cull.enabled = 0;
+    if (fov) {
+        memcpy(v->cull.planes, fov->cullPlanes, sizeof(v->cull.planes));
+        v->cull.enabled = std::isinf(fov->cullPlanes[3]) ? 0 : 1;
+    }
     v->lastCameraToWorld = m;
     if (v->jitterSpin && jitter)
         subsampleOffsets(v->msaaRate, jitter[0] * 6.28f, v->offsets);
@@ -479,7 +487,15 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     mark(5);
     launchResolve(v, s, m, st);
     mark(6);
-    launchRemap(v, st);
+    if (fov) {
+        matrix3x3 sampleToEye;
+        matrix4x4 eyeToSamplePrevious;
+        memcpy(&sampleToEye, fov->sampleToEye, sizeof(sampleToEye));
+        memcpy(&eyeToSamplePrevious, fov->eyeToSamplePrevious, sizeof(eyeToSamplePrevious));
+        launchFoveatedReconstruct(v, sampleToEye, eyeToSamplePrevious, st);
+    } else if (v->remap.ptr) {
+        launchRemap(v, st);
+    }
     mark(7);
     HVVR_HIP_CHECK(hipGetLastError());
     v->stagesRecorded = prof;
@@ -498,6 +514,146 @@ int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, 
             return rc;
     }
     return renderOne(v, s, m16, jitter, v->ctx->stream);
+}
+
+static int viewSync(hvvr_gpu_view* v);
+
+// the sRGB byte -> linear table the reconstruction samples through (sRgbToLinear,
+// util_graphics.h:16-18), and byte -> linear -> byte (linearToSRgb, :26-28, truncated)
+static void srgbTables(std::vector<float>& toLinear, std::vector<uint8_t>& roundTrip) {
+    toLinear.resize(256);
+    roundTrip.resize(256);
+    for (int k = 0; k < 256; k++) {
+        float x = float(k) / 255.0f;
+        float lin = (x <= 0.04045f) ? x * (1.0f / 12.92f) : powf((x + .055f) * (1.0f / 1.055f), 2.4f);
+        float enc = (lin <= 0.0031308f) ? (lin * 12.92f) : (powf(lin, 1.0f / 2.4f) * 1.055f - 0.055f);
+        float b = enc * 255.0f;
+        toLinear[k] = lin;
+        roundTrip[k] = uint8_t(b > 0.0f ? (b < 255.0f ? uint32_t(b) : 255u) : 0u);
+    }
+}
+
+int hvvr_gpu_view_set_foveated(hvvr_gpu_view* v, const hvvr_gpu_foveated_desc* d) {
+    if (!v || !d || !d->polarRemap)
+        return fail("null argument");
+    if (v->remap.ptr)
+        return fail("a foveated view is created without a remap");
+    if (d->samplesPerRing == 0 || d->ringCount == 0 ||
+        uint64_t(d->samplesPerRing) * d->ringCount != uint64_t(v->validSampleCount))
+        return fail("samplesPerRing * ringCount must equal the view's validSampleCount");
+    if (!(d->maxEccentricity > 0.0f) || !std::isfinite(d->maxEccentricity))
+        return fail("maxEccentricity must be finite and > 0");
+    for (int k = 0; k < 6; k++)
+        if (!std::isfinite(d->eccentricityMap[k]))
+            return fail("eccentricity map values must be finite");
+    if (!(d->taaAlpha >= 0.0f && d->taaAlpha <= 1.0f) || !std::isfinite(d->taaStddevMultiplier) ||
+        !std::isfinite(d->contrastFe))
+        return fail("invalid temporal filter or contrast settings");
+    // every polar texel is written by exactly one sample
+    std::vector<uint8_t> seen(size_t(d->samplesPerRing) * d->ringCount, 0);
+    for (uint32_t i = 0; i < v->validSampleCount; i++) {
+        uint32_t x = d->polarRemap[2 * i], y = d->polarRemap[2 * i + 1];
+        if (x >= d->samplesPerRing || y >= d->ringCount)
+            return fail("polar remap entry out of range");
+        uint8_t& s = seen[size_t(y) * d->samplesPerRing + x];
+        if (s)
+            return fail("polar remap maps two samples to one texel");
+        s = 1;
+    }
+    FoveatedState& f = v->fov;
+    f.samplesPerRing = d->samplesPerRing;
+    f.ringCount = d->ringCount;
+    memcpy(f.emap, d->eccentricityMap, sizeof(f.emap));
+    f.maxEccentricity = d->maxEccentricity;
+    f.taaAlpha = d->taaAlpha;
+    f.taaStddevMultiplier = d->taaStddevMultiplier;
+    f.contrast = d->contrastEnhancement != 0;
+    f.contrastFe = d->contrastFe;
+    f.historyIndex = 0;
+    const size_t polar = size_t(d->samplesPerRing) * d->ringCount;
+    const size_t pixels = size_t(v->width) * v->height;
+    std::vector<float> toLinear;
+    std::vector<uint8_t> roundTrip;
+    srgbTables(toLinear, roundTrip);
+    hipStream_t st = v->ctx->stream;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess)
+            e = r;
+    };
+    step(upload(f.polarRemap, d->polarRemap, v->validSampleCount, st));
+    step(upload(f.srgbToLinear, toLinear.data(), 256, st));
+    step(upload(f.srgbRoundTrip, roundTrip.data(), 256, st));
+    step(f.polarColor.resize(polar));
+    step(f.polarDepth.resize(polar));
+    step(f.moment1.resize(polar));
+    step(f.moment2.resize(polar));
+    for (int k = 0; k < 2; k++) {
+        step(f.history[k].resize(pixels));
+        step(f.filtered[k].resize(pixels));
+    }
+    step(v->tMax.resize(v->sampleCount));
+    for (int k = 0; k < 2; k++) {
+        if (e == hipSuccess)
+            e = hipMemsetAsync(f.history[k].ptr, 0, f.history[k].bytes(), st);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(f.filtered[k].ptr, 0, f.filtered[k].bytes(), st);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        v->foveated = false;
+        return fail(std::string("foveated allocation failed: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+    }
+    v->foveated = true;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_render_foveated(hvvr_gpu_view* v, hvvr_gpu_scene* s, const hvvr_gpu_foveated_frame* fr,
+                                  const float* jitter) {
+    if (!v || !s || !fr)
+        return fail("null argument");
+    if (!v->foveated)
+        return fail("view is not foveated: call hvvr_gpu_view_set_foveated first", HVVR_GPU_ERR_STATE);
+    if (v->ctx != s->ctx)
+        return fail("view and scene belong to different contexts");
+    if (!s->everUpdated || s->dirty) {
+        int rc = hvvr_gpu_scene_update(s);
+        if (rc)
+            return rc;
+    }
+    return renderOne(v, s, fr->eyeToWorld, jitter, v->ctx->stream, fr);
+}
+
+int hvvr_gpu_view_read_foveated(hvvr_gpu_view* v, int buffer, void* dst, uint64_t bytes) {
+    if (!dst)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    if (!v->foveated)
+        return fail("view is not foveated", HVVR_GPU_ERR_STATE);
+    FoveatedState& f = v->fov;
+    // the last frame wrote history[historyIndex ^ 1] (the index has advanced since)
+    const uint32_t last = f.historyIndex ^ 1u;
+    const void* src = nullptr;
+    uint64_t n = 0;
+    switch (buffer) {
+        case HVVR_GPU_FOVEATED_POLAR_COLOR: src = f.polarColor.ptr; n = f.polarColor.bytes(); break;
+        case HVVR_GPU_FOVEATED_POLAR_DEPTH: src = f.polarDepth.ptr; n = f.polarDepth.bytes(); break;
+        case HVVR_GPU_FOVEATED_MOMENT1: src = f.moment1.ptr; n = f.moment1.bytes(); break;
+        case HVVR_GPU_FOVEATED_MOMENT2: src = f.moment2.ptr; n = f.moment2.bytes(); break;
+        case HVVR_GPU_FOVEATED_FILTERED: src = f.history[last].ptr; n = f.history[last].bytes(); break;
+        case HVVR_GPU_FOVEATED_PREVIOUS: src = f.history[last ^ 1u].ptr; n = f.history[last ^ 1u].bytes(); break;
+        case HVVR_GPU_FOVEATED_BLUR1: src = f.filtered[0].ptr; n = f.filtered[0].bytes(); break;
+        case HVVR_GPU_FOVEATED_BLUR2: src = f.filtered[1].ptr; n = f.filtered[1].bytes(); break;
+        case HVVR_GPU_FOVEATED_TMAX: src = v->tMax.ptr; n = v->tMax.bytes(); break;
+        default: return fail("unknown foveated buffer");
+    }
+    if (bytes != n)
+        return fail("foveated buffer size mismatch: expected " + std::to_string(n) + " bytes");
+    HVVR_HIP_CHECK(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+    return HVVR_GPU_OK;
 }
 
 int hvvr_gpu_render_views(hvvr_gpu_scene* s, uint32_t count, hvvr_gpu_view* const* views, const float* m16,

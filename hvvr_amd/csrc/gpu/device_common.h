@@ -31,6 +31,10 @@ __device__ inline uint32_t floatUnflip(uint32_t f) {
 __device__ inline float maxTotal(float a, float b) {
     return floatFlip(fbits(a)) >= floatFlip(fbits(b)) ? a : b;
 }
+// linearToSRgb (shared/util_graphics.h:26-28)
+__device__ inline float linearToSRgb(float x) {
+    return (x <= 0.0031308f) ? (x * 12.92f) : (powf(x, 1.0f / 2.4f) * 1.055f - 0.055f);
+}
 
 // Frustum precompute from world-space corner rays (origins o, directions d).
 __device__ inline void frustumPrecompute(const vector3 o[4], const vector3 d[4], FrustumPre& f) {

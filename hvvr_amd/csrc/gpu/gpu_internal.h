@@ -57,6 +57,32 @@ struct RefitLevel {
     uint32_t start, count;
 };
 
+// world-space planes (normal, dist); a tile frustum wholly on the positive side of one
+// is culled (render.cpp:271-279, 397-416)
+struct CullPlanes {
+    vector4 planes[4];
+    int enabled;
+};
+
+// polar foveated reconstruction of one view (raycaster/gpu_foveated.cu)
+struct FoveatedState {
+    uint32_t samplesPerRing = 0, ringCount = 0; // polar image: width = azimuth columns, height = rings
+    float emap[6] = {};                         // EccentricityMap (m, w0, S, 1/log(1+m), 1/w0, 1/m)
+    float maxEccentricity = 0.0f;
+    float taaAlpha = 0.05f, taaStddevMultiplier = 1.0f; // TemporalFilterSettings (gpu_camera.h:41-45)
+    bool contrast = true;                                // ContrastEnhancementSettings (gpu_camera.cu:180-181)
+    float contrastFe = 1.0f;
+    uint32_t historyIndex = 0;                           // which history image this frame writes
+    DeviceBuffer<vector2ui> polarRemap;                  // [validSampleCount]
+    DeviceBuffer<uint32_t> polarColor;                   // [rings][samplesPerRing] RGBA8 sRGB
+    DeviceBuffer<float> polarDepth;                      // [rings][samplesPerRing]
+    DeviceBuffer<uint64_t> moment1, moment2;             // [rings][samplesPerRing] RGBA16 unorm
+    DeviceBuffer<uint32_t> history[2];                   // [height][width] RGBA8 sRGB, this and the last frame
+    DeviceBuffer<uint32_t> filtered[2];                  // contrast enhancement: after the first and second pass
+    DeviceBuffer<float> srgbToLinear;                    // [256] sRGB byte -> linear
+    DeviceBuffer<uint8_t> srgbRoundTrip;                 // [256] byte -> linear -> byte (the polar remap)
+};
+
 } // namespace hvvr
 
 struct hvvr_gpu_context {
@@ -118,6 +144,12 @@ struct hvvr_gpu_view {
                                                         // [3] list entries allocated, [4] node visits
     uint64_t listCapacity = 0;
     uint32_t waveStackLimit = 64; // register stack entries per tile traversal (<= 64)
+    // foveated path (hvvr_gpu_view_set_foveated): cull planes and the t-max buffer apply to
+    // foveated renders only
+    bool foveated = false;
+    hvvr::CullPlanes cull = {};
+    hvvr::DeviceBuffer<float> tMax; // [sampleCount]
+    hvvr::FoveatedState fov;
     hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
     bool stagesRecorded = false;
     bool rendered = false;
@@ -140,5 +172,9 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream);
 void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchRemap(hvvr_gpu_view* v, hipStream_t stream);
+// foveated reconstruction: polar remap, moments, reprojection + temporal filter, contrast
+// enhancement; `sampleToEye` and `eyeToSamplePrevious` as in hvvr_gpu_foveated_frame
+void launchFoveatedReconstruct(hvvr_gpu_view* v, const matrix3x3& sampleToEye, const matrix4x4& eyeToSamplePrevious,
+                               hipStream_t stream);
 
 } // namespace hvvr

@@ -38,7 +38,58 @@ struct ResolveArgs {
     const vector2* lensTable;
     const uint32_t* occRank;
     float lensRadius, focalDistance;
+    // foveated path: per sample, the view-axis distance of subsample 0's hit (infinity on
+    // a miss; ResolveKernel's TMaxBuffer, resolve.cu:386-457); null = not written
+    float* tMax;
 };
+
+// IntersectTriangleThread::calcUVW (prim_tests.h:233-245): barycentrics of the ray at
+// subsample offset alpha; ndir/ndu/ndv are the negated world-space beam vectors
+__device__ inline void pinholeUVW(const PrecomputedTriangleIntersect& T, const vector3& cam, const vector3& ndir,
+                                  const vector3& ndu, const vector3& ndv, float cx, float cy, float& bx, float& by,
+                                  float& bz) {
+    vector3 normal = cross(T.edge0, T.edge1);
+    vector3 v0o = cam - T.v0;
+    float denomC = dot(ndir, normal);
+    vector3 eC = cross(ndir, v0o);
+    float vC = dot(T.edge1, eC);
+    float wC = -dot(T.edge0, eC);
+    float dD0 = dot(ndu, normal), dD1 = dot(ndv, normal);
+    vector3 cu = cross(ndu, v0o), cv = cross(ndv, v0o);
+    float dV0 = dot(T.edge1, cu), dV1 = dot(T.edge1, cv);
+    float dW0 = -dot(T.edge0, cu), dW1 = -dot(T.edge0, cv);
+    float denom = fmaf(dD1, cy, fmaf(dD0, cx, denomC));
+    float v = fmaf(dV1, cy, fmaf(dV0, cx, vC));
+    float w = fmaf(dW1, cy, fmaf(dW0, cx, wC));
+    float u = (denom - v) - w;
+    float dinv = 1.0f / denom;
+    bx = u * dinv;
+    by = v * dinv;
+    bz = w * dinv;
+}
+
+// IntersectTriangleTile<true>::setup + IntersectTriangleThreadDoF::calcUVW
+// (prim_tests.h:263-291, 366-392) at lens position (lx, ly) and direction offset (dx, dy)
+__device__ inline void dofUVW(const PrecomputedTriangleIntersect& T, const vector3& cam, const DofTile& tileLens,
+                              const vector3& lcfc, float lx, float ly, float dx, float dy, float& bx, float& by,
+                              float& bz) {
+    const vector3 lu = tileLens.lensU, lv = tileLens.lensV;
+    const vector3 normal = cross(T.edge0, T.edge1);
+    const vector3 v0ToLens = cam - T.v0;
+    const float dDA0 = dot(-lu, normal), dDA1 = dot(-lv, normal);
+    const float denomC = dot(-lcfc, normal);
+    const float denom = fmaf(dDA1, dy, fmaf(dDA0, dx, denomC));
+    const vector3 o = axpy2(v0ToLens, lx, lu, ly, lv);
+    const vector3 rd = axpy2(lcfc, dx, lu, dy, lv);
+    const vector3 e = crossFma(-rd, o);
+    const float v = dotFma(T.edge1, e);
+    const float w = -dotFma(T.edge0, e);
+    const float u = (denom - v) - w;
+    const float dinv = 1.0f / denom;
+    bx = u * dinv;
+    by = v * dinv;
+    bz = w * dinv;
+}
 
 __device__ inline vector3 ggxSpecular(vector3 N, vector3 L, vector3 V, float roughness, vector3 F0) {
     // GGX_SchlickFresnel_SchlickSmithGGX (shading_helpers.h:89-112)
@@ -119,10 +170,6 @@ __device__ inline float acesFilm(float x) {
     const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
     return saturate((x * (a * x + b)) / (x * (c * x + d) + e));
 }
-__device__ inline float linearToSRgb(float x) {
-    // util_graphics.h:26-28
-    return (x <= 0.0031308f) ? (x * 12.92f) : (powf(x, 1.0f / 2.4f) * 1.055f - 0.055f);
-}
 
 template <int A, bool DoF>
 __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
@@ -131,7 +178,10 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
     const uint32_t tid = threadIdx.x;
     const uint32_t sample = tile * TILE_SIZE + tid;
     if (args.tileCounts[tile] == 0) {
+        // ClearEmptyKernel (resolve.cu:527-542)
         args.sampleResults[sample] = 0xff000000u;
+        if (args.tMax)
+            args.tMax[sample] = INFINITY;
         return;
     }
     const uint32_t lane = tid & 63u;
@@ -158,6 +208,7 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
 
     vector3 result(0.0f);
     uint32_t combined = 0;
+    uint32_t tri0 = badTriIndex; // the triangle subsample 0 hit
     for (int slot = 0; slot < A; slot++) {
         if (combined == maskAll)
             break;
@@ -166,6 +217,8 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
             break;
         uint32_t mask = g.sampleMask;
         combined |= mask;
+        if (mask & 1u)
+            tri0 = g.triIndex;
         float inv = 1.0f / float(__popc(mask));
         float cx = 0.0f, cy = 0.0f;
         float lx = 0.0f, ly = 0.0f, dx = 0.0f, dy = 0.0f;
@@ -186,45 +239,10 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
         }
         PrecomputedTriangleIntersect T = args.tris[g.triIndex];
         float bx, by, bz;
-        if constexpr (DoF) {
-            // IntersectTriangleTile<true>::setup + IntersectTriangleThreadDoF::calcUVW
-            // (prim_tests.h:263-291, 366-392) at the centroid
-            const vector3 lu = dofTile.lensU, lv = dofTile.lensV;
-            const vector3 normal = cross(T.edge0, T.edge1);
-            const vector3 v0ToLens = cam - T.v0;
-            const float dDA0 = dot(-lu, normal), dDA1 = dot(-lv, normal);
-            const float denomC = dot(-lcfc, normal);
-            const float denom = fmaf(dDA1, dy, fmaf(dDA0, dx, denomC));
-            const vector3 o = axpy2(v0ToLens, lx, lu, ly, lv);
-            const vector3 rd = axpy2(lcfc, dx, lu, dy, lv);
-            const vector3 e = crossFma(-rd, o);
-            const float v = dotFma(T.edge1, e);
-            const float w = -dotFma(T.edge0, e);
-            const float u = (denom - v) - w;
-            const float dinv = 1.0f / denom;
-            bx = u * dinv;
-            by = v * dinv;
-            bz = w * dinv;
-        } else {
-            vector3 normal = cross(T.edge0, T.edge1);
-            vector3 v0o = cam - T.v0;
-            float denomC = dot(ndir, normal);
-            vector3 eC = cross(ndir, v0o);
-            float vC = dot(T.edge1, eC);
-            float wC = -dot(T.edge0, eC);
-            float dD0 = dot(ndu, normal), dD1 = dot(ndv, normal);
-            vector3 cu = cross(ndu, v0o), cv = cross(ndv, v0o);
-            float dV0 = dot(T.edge1, cu), dV1 = dot(T.edge1, cv);
-            float dW0 = -dot(T.edge0, cu), dW1 = -dot(T.edge0, cv);
-            float denom = fmaf(dD1, cy, fmaf(dD0, cx, denomC));
-            float v = fmaf(dV1, cy, fmaf(dV0, cx, vC));
-            float w = fmaf(dW1, cy, fmaf(dW0, cx, wC));
-            float u = (denom - v) - w;
-            float dinv = 1.0f / denom;
-            bx = u * dinv;
-            by = v * dinv;
-            bz = w * dinv;
-        }
+        if constexpr (DoF)
+            dofUVW(T, cam, dofTile, lcfc, lx, ly, dx, dy, bx, by, bz); // at the centroid
+        else
+            pinholeUVW(T, cam, ndir, ndu, ndv, cx, cy, bx, by, bz);
         PrecomputedTriangleShade S = args.triShade[g.triIndex];
         ShadingVertex V0 = args.verts[S.indices[0]], V1 = args.verts[S.indices[1]], V2 = args.verts[S.indices[2]];
         vector3 pos = V0.pos * bx + V1.pos * by + V2.pos * bz;
@@ -241,6 +259,27 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
     float b = linearToSRgb(acesFilm(result.z));
     uint32_t ri = uint32_t(r * 255.0f), gi = uint32_t(gch * 255.0f), bi = uint32_t(b * 255.0f);
     args.sampleResults[sample] = ri | (gi << 8) | (bi << 16) | 0xff000000u;
+
+    if (args.tMax) {
+        // hit point of subsample 0 from its barycentrics, projected on the view axis
+        float t = INFINITY;
+        if (tri0 != badTriIndex) {
+            PrecomputedTriangleIntersect T = args.tris[tri0];
+            float bx, by, bz;
+            const vector2 a0 = args.offsets[0];
+            if constexpr (DoF) {
+                const vector2 lensUV = args.lensTable[lensBase];
+                const vector2 dirUV = dofDirUV(a0, dofTile.focalToLensScale, lensUV);
+                dofUVW(T, cam, dofTile, lcfc, lensUV.x, lensUV.y, dirUV.x, dirUV.y, bx, by, bz);
+            } else {
+                pinholeUVW(T, cam, ndir, ndu, ndv, a0.x, a0.y, bx, by, bz);
+            }
+            const vector3 v1 = T.v0 + T.edge0, v2 = T.v0 + T.edge1;
+            const vector3 pos = T.v0 * bx + v1 * by + v2 * bz;
+            t = dot(pos - cam, -m.m2.xyz());
+        }
+        args.tMax[sample] = t;
+    }
 }
 
 __global__ void remapKernel(const uint32_t* __restrict__ src, const int32_t* __restrict__ remap,
@@ -286,6 +325,7 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
     a.occRank = v->occRank.ptr;
     a.lensRadius = v->lens.radius;
     a.focalDistance = v->lens.focalDistance;
+    a.tMax = v->foveated ? v->tMax.ptr : nullptr;
     dim3 grid(v->tileCount), block(128);
     const bool dof = v->dof;
     switch (v->msaaRate) {

@@ -21,10 +21,22 @@
 
 namespace hvvr {
 
+// planeCullsFrustum (render.cpp:271-279): every corner ray, at its origin and 10^4 along
+// it, lies on the positive side of the plane
+__device__ inline bool planeCullsFrustum(const vector4& p, const vector3 o[4], const vector3 d[4]) {
+    const vector3 n(p.x, p.y, p.z);
+    bool allOut = true;
+    for (int v = 0; v < 4; v++) {
+        allOut = allOut && dot(n, o[v]) > p.w;
+        allOut = allOut && dot(n, o[v] + d[v] * 10000.0f) > p.w;
+    }
+    return allOut;
+}
+
 __global__ void frustumSetupKernel(const SimpleRayFrustum* __restrict__ tileCam, uint32_t tileCount,
                                    const SimpleRayFrustum* __restrict__ blockCam, uint32_t blockCount, matrix4x4 m,
-                                   SimpleRayFrustum* __restrict__ tileWorld, FrustumPre* __restrict__ tilePre,
-                                   FrustumPre* __restrict__ blockPre) {
+                                   CullPlanes cull, SimpleRayFrustum* __restrict__ tileWorld,
+                                   FrustumPre* __restrict__ tilePre, FrustumPre* __restrict__ blockPre) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= tileCount + blockCount)
         return;
@@ -38,6 +50,17 @@ __global__ void frustumSetupKernel(const SimpleRayFrustum* __restrict__ tileCam,
         vector4 t = m * vector4(src.origins[v], 1.0f);
         o[v] = culled ? vector3(INFINITY, INFINITY, INFINITY) : vector3(t.x, t.y, t.z);
         d[v] = culled ? vector3(0.0f, 0.0f, 0.0f) : r * src.directions[v];
+    }
+    // foveated path: tiles wholly outside the screen rectangle are culled the same way
+    // (blockTransformAndCullTask, render.cpp:311-340)
+    if (isTile && cull.enabled && !culled) {
+        for (int k = 0; k < 4; k++)
+            culled = culled || planeCullsFrustum(cull.planes[k], o, d);
+        if (culled)
+            for (int v = 0; v < 4; v++) {
+                o[v] = vector3(INFINITY, INFINITY, INFINITY);
+                d[v] = vector3(0.0f, 0.0f, 0.0f);
+            }
     }
     FrustumPre f;
     frustumPrecompute(o, d, f);
@@ -647,8 +670,8 @@ __global__ void fallbackAllocKernel(const uint32_t* __restrict__ overflowTiles, 
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream) {
     uint32_t n = v->tileCount + v->blockCount;
     frustumSetupKernel<<<(n + 127) / 128, 128, 0, stream>>>(v->tileFrustaCam.ptr, v->tileCount, v->blockFrustaCam.ptr,
-                                                            v->blockCount, m, v->tileFrustaWorld.ptr, v->tilePre.ptr,
-                                                            v->blockPre.ptr);
+                                                            v->blockCount, m, v->cull, v->tileFrustaWorld.ptr,
+                                                            v->tilePre.ptr, v->blockPre.ptr);
 }
 
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {

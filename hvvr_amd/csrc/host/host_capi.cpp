@@ -36,6 +36,11 @@ struct hvvr_host_view {
     SampleSetup setup;
     matrix4x4 cameraToWorld = matrix4x4::identity();
     ThinLens lens = {0.0f, 1.0f};
+    matrix3x3 sampleToCam = matrix3x3::identity();
+};
+
+struct hvvr_host_foveated {
+    FoveatedSetup setup;
 };
 
 extern "C" {
@@ -166,8 +171,9 @@ int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint3
     if (cam.barrelK1 != 0.0f || cam.barrelK2 != 0.0f)
         applyBarrelDistortion(samples, cam.barrelK1, cam.barrelK2);
     hvvr_host_view* v = new hvvr_host_view();
-    buildSampleSetup(samples.data(), uint32_t(samples.size()), 1, cam.width, cam.height,
-                     sampleToCamera(cam.viewportLower, cam.viewportUpper), cam.lens, v->setup);
+    v->sampleToCam = sampleToCamera(cam.viewportLower, cam.viewportUpper);
+    buildSampleSetup(samples.data(), uint32_t(samples.size()), 1, cam.width, cam.height, v->sampleToCam, cam.lens,
+                     v->setup);
     v->cameraToWorld = matrix4x4(cam.cameraToWorld);
     v->lens = cam.lens;
     *out = v;
@@ -181,8 +187,8 @@ int hvvr_host_view_create_from_samples(const void* samples, uint32_t count, uint
         return fail("invalid argument");
     hvvr_host_view* v = new hvvr_host_view();
     ThinLens lens = {lensRadius, focalDistance};
-    buildSampleSetup((const Sample*)samples, count, split, width, height,
-                     sampleToCamera(vector2(vp[0], vp[1]), vector2(vp[2], vp[3])), lens, v->setup);
+    v->sampleToCam = sampleToCamera(vector2(vp[0], vp[1]), vector2(vp[2], vp[3]));
+    buildSampleSetup((const Sample*)samples, count, split, width, height, v->sampleToCam, lens, v->setup);
     v->lens = lens;
     *out = v;
     return 0;
@@ -226,6 +232,102 @@ int hvvr_host_view_camera(const hvvr_host_view* v, float m[16], float lens[2]) {
         lens[0] = v->lens.radius;
         lens[1] = v->lens.focalDistance;
     }
+    return 0;
+}
+
+int hvvr_host_view_sample_space(const hvvr_host_view* v, float s2c[9], float bounds[4]) {
+    if (!v)
+        return fail("null argument");
+    if (s2c)
+        memcpy(s2c, &v->sampleToCam, 36);
+    if (bounds) {
+        bounds[0] = v->setup.boundsLower.x;
+        bounds[1] = v->setup.boundsLower.y;
+        bounds[2] = v->setup.boundsUpper.x;
+        bounds[3] = v->setup.boundsUpper.y;
+    }
+    return 0;
+}
+
+int hvvr_host_foveated_create(const float pattern[4], hvvr_host_foveated** out) {
+    if (!out || !pattern)
+        return fail("null argument");
+    FoveatedSamplePattern p;
+    p.degreeTrackingError = pattern[0];
+    p.maxFOVDegrees = pattern[1];
+    p.marSlope = pattern[2];
+    p.fovealMARDegrees = pattern[3];
+    hvvr_host_foveated* f = new hvvr_host_foveated();
+    std::string err;
+    if (!buildFoveatedSetup(p, f->setup, &err)) {
+        delete f;
+        return fail(err);
+    }
+    *out = f;
+    return 0;
+}
+
+void hvvr_host_foveated_destroy(hvvr_host_foveated* f) {
+    delete f;
+}
+
+int hvvr_host_foveated_counts(const hvvr_host_foveated* f, uint64_t counts[6]) {
+    if (!f || !counts)
+        return fail("null argument");
+    counts[0] = f->setup.validSampleCount;
+    counts[1] = f->setup.sampleCount;
+    counts[2] = f->setup.blockCount;
+    counts[3] = f->setup.samplesPerRing;
+    counts[4] = f->setup.ringCount;
+    counts[5] = f->setup.tileFrusta.size();
+    return 0;
+}
+
+int hvvr_host_foveated_copy(const hvvr_host_foveated* f, int array, void* dst, uint64_t bytes) {
+    if (!f || !dst)
+        return fail("null argument");
+    switch (array) {
+        case HVVR_FOVEATED_BEAMS: return copyOut(f->setup.beams, dst, bytes);
+        case HVVR_FOVEATED_TILE_FRUSTA: return copyOut(f->setup.tileFrusta, dst, bytes);
+        case HVVR_FOVEATED_BLOCK_FRUSTA: return copyOut(f->setup.blockFrusta, dst, bytes);
+        case HVVR_FOVEATED_POLAR_REMAP: return copyOut(f->setup.polarRemap, dst, bytes);
+        default: return fail("unknown foveated array");
+    }
+}
+
+int hvvr_host_foveated_params(const hvvr_host_foveated* f, float emap[6], float* maxEccentricity) {
+    if (!f)
+        return fail("null argument");
+    if (emap) {
+        const EccentricityMap& e = f->setup.emap;
+        float v[6] = {e.m, e.w0, e.S, e.invLogA, e.invW0, e.invM};
+        memcpy(emap, v, sizeof(v));
+    }
+    if (maxEccentricity)
+        *maxEccentricity = f->setup.maxEccentricity;
+    return 0;
+}
+
+int hvvr_host_foveated_frame(const float cameraToWorld[16], const float eyeDir[3], const float s2c[9],
+                             const float bounds[4], float history[25], float out[57]) {
+    if (!cameraToWorld || !eyeDir || !s2c || !bounds || !history || !out)
+        return fail("null argument");
+    matrix4x4 m;
+    memcpy(&m, cameraToWorld, 64);
+    matrix3x3 sampleToCam;
+    memcpy(&sampleToCam, s2c, 36);
+    FoveatedHistory h;
+    memcpy(&h.worldToEyePrevious, history, 64);
+    memcpy(&h.eyePreviousToSamplePrevious, history + 16, 36);
+    FoveatedFrame fr;
+    foveatedFrame(m, vector3(eyeDir[0], eyeDir[1], eyeDir[2]), sampleToCam, vector2(bounds[0], bounds[1]),
+                  vector2(bounds[2], bounds[3]), h, fr);
+    memcpy(history, &h.worldToEyePrevious, 64);
+    memcpy(history + 16, &h.eyePreviousToSamplePrevious, 36);
+    memcpy(out, &fr.eyeToWorld, 64);
+    memcpy(out + 16, fr.cullPlanes, 64);
+    memcpy(out + 32, &fr.sampleToEye, 36);
+    memcpy(out + 41, &fr.eyeToSamplePrevious, 64);
     return 0;
 }
 

@@ -5,6 +5,7 @@
 
 #include "hvvr/raycaster.h"
 #include "hvvr/types.h"
+#include "../common/foveated.h"
 #include "../common/subsample.h"
 
 #include <stdint.h>
@@ -62,6 +63,43 @@ matrix3x3 sampleToCamera(vector2 viewportLower, vector2 viewportUpper);
 // camera.cpp:21-64 + sample_hierarchy.cpp:149-293 (perspective mapping)
 void buildSampleSetup(const Sample* samples, uint32_t count, uint32_t split, uint32_t width, uint32_t height,
                       const matrix3x3& sampleToCam, ThinLens lens, SampleSetup& out);
+
+// ---------------------------------------------------------------------------------
+// polar foveated path (raycaster/foveated.cpp, samples.cpp:44-142, frusta.cu:26-121)
+// ---------------------------------------------------------------------------------
+// Eye-space polar samples, ring-major: sample r * samplesPerRing + i is azimuth column i
+// of ring r. False (with a message) for a pattern outside the supported range.
+bool polarFoveatedSamples(const FoveatedSamplePattern& p, EccentricityMap& emap, uint32_t& samplesPerRing,
+                          uint32_t& ringCount, std::vector<DirectionalBeam>& out, std::string* error);
+
+struct FoveatedSetup {
+    FoveatedSamplePattern pattern;
+    EccentricityMap emap;
+    float maxEccentricity = 0.0f; // radians
+    uint32_t samplesPerRing = 0, ringCount = 0;
+    uint32_t validSampleCount = 0, sampleCount = 0, blockCount = 0;
+    std::vector<DirectionalBeam> beams;          // [sampleCount], eye space, clustered; padding repeats the last
+    std::vector<SimpleRayFrustum> tileFrusta;    // [blockCount * 64], eye space; degenerate past the samples
+    std::vector<SimpleRayFrustum> blockFrusta;   // [blockCount]
+    std::vector<vector2ui> polarRemap;           // [validSampleCount] polar texel (column, ring) of each sample
+};
+bool buildFoveatedSetup(const FoveatedSamplePattern& p, FoveatedSetup& out, std::string* error);
+
+// previous-frame state of a foveated camera (camera.h:121-122)
+struct FoveatedHistory {
+    matrix4x4 worldToEyePrevious = matrix4x4(vector4(0.0f), vector4(0.0f), vector4(0.0f), vector4(0.0f));
+    matrix3x3 eyePreviousToSamplePrevious = matrix3x3::identity();
+};
+struct FoveatedFrame {
+    matrix4x4 eyeToWorld;          // beam (eye) space to world
+    vector4 cullPlanes[4];         // world space (normal, dist): tiles wholly outside one are culled
+    matrix3x3 sampleToEye;         // screen sample space to eye space
+    matrix4x4 eyeToSamplePrevious; // eye space to the previous frame's sample space
+};
+// One frame's transforms from the camera, the gaze direction (camera space) and the
+// camera's sample bounds; advances `history`.
+void foveatedFrame(const matrix4x4& cameraToWorld, const vector3& eyeDir, const matrix3x3& sampleToCam,
+                   vector2 boundsLower, vector2 boundsUpper, FoveatedHistory& history, FoveatedFrame& out);
 
 // ---------------------------------------------------------------------------------
 // seeded generators

@@ -10,6 +10,7 @@
 //   * the image-location -> sample remap (camera.cpp:57-63).
 
 #include "host_internal.h"
+#include "cluster.h"
 
 #include <algorithm>
 #include <limits>
@@ -72,39 +73,6 @@ struct SortedSample {
     Sample s;
     uint32_t channel;
 };
-
-bool yThenX(const SortedSample& a, const SortedSample& b) {
-    return (a.s.position.y < b.s.position.y) || (a.s.position.y == b.s.position.y && a.s.position.x < b.s.position.x);
-}
-bool xThenY(const SortedSample& a, const SortedSample& b) {
-    return (a.s.position.x < b.s.position.x) || (a.s.position.x == b.s.position.x && a.s.position.y < b.s.position.y);
-}
-
-void sortRange(std::vector<SortedSample>& v, size_t begin, size_t end, bool (*cmp)(const SortedSample&,
-                                                                                    const SortedSample&)) {
-    begin = std::min(begin, v.size());
-    end = std::min(end, v.size());
-    if (begin < end)
-        std::stable_sort(v.begin() + begin, v.begin() + end, cmp);
-}
-
-// sample_clustering.h:17-49: y strata of blocks, blocks sorted y-then-x, then rows of
-// 8 tiles sorted x-then-y
-void naiveXYCluster(std::vector<SortedSample>& v, size_t k) {
-    size_t yStrata = (size_t)sqrt((double)k);
-    if (yStrata == 0)
-        yStrata = 1;
-    size_t xStrata = (k + yStrata - 1) / yStrata;
-    size_t blockYStrata = (size_t)sqrt((double)TILES_PER_BLOCK);
-    size_t blockXStrata = (TILES_PER_BLOCK + blockYStrata - 1) / blockYStrata;
-    sortRange(v, 0, v.size(), yThenX);
-    for (size_t i = 0; i < yStrata; i++)
-        sortRange(v, i * BLOCK_SIZE * xStrata, (i + 1) * BLOCK_SIZE * xStrata, xThenY);
-    for (size_t i = 0; i < k; i++)
-        sortRange(v, i * BLOCK_SIZE, (i + 1) * BLOCK_SIZE, yThenX);
-    for (size_t i = 0; i < v.size(); i += TILE_SIZE * blockXStrata)
-        sortRange(v, i, i + TILE_SIZE * blockXStrata, xThenY);
-}
 
 struct Unpacked {
     vector2 center, majorAxis, minorAxis;
@@ -192,7 +160,7 @@ void buildSampleSetup(const Sample* raw, uint32_t count, uint32_t split, uint32_
     for (uint32_t n = 0; n < count; n++)
         sorted[n] = {raw[n], n % split};
     uint32_t blockCount = (count + BLOCK_SIZE - 1) / BLOCK_SIZE;
-    naiveXYCluster(sorted, blockCount);
+    naiveXYCluster(sorted, blockCount, [](const SortedSample& x) { return x.s.position; });
 
     out.boundsLower = vector2(1.0f, 1.0f);
     out.boundsUpper = vector2(0.0f, 0.0f);

@@ -10,8 +10,6 @@ namespace hvvr {
 
 namespace {
 
-const float kPi = 3.141592653589f;
-
 ShadingVertex makeVertex(vector3 p, vector3 n, vector2 uv) {
     ShadingVertex v;
     v.pos = p;

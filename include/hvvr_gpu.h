@@ -95,7 +95,8 @@ typedef struct {
     const void* beams;              /* DirectionalBeam[sampleCount], camera space */
     const void* tileFrusta;         /* SimpleRayFrustum[sampleCount/128], camera space */
     const void* blockFrusta;        /* SimpleRayFrustum[sampleCount/8192], camera space */
-    const int32_t* remap;           /* [width*height*split], sample index or -1 */
+    const int32_t* remap;           /* [width*height*split], sample index or -1; NULL = no remap
+                                       (a foveated view, whose image is its reconstruction) */
     uint32_t width, height, split;  /* output image; split = colour samples per pixel (1..3) */
     float lensRadius, focalDistance;/* ThinLens: radius 0 = pinhole; radius > 0 = depth of field
                                        (IntersectTriangleThreadDoF, prim_tests.h:247-392), focal > 0 */
@@ -123,6 +124,51 @@ int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entrie
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
 
+/* ---- polar foveated rendering (RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT,
+ * render.cpp:454-488, gpu_foveated.cu, remap.cu:127-158) ----
+ * A foveated view is created with eye-space polar beams and frusta (hvvr_host_foveated_*)
+ * and remap NULL; width x height is the reconstruction target. Each frame traces the
+ * polar samples, scatters them into the polar image (ring x azimuth), and reconstructs
+ * the screen image with a bicubic polar lookup, a reprojected and neighbourhood-clamped
+ * temporal blend, and eccentricity-dependent contrast enhancement. */
+typedef struct {
+    const uint32_t* polarRemap;     /* [validSampleCount][2]: polar texel (column, ring) of each sample */
+    uint32_t samplesPerRing, ringCount; /* samplesPerRing * ringCount == validSampleCount */
+    float eccentricityMap[6];       /* m, w0, S, 1/log(1+m), 1/w0, 1/m (hvvr_host_foveated_params) */
+    float maxEccentricity;          /* radians */
+    float taaAlpha;                 /* TemporalFilterSettings::alpha (gpu_camera.h:41-45), 0.05 */
+    float taaStddevMultiplier;      /* TemporalFilterSettings::stddevMultiplier, 1 */
+    int contrastEnhancement;        /* ContrastEnhancementSettings::enable (gpu_camera.cu:180-181), 1 */
+    float contrastFe;               /* ContrastEnhancementSettings::f_e, 1 */
+} hvvr_gpu_foveated_desc;
+int hvvr_gpu_view_set_foveated(hvvr_gpu_view* view, const hvvr_gpu_foveated_desc* desc);
+/* Per-frame transforms, as hvvr_host_foveated_frame computes them. */
+typedef struct {
+    float eyeToWorld[16];          /* eye (beam) space to world; the frame's camera-to-world */
+    float cullPlanes[16];          /* 4 world-space planes (normal xyz, dist); tiles wholly on the
+                                      positive side of one are skipped; dist[0] = +inf: no culling */
+    float sampleToEye[9];          /* screen sample space to eye space (column-major 3x3) */
+    float eyeToSamplePrevious[16]; /* eye space to the previous frame's sample space */
+} hvvr_gpu_foveated_frame;
+/* One foveated frame (asynchronous). The reconstruction is read with
+ * hvvr_gpu_view_read_image. Stage [6] of the stage times covers the polar remap and the
+ * reconstruction. */
+int hvvr_gpu_view_render_foveated(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const hvvr_gpu_foveated_frame* frame,
+                                  const float* jitter);
+enum {
+    HVVR_GPU_FOVEATED_POLAR_COLOR = 0, /* uint32 RGBA8 sRGB [ringCount][samplesPerRing] */
+    HVVR_GPU_FOVEATED_POLAR_DEPTH = 1, /* float [ringCount][samplesPerRing] t-max (inf = miss) */
+    HVVR_GPU_FOVEATED_MOMENT1 = 2,     /* uint64 RGBA16 unorm [ringCount][samplesPerRing] */
+    HVVR_GPU_FOVEATED_MOMENT2 = 3,
+    HVVR_GPU_FOVEATED_FILTERED = 4,    /* uint32 [height][width]: this frame's temporal filter output */
+    HVVR_GPU_FOVEATED_PREVIOUS = 5,    /* uint32 [height][width]: the previous frame's */
+    HVVR_GPU_FOVEATED_BLUR1 = 6,       /* uint32 [height][width]: contrast blur, first pass (along y) */
+    HVVR_GPU_FOVEATED_BLUR2 = 7,       /* uint32 [height][width]: second pass (along x) */
+    HVVR_GPU_FOVEATED_TMAX = 8         /* float [sampleCount] per-sample t-max */
+};
+/* Synchronous read of a foveated intermediate of the last frame; bytes must match. */
+int hvvr_gpu_view_read_foveated(hvvr_gpu_view* view, int buffer, void* dst, uint64_t bytes);
+
 /* Synchronous reads of the last rendered frame. */
 int hvvr_gpu_view_read_image(hvvr_gpu_view* view, uint32_t* image, uint32_t strideInPixels);
 int hvvr_gpu_view_read_sample_colors(hvvr_gpu_view* view, uint32_t* colors); /* [sampleCount] RGBA8 */
@@ -146,7 +192,8 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled
  * (waits for that frame's last event):
  * [0] frustum setup, [1] block traversal, [2] tile lists, [3] tile lists of tiles that
- * did not fit in LDS, [4] intersect, [5] resolve, [6] remap. Returns the count. */
+ * did not fit in LDS, [4] intersect, [5] resolve, [6] remap (foveated: polar remap and
+ * reconstruction). Returns the count. */
 int hvvr_gpu_view_stage_times(hvvr_gpu_view* view, float* ms, int maxCount);
 
 #ifdef __cplusplus

@@ -73,6 +73,35 @@ int hvvr_host_view_copy(const hvvr_host_view* view, int array, void* dst, uint64
 /* camera-to-world matrix (column-major float[16]) and lens (radius, focal distance) of a preset */
 int hvvr_host_view_camera(const hvvr_host_view* view, float cameraToWorld[16], float lens[2]);
 
+/* Sample space of a view: sampleToCamera (column-major 3x3) and the bounds of the sample
+ * positions (lower x, y, upper x, y), camera.cpp:21-42. Either output may be NULL. */
+int hvvr_host_view_sample_space(const hvvr_host_view* view, float sampleToCamera[9], float boundsLowerUpper[4]);
+
+/* ---- polar foveated sampling (raycaster/foveated.cpp, samples.cpp:44-142, frusta.cu:26-121) ---- */
+typedef struct hvvr_host_foveated hvvr_host_foveated;
+enum {
+    HVVR_FOVEATED_BEAMS = 0,        /* DirectionalBeam[sampleCount], eye space, clustered */
+    HVVR_FOVEATED_TILE_FRUSTA = 1,  /* SimpleRayFrustum[blockCount*64], eye space */
+    HVVR_FOVEATED_BLOCK_FRUSTA = 2, /* SimpleRayFrustum[blockCount], eye space */
+    HVVR_FOVEATED_POLAR_REMAP = 3   /* uint32[validSampleCount][2]: polar texel (column, ring) */
+};
+/* pattern: degreeTrackingError, maxFOVDegrees, marSlope, fovealMARDegrees
+ * (RayCasterSpecification::FoveatedSamplePattern, raycaster_spec.h:29-34) */
+int hvvr_host_foveated_create(const float pattern[4], hvvr_host_foveated** out);
+void hvvr_host_foveated_destroy(hvvr_host_foveated* f);
+/* counts[0..5] = validSampleCount, sampleCount, blockCount, samplesPerRing, ringCount, tileCount */
+int hvvr_host_foveated_counts(const hvvr_host_foveated* f, uint64_t counts[6]);
+int hvvr_host_foveated_copy(const hvvr_host_foveated* f, int array, void* dst, uint64_t bytes);
+/* eccentricity map (m, w0, S, 1/log(1+m), 1/w0, 1/m; radians) and the maximum eccentricity */
+int hvvr_host_foveated_params(const hvvr_host_foveated* f, float emap[6], float* maxEccentricity);
+/* One frame's transforms (render.cpp:454-488): from the camera-to-world matrix, the gaze
+ * direction in camera space and the camera's sample space, out[57] = eyeToWorld[16],
+ * cullPlanes[4][4] (world-space normal, dist), sampleToEye[9], eyeToSamplePrevious[16].
+ * history[25] carries worldToEyePrevious[16] (zero at first) and
+ * eyePreviousToSamplePrevious[9] (identity at first) from frame to frame. */
+int hvvr_host_foveated_frame(const float cameraToWorld[16], const float eyeDir[3], const float sampleToCamera[9],
+                             const float boundsLowerUpper[4], float history[25], float out[57]);
+
 /* Grid samples (Sample[width*height]) with optional barrel distortion. */
 int hvvr_host_grid_samples(uint32_t width, uint32_t height, float k1, float k2, void* dst);
 /* Spiral subsample offsets: out[2*i], out[2*i+1] for i < msaaRate. */

@@ -17,6 +17,8 @@
  *   resolve / shade    raycaster/resolve.cu:161-284,324-384, raycaster/shading.h:19-161,
  *                      raycaster/shading_helpers.h:89-200, shared/util_graphics.h:26-28
  *   clear / remap      raycaster/resolve.cu:527-542, raycaster/remap.cu:35-77
+ *   foveated           raycaster/gpu_foveated.cu:37-333, remap.cu:127-141,
+ *                      raycaster/foveated.h:86-111, resolve.cu:386-457 (t-max)
  *   refit              raycaster/gpu_scene_state.cu:88-375
  *
  * Arithmetic contract (shared with the GPU kernels, so results compare bit for bit):
@@ -69,6 +71,11 @@ void or_layout_sizes(uint32_t out[10]);
 /* in: n camera-space frusta; m: camera-to-world (column-major 4x4).
  * out: world-space frusta and the pre-swizzled test data. */
 void or_frustum_setup(const o_sfrustum* in, uint32_t n, const float m[16], o_sfrustum* worldOut, o_frustum* preOut);
+/* with screen cull planes (planes[16] = 4 x (normal, dist) in world space, or NULL; dist[0]
+ * = +inf disables them): a frustum wholly outside one becomes degenerate
+ * (render.cpp:271-340, the foveated path) */
+void or_frustum_setup_cull(const o_sfrustum* in, uint32_t n, const float m[16], const float* planes,
+                           o_sfrustum* worldOut, o_frustum* preOut);
 /* AABB tests on one node: coarse mask, then refine of a previous mask */
 uint32_t or_test_children(const o_frustum* f, const o_node* node);
 uint32_t or_test_children_refine(const o_frustum* f, const o_node* node, uint32_t prev);
@@ -138,6 +145,42 @@ int or_resolve_view_dof(const o_beam* beams, uint32_t sampleCount, const float m
 /* image[y*stride + x] = results[remap[...]] (split 1, 2 or 3) */
 void or_remap(const uint32_t* results, const int32_t* remap, uint32_t width, uint32_t height, uint32_t split,
               uint32_t stride, uint32_t* image);
+
+/* ---- foveated path (raycaster/gpu_foveated.cu, remap.cu:127-158, resolve.cu:386-457) ---- */
+/* Per sample: distance along the view axis (-m column 2) of subsample 0's hit point from its
+ * barycentrics (calcUVW at subsample 0), +inf on a miss or in a tile with an empty list. */
+int or_tmax_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                 const uint32_t* triIds, const o_tri* tris, const o_dof* dof, const uint32_t* counts, float* tMaxOut);
+/* sRGB byte -> linear (util_graphics.h:16-18), and byte -> linear -> byte (truncated) */
+void or_srgb_tables(float toLinear[256], uint8_t roundTrip[256]);
+/* RemapPolarFoveatedKernel: polar[remap[i]] = sRGB round trip of results[i], depth = tMax[i] */
+void or_polar_remap(const uint32_t* results, const float* tMax, const uint32_t* remap, uint32_t count,
+                    uint32_t width, uint32_t* polarColor, float* polarDepth);
+/* ComputeMoments: 3x3 mean / mean square (RGBA16 unorm, 4 x 16 bits) per polar texel */
+void or_polar_moments(const uint32_t* polarColor, uint32_t w, uint32_t h, uint64_t* m1, uint64_t* m2);
+typedef struct {
+    const uint32_t* polarColor;
+    const float* polarDepth;
+    const uint64_t *moment1, *moment2;
+    const uint32_t* previous; /* [height][width] */
+    uint32_t polarW, polarH, width, height;
+    float sampleToEye[9];
+    float eyeToSamplePrevious[16];
+    float emap[6];            /* m, w0, S, 1/log(1+m), 1/w0, 1/m */
+    float maxEccentricity;
+    float alpha, stddevMultiplier;
+} o_reproject;
+/* FoveatedPolarToScreenSpaceKernel: the temporally filtered screen image */
+void or_foveated_reproject(const o_reproject* a, uint32_t* out);
+/* SeparableFilterUsingEccentricity with step (stepX, stepY) */
+void or_foveated_blur(const uint32_t* in, uint32_t* out, uint32_t w, uint32_t h, int stepX, int stepY,
+                      const float sampleToEye[9]);
+/* FinishConstrastEnhancement */
+void or_foveated_contrast(const uint32_t* unfiltered, const uint32_t* filtered, uint32_t* out, uint32_t w, uint32_t h,
+                          const float sampleToEye[9], float maxEccentricity, float fe);
+/* EccentricityMap::apply / applyInverse (raycaster/foveated.h:86-111) */
+float or_emap_apply(const float emap[6], float i);
+float or_emap_apply_inverse(const float emap[6], float e);
 
 /* ---- refit ---- */
 /* World-space vertices, precomputed triangles and node bounds from scratch. nodes:

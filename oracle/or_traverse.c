@@ -98,7 +98,24 @@ static void frustum_pre(const ov3 o[4], const ov3 d[4], o_frustum* f) {
     }
 }
 
+/* planeCullsFrustum (render.cpp:271-279): all 4 corner rays, at the origin and 10^4 along
+ * the ray, on the positive side of the plane (normal p[0..2], dist p[3]) */
+static int plane_culls(const float* p, const ov3 o[4], const ov3 d[4]) {
+    ov3 n = v3(p[0], p[1], p[2]);
+    int allOut = 1;
+    for (int v = 0; v < 4; v++) {
+        allOut = allOut && dot3(n, o[v]) > p[3];
+        allOut = allOut && dot3(n, add3(o[v], scale3(d[v], 10000.0f))) > p[3];
+    }
+    return allOut;
+}
+
 void or_frustum_setup(const o_sfrustum* in, uint32_t n, const float m[16], o_sfrustum* worldOut, o_frustum* preOut) {
+    or_frustum_setup_cull(in, n, m, NULL, worldOut, preOut);
+}
+
+void or_frustum_setup_cull(const o_sfrustum* in, uint32_t n, const float m[16], const float* planes,
+                           o_sfrustum* worldOut, o_frustum* preOut) {
     for (uint32_t i = 0; i < n; i++) {
         ov3 o[4], d[4];
         /* an infinite first origin marks a culled (degenerate) frustum (render.cpp:331-338) */
@@ -108,6 +125,16 @@ void or_frustum_setup(const o_sfrustum* in, uint32_t n, const float m[16], o_sfr
             m4mul(m, in[i].o[v].x, in[i].o[v].y, in[i].o[v].z, 1.0f, t);
             o[v] = culled ? v3(INFINITY, INFINITY, INFINITY) : v3(t[0], t[1], t[2]);
             d[v] = culled ? v3(0.0f, 0.0f, 0.0f) : m3mul(m, in[i].d[v]);
+        }
+        /* blockTransformAndCullTask (render.cpp:311-340) */
+        if (planes && !isinf(planes[3]) && !culled) {
+            for (int k = 0; k < 4; k++)
+                culled = culled || plane_culls(planes + 4 * k, o, d);
+            if (culled)
+                for (int v = 0; v < 4; v++) {
+                    o[v] = v3(INFINITY, INFINITY, INFINITY);
+                    d[v] = v3(0.0f, 0.0f, 0.0f);
+                }
         }
         if (worldOut) {
             memcpy(worldOut[i].o, o, sizeof(o));

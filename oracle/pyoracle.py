@@ -52,13 +52,16 @@ def layout_sizes():
     return out
 
 
-def frustum_setup(frusta, m):
+def frustum_setup(frusta, m, cull_planes=None):
+    """World-space frusta + test data; cull_planes (16 floats, 4 x (normal, dist)) culls
+    frusta wholly outside one plane (the foveated path's tiles)."""
     from hvvr_amd import DTYPES
     frusta = np.ascontiguousarray(frusta)
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     world = np.zeros(len(frusta), DTYPES["sfrustum"])
     pre = np.zeros(len(frusta), O_FRUSTUM)
-    lib().or_frustum_setup(_p(frusta), c_uint32(len(frusta)), _p(m), _p(world), _p(pre))
+    planes = None if cull_planes is None else np.ascontiguousarray(cull_planes, np.float32).reshape(16)
+    lib().or_frustum_setup_cull(_p(frusta), c_uint32(len(frusta)), _p(m), _p(planes), _p(world), _p(pre))
     return world, pre
 
 
@@ -213,6 +216,29 @@ def refit(scene):
     return wv, tris, nodes
 
 
+def render_beams(scene, beams, tile_frusta, block_frusta, A, m, lens=(0.0, 1.0), cull_planes=None, precision=0):
+    """Oracle frame of a beam set (no remap): refit, frustum setup (tile frusta outside a
+    cull plane skipped), traversal, intersect, resolve with empty tiles cleared, and the
+    per-sample t-max of the foveated path."""
+    m = np.asarray(m, np.float32).reshape(16)
+    wv, tris, nodes = refit(scene)
+    world_tiles, tile_pre = frustum_setup(tile_frusta, m, cull_planes)
+    _, block_pre = frustum_setup(block_frusta, m)
+    counts, lists = build_tile_lists(nodes, block_pre, tile_pre)
+    from hvvr_amd import lens_table, subsample_offsets
+    offsets = subsample_offsets(A)
+    radius, focal = float(lens[0]), float(lens[1])
+    dof = (radius, focal, lens_table(A)) if radius > 0 else None
+    tri_ids, depth, surv = intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris, dof)
+    env, _ = lighting_env(scene.lights)
+    colors, linear = resolve_view(beams, m, offsets, A, tri_ids, tris, scene.triShade, wv, scene.materials, env,
+                                  precision, dof, counts)
+    colors[np.repeat(counts == 0, 128)[: len(colors)]] = 0xFF000000
+    tmax = tmax_view(beams, m, offsets, A, tri_ids, tris, counts, dof)
+    return dict(nodes=nodes, tris=tris, counts=counts, lists=lists, tri_ids=tri_ids, depth=depth, survivors=surv,
+                colors=colors, linear=linear, tmax=tmax, offsets=offsets)
+
+
 def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=True):
     """Full oracle frame: refit, frustum setup, traversal, intersect, resolve, remap."""
     m = view.camera_to_world if camera_to_world is None else np.asarray(camera_to_world, np.float32)
@@ -238,3 +264,100 @@ def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=T
         out["linear"] = linear
         out["image"] = remap(colors, view.remap, view.width, view.height, view.split)
     return out
+
+
+# ---------------------------------------------------------------------------------------
+# foveated path
+# ---------------------------------------------------------------------------------------
+def tmax_view(beams, m, offsets, A, tri_ids, tris, counts, dof=None):
+    beams = np.ascontiguousarray(beams)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.zeros(len(beams), np.float32)
+    d, keep = _dof(dof)
+    rc = lib().or_tmax_view(_p(beams), c_uint32(len(beams)), _p(m), _p(np.ascontiguousarray(offsets, np.float32)),
+                            c_uint32(A), _p(np.ascontiguousarray(tri_ids, np.uint32)), _p(tris),
+                            ctypes.byref(d) if d else None, _p(np.ascontiguousarray(counts, np.uint32)), _p(out))
+    assert rc == 0
+    return out
+
+
+def srgb_tables():
+    lin = np.zeros(256, np.float32)
+    rt = np.zeros(256, np.uint8)
+    lib().or_srgb_tables(_p(lin), _p(rt))
+    return lin, rt
+
+
+def polar_remap(results, tmax, polar_remap_xy, spr, rings):
+    color = np.zeros(spr * rings, np.uint32)
+    depth = np.zeros(spr * rings, np.float32)
+    xy = np.ascontiguousarray(polar_remap_xy, np.uint32).reshape(-1)
+    lib().or_polar_remap(_p(np.ascontiguousarray(results, np.uint32)), _p(np.ascontiguousarray(tmax, np.float32)),
+                         _p(xy), c_uint32(len(xy) // 2), c_uint32(spr), _p(color), _p(depth))
+    return color.reshape(rings, spr), depth.reshape(rings, spr)
+
+
+def polar_moments(polar_color):
+    pc = np.ascontiguousarray(polar_color, np.uint32)
+    h, w = pc.shape
+    m1 = np.zeros((h, w), np.uint64)
+    m2 = np.zeros((h, w), np.uint64)
+    lib().or_polar_moments(_p(pc), c_uint32(w), c_uint32(h), _p(m1), _p(m2))
+    return m1, m2
+
+
+class _Reproject(ctypes.Structure):
+    _fields_ = [("polarColor", c_void_p), ("polarDepth", c_void_p), ("moment1", c_void_p), ("moment2", c_void_p),
+                ("previous", c_void_p), ("polarW", c_uint32), ("polarH", c_uint32), ("width", c_uint32),
+                ("height", c_uint32), ("sampleToEye", ctypes.c_float * 9), ("eyeToSamplePrevious", ctypes.c_float * 16),
+                ("emap", ctypes.c_float * 6), ("maxEccentricity", ctypes.c_float), ("alpha", ctypes.c_float),
+                ("stddevMultiplier", ctypes.c_float)]
+
+
+def foveated_reproject(polar_color, polar_depth, m1, m2, previous, width, height, sample_to_eye,
+                       eye_to_sample_previous, emap, max_ecc, alpha=0.05, stddev=1.0):
+    arrays = [np.ascontiguousarray(polar_color, np.uint32), np.ascontiguousarray(polar_depth, np.float32),
+              np.ascontiguousarray(m1, np.uint64), np.ascontiguousarray(m2, np.uint64),
+              np.ascontiguousarray(previous, np.uint32)]
+    h, w = arrays[0].shape
+    a = _Reproject(_p(arrays[0]), _p(arrays[1]), _p(arrays[2]), _p(arrays[3]), _p(arrays[4]), w, h, width, height,
+                   (ctypes.c_float * 9)(*np.asarray(sample_to_eye, np.float32).reshape(9)),
+                   (ctypes.c_float * 16)(*np.asarray(eye_to_sample_previous, np.float32).reshape(16)),
+                   (ctypes.c_float * 6)(*np.asarray(emap, np.float32).reshape(6)), max_ecc, alpha, stddev)
+    out = np.zeros((height, width), np.uint32)
+    lib().or_foveated_reproject(ctypes.byref(a), _p(out))
+    return out
+
+
+def foveated_blur(img, step, sample_to_eye):
+    img = np.ascontiguousarray(img, np.uint32)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    s2e = np.ascontiguousarray(sample_to_eye, np.float32).reshape(9)
+    lib().or_foveated_blur(_p(img), _p(out), c_uint32(w), c_uint32(h), c_int(step[0]), c_int(step[1]), _p(s2e))
+    return out
+
+
+def foveated_contrast(unfiltered, filtered, sample_to_eye, max_ecc, fe=1.0):
+    u = np.ascontiguousarray(unfiltered, np.uint32)
+    f = np.ascontiguousarray(filtered, np.uint32)
+    h, w = u.shape
+    out = np.zeros_like(u)
+    s2e = np.ascontiguousarray(sample_to_eye, np.float32).reshape(9)
+    lib().or_foveated_contrast(_p(u), _p(f), _p(out), c_uint32(w), c_uint32(h), _p(s2e), ctypes.c_float(max_ecc),
+                               ctypes.c_float(fe))
+    return out
+
+
+def emap_apply(emap, i):
+    L = lib()
+    L.or_emap_apply.restype = ctypes.c_float
+    e = (ctypes.c_float * 6)(*np.asarray(emap, np.float32))
+    return L.or_emap_apply(e, ctypes.c_float(i))
+
+
+def emap_apply_inverse(emap, e_rad):
+    L = lib()
+    L.or_emap_apply_inverse.restype = ctypes.c_float
+    e = (ctypes.c_float * 6)(*np.asarray(emap, np.float32))
+    return L.or_emap_apply_inverse(e, ctypes.c_float(e_rad))

@@ -1,0 +1,295 @@
+"""Polar foveated path: eye-space polar samples (raycaster/samples.cpp:44-142), their
+clustering and frusta (foveated.cpp:19-84, frusta.cu:26-121), the per-frame eye transforms
+and screen cull planes (render.cpp:397-416, 454-488), the t-max buffer (resolve.cu:386-457)
+and the reconstruction kernels (gpu_foveated.cu, remap.cu:127-141).
+
+CPU pins: the sample counts the reference's formulas give (878 x 359 = 315,202 for the
+feb2017 pattern, 278 x 259 = 72,002 for the default); ring eccentricities equal to the
+eccentricity map's closed form, which satisfies its recurrence; the beam differentials'
+geometry; the polar remap as a bijection that puts each beam at its own ring and azimuth;
+every sample ray inside its tile and block frustum; the reprojection chain returning a
+point to its own sample coordinates; the oracle's reconstruction stages on images whose
+answer is known (constant images, zero contrast gain).
+
+GPU: samples, tile lists and GBuffer bit-exact against the oracle on the same eye-space
+beams, colours within 1/255, t-max within 1e-5 relative; each reconstruction stage against
+the oracle run on the GPU's own input to that stage, within 1/255 per channel."""
+
+import numpy as np
+import pytest
+
+from conftest import rgba_delta
+
+
+@pytest.fixture(scope="module")
+def feb(H):
+    return H.HostFoveated(H.HostFoveated.FEB2017)
+
+
+@pytest.fixture(scope="module")
+def screen(H):
+    # the reconstruction target: one eye of the stereo pair (its viewport and sample bounds)
+    return H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 1080, 1200)
+
+
+def test_polar_sample_counts(H):
+    f = H.HostFoveated(H.HostFoveated.FEB2017)
+    assert (f.samples_per_ring, f.ring_count, f.valid_sample_count) == (878, 359, 315202)
+    assert f.block_count == 39 and f.sample_count == 39 * 8192 and f.tile_count == 39 * 64
+    d = H.HostFoveated(H.HostFoveated.DEFAULT)
+    assert (d.samples_per_ring, d.ring_count, d.valid_sample_count) == (278, 259, 72002)
+
+
+def test_eccentricity_map_closed_form(O, feb):
+    e = feb.emap
+    m, w0, S = float(e[0]), float(e[1]), float(e[2])
+    assert abs(m - 0.015) < 1e-9 and abs(w0 - np.radians(1 / 20)) < 1e-9 and abs(S - np.radians(7.0)) < 1e-8
+    # E(n) = w0 n inside the tracking-error radius, E(n+1) = E(n) + w0 + m (E(n) - S) beyond it
+    E = np.array([O.emap_apply(e, float(n)) for n in range(360)], np.float64)
+    n_in = int(S / w0)
+    assert np.allclose(E[: n_in + 1], w0 * np.arange(n_in + 1), rtol=1e-6)
+    nxt = E[n_in + 1: -1] + w0 + m * (E[n_in + 1: -1] - S)
+    assert np.allclose(E[n_in + 2:], nxt, rtol=2e-5)
+    # the inverse maps back
+    for n in (0.0, 3.5, 100.0, 140.0, 250.0, 358.0):
+        assert abs(O.emap_apply_inverse(e, O.emap_apply(e, n)) - n) < 2e-3 * max(1.0, n)
+
+
+def _ring_and_column(f, beams):
+    c = beams["centerRay"].astype(np.float64)
+    ecc = np.arccos(np.clip(-c[:, 2], -1, 1))
+    # ring base vector (-sin E, 0, -cos E) turned about -z by the azimuth
+    az = np.mod(np.arctan2(c[:, 1], -c[:, 0]), 2 * np.pi)
+    return ecc, az
+
+
+def test_polar_beams_geometry(O, feb):
+    f = feb
+    b = f.beams[: f.valid_sample_count]
+    c = b["centerRay"].astype(np.float64)
+    du = b["du"].astype(np.float64)
+    dv = b["dv"].astype(np.float64)
+    assert np.allclose(np.linalg.norm(c, axis=1), 1.0, atol=1e-6)
+    # du reaches the next ring on the tangent plane; dv runs along the ring: both tangent
+    assert np.abs((du * c).sum(1)).max() < 1e-6
+    assert np.abs((dv * c).sum(1)).max() < 1e-6
+    assert np.abs((du * dv).sum(1)).max() < 1e-6
+    ring = f.polar_remap[:, 1].astype(np.int64)
+    col = f.polar_remap[:, 0].astype(np.int64)
+    ecc, az = _ring_and_column(f, b)
+    E = np.array([O.emap_apply(f.emap, float(r)) for r in range(f.ring_count + 1)], np.float64)
+    assert np.abs(ecc - E[ring]).max() < 2e-3 * np.maximum(E[ring], 1e-2).max()
+    # azimuth (i + 0.5) / samplesPerRing of a turn, on rings that have one
+    away = ring > 0
+    want = (col[away] + 0.5) / f.samples_per_ring * 2 * np.pi
+    err = np.angle(np.exp(1j * (az[away] - want)))
+    assert np.abs(err).max() < 1e-3
+    # |dv| = 2 pi sin(E) / samplesPerRing
+    assert np.allclose(np.linalg.norm(dv, axis=1), 2 * np.pi * np.sin(E[ring]) / f.samples_per_ring,
+                       rtol=1e-3, atol=1e-7)
+
+
+def test_polar_remap_is_a_bijection(feb):
+    f = feb
+    flat = f.polar_remap[:, 1].astype(np.int64) * f.samples_per_ring + f.polar_remap[:, 0]
+    assert np.array_equal(np.sort(flat), np.arange(f.samples_per_ring * f.ring_count))
+    # padding repeats the last sample
+    pad = f.beams[f.valid_sample_count:]
+    assert np.all(pad == f.beams[f.valid_sample_count - 1])
+
+
+def _inside(frustum, dirs, eps=1e-6):
+    d = frustum["directions"].astype(np.float64)
+    centre = d.sum(0)
+    ok = np.ones(len(dirs), bool)
+    for k in range(4):
+        n = np.cross(d[k], d[(k + 1) % 4])
+        sign = np.sign(n @ centre)
+        ok &= sign * (dirs @ n) >= -eps * np.linalg.norm(dirs, axis=1)
+    return ok
+
+
+def test_every_sample_inside_its_frusta(feb):
+    f = feb
+    c = f.beams["centerRay"].astype(np.float64)
+    n = f.valid_sample_count
+    bad = 0
+    for t in range(f.tile_count):
+        lo, hi = t * 128, min((t + 1) * 128, n)
+        if lo >= n:
+            assert np.isinf(f.tile_frusta[t]["origins"][0][0])  # no samples: degenerate
+            continue
+        bad += (~_inside(f.tile_frusta[t], c[lo:hi])).sum()
+    assert bad == 0
+    for b in range(f.block_count):
+        lo, hi = b * 8192, min((b + 1) * 8192, n)
+        assert _inside(f.block_frusta[b], c[lo:hi]).all()
+
+
+def test_frame_transforms(H, screen):
+    hist = H.foveated_history()
+    m = screen.camera_to_world
+    fr = H.foveated_frame(m, (0.0, 0.0, -1.0), screen.sample_to_camera, screen.sample_bounds, hist)
+    # gaze straight ahead: eye space is camera space
+    assert np.allclose(fr["eye_to_world"], m, atol=1e-6)
+    # first frame: no history, everything reprojects to (0, 0, 0, 0)
+    assert np.all(fr["eye_to_sample_previous"] == 0)
+    gaze = np.array([np.sin(0.2), 0.1, -np.cos(0.2)], np.float32)
+    gaze /= np.linalg.norm(gaze)
+    fr2 = H.foveated_frame(m, gaze, screen.sample_to_camera, screen.sample_bounds, hist)
+    s2e = fr2["sample_to_eye"].reshape(3, 3).T
+    M = fr2["eye_to_sample_previous"].reshape(4, 4).T
+    e2w = fr2["eye_to_world"].reshape(4, 4).T
+    # the gaze direction is eye-space -z
+    assert np.allclose(e2w[:3, :3] @ np.array([0, 0, -1.0]), m.reshape(4, 4).T[:3, :3] @ gaze, atol=1e-5)
+    # a point on sample (u, v)'s ray reprojects to (u, v) of the previous (straight-ahead) frame
+    for u, v in ((0.3, 0.6), (0.5, 0.5), (0.9, 0.1)):
+        # sample space of the previous frame = camera space of this one (same camera)
+        p_cam = screen.sample_to_camera.reshape(3, 3).T @ np.array([u, v, 1.0]) * 4.0
+        p_eye = np.linalg.inv(e2w[:3, :3]) @ (m.reshape(4, 4).T[:3, :3] @ p_cam)
+        q = M @ np.append(p_eye, 1.0)
+        assert np.allclose(q[:2] / q[2], (u, v), atol=1e-4)
+    # cull planes: points inside the screen rectangle are on the negative side of all four
+    planes = fr2["cull_planes"].reshape(4, 4)
+    cam_pos = m[12:15]
+    lo, hi = screen.sample_bounds[:2], screen.sample_bounds[2:]
+    inside = screen.sample_to_camera.reshape(3, 3).T @ np.array([(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, 1.0])
+    p = cam_pos + m.reshape(4, 4).T[:3, :3] @ inside * 10.0
+    assert np.all(planes[:, :3] @ p <= planes[:, 3])
+    outside = screen.sample_to_camera.reshape(3, 3).T @ np.array([hi[0] + 1.0, (lo[1] + hi[1]) / 2, 1.0])
+    p = cam_pos + m.reshape(4, 4).T[:3, :3] @ outside * 10.0
+    assert np.any(planes[:, :3] @ p > planes[:, 3])
+
+
+def _const(h, w, rgba):
+    return np.full((h, w), rgba, np.uint32)
+
+
+def test_oracle_reconstruction_known_answers(O, screen):
+    lin, rt = O.srgb_tables()
+    k = 150
+    c = np.uint32(0xFF000000 | (k << 16) | (k << 8) | k)
+    polar = _const(40, 64, c)
+    m1, m2 = O.polar_moments(polar)
+    # constant image: mean = value, mean square = value^2 (16-bit, truncated)
+    v = lin[k]
+    assert np.all((m1 & 0xFFFF) == np.uint64(np.float32(v) * np.float32(65535.0)))
+    assert abs(int(m2[0, 0] & 0xFFFF) - int(v * v * 65535)) <= 1
+    depth = np.full((40, 64), np.inf, np.float32)
+    s2e = screen.sample_to_camera
+    emap = np.array([0.015, np.radians(1 / 20), np.radians(7.0), 1 / np.log1p(0.015), 1 / np.radians(1 / 20),
+                     1 / 0.015], np.float32)
+    out = O.foveated_reproject(polar, depth, m1, m2, _const(24, 32, 0), 32, 24, s2e, np.zeros(16), emap,
+                               np.radians(90.0))
+    # constant input, no depth (no history): the bicubic lookup returns the constant (the
+    # sRGB round trip may truncate one step)
+    for sh in (0, 8, 16):
+        ch = (out >> sh) & 0xFF
+        assert np.all(np.abs(ch.astype(int) - k) <= 1)
+    # blur of a constant: the constant (within the round trip); contrast with no difference
+    # from the mean, or with zero gain, is the identity
+    img = _const(24, 32, c)
+    b = O.foveated_blur(img, (1, 0), s2e)
+    assert np.all(np.abs(((b >> 8) & 0xFF).astype(int) - k) <= 1)
+    ramp = (np.arange(24 * 32, dtype=np.uint32).reshape(24, 32) % 200 + 20)
+    img2 = (0xFF000000 | (ramp << 16) | (ramp << 8) | ramp).astype(np.uint32)
+    same = O.foveated_contrast(img2, img2, s2e, np.radians(90.0))
+    assert rgba_delta(same, img2).max() <= 1
+    nogain = O.foveated_contrast(img2, img, s2e, np.radians(90.0), fe=0.0)
+    assert rgba_delta(nogain, img2).max() <= 1
+
+
+# ---------------------------------------------------------------------------------------
+# GPU parity
+# ---------------------------------------------------------------------------------------
+def _gaze(yaw_deg, pitch_deg):
+    y, p = np.radians(yaw_deg), np.radians(pitch_deg)
+    g = np.array([np.sin(y) * np.cos(p), np.sin(p), -np.cos(y) * np.cos(p)], np.float32)
+    return g / np.linalg.norm(g)
+
+
+def _check_samples(O, gv, ref, A):
+    counts, lists = gv.read_tile_lists()
+    assert np.array_equal(counts, ref["counts"]) and np.array_equal(lists, ref["lists"])
+    tri, mask = gv.read_gbuffer()
+    rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+    bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
+    assert len(bad) == 0, f"{len(bad)} samples differ, first {bad[:5]}"
+    assert gv.stats().survivorPairs == ref["survivors"]
+    assert rgba_delta(gv.read_sample_colors(), ref["colors"]).max() <= 1
+    t = gv.read_foveated("tmax")
+    rt = ref["tmax"]
+    fin = np.isfinite(rt)
+    assert np.array_equal(np.isfinite(t), fin)
+    assert np.allclose(t[fin], rt[fin], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lens", [(0.0, 1.0), (0.01, 4.0)])
+def test_foveated_samples_match_oracle(H, O, ctx, scene2, feb, screen, lens):
+    A = 4
+    gv = H.GpuFoveatedView(ctx, feb, 270, 300, A, lens_radius=lens[0], focal_distance=lens[1])
+    gs = H.GpuScene(ctx, scene2)
+    hist = H.foveated_history()
+    fr = H.foveated_frame(screen.camera_to_world, _gaze(12.0, -5.0), screen.sample_to_camera, screen.sample_bounds,
+                          hist)
+    gv.render_foveated(gs, fr)
+    ref = O.render_beams(scene2, feb.beams, feb.tile_frusta, feb.block_frusta, A, fr["eye_to_world"], lens,
+                         fr["cull_planes"])
+    assert (ref["counts"] == 0).sum() > 0  # some tiles culled or empty
+    _check_samples(O, gv, ref, A)
+
+
+def _stage_checks(O, gv, feb, fr, taa_prev):
+    colors = gv.read_sample_colors()
+    tmax = gv.read_foveated("tmax")
+    pc, pd = O.polar_remap(colors[: feb.valid_sample_count], tmax[: feb.valid_sample_count], feb.polar_remap,
+                           feb.samples_per_ring, feb.ring_count)
+    gpc, gpd = gv.read_foveated("polar_color"), gv.read_foveated("polar_depth")
+    assert np.array_equal(gpc, pc)
+    assert np.array_equal(gpd.view(np.uint32), pd.view(np.uint32))
+    m1, m2 = O.polar_moments(gpc)
+    g1, g2 = gv.read_foveated("moment1"), gv.read_foveated("moment2")
+    for a, b in ((g1, m1), (g2, m2)):
+        d = np.abs(np.stack([((a >> np.uint64(s)) & np.uint64(0xFFFF)).astype(np.int64) -
+                             ((b >> np.uint64(s)) & np.uint64(0xFFFF)).astype(np.int64) for s in (0, 16, 32, 48)]))
+        assert d.max() <= 1
+    # Every stage ends in a truncating 8-bit sRGB encode. In flat regions its input is a
+    # decoded byte re-encoded, an integer in exact arithmetic, so a 1-ulp difference
+    # between the GPU's and the host's powf / acosf / logf moves it across the integer and
+    # flips the last bit (likewise alpha = 255 x a sum of weights that is 1 in exact
+    # arithmetic). Bound: 1/255 per channel, on a minority of channels.
+    def close(got, want):
+        d = rgba_delta(got, want)
+        return d.max() <= 1 and (d > 0).mean() < 0.15
+    w, h = gv.view.width, gv.view.height
+    filt = O.foveated_reproject(gpc, gpd, g1, g2, taa_prev, w, h, fr["sample_to_eye"], fr["eye_to_sample_previous"],
+                                feb.emap, feb.max_eccentricity)
+    gf = gv.read_foveated("filtered")
+    assert close(gf, filt)
+    b1 = O.foveated_blur(gf, (0, 1), fr["sample_to_eye"])
+    gb1 = gv.read_foveated("blur1")
+    assert close(gb1, b1)
+    b2 = O.foveated_blur(gb1, (1, 0), fr["sample_to_eye"])
+    gb2 = gv.read_foveated("blur2")
+    assert close(gb2, b2)
+    img = O.foveated_contrast(gf, gb2, fr["sample_to_eye"], feb.max_eccentricity)
+    assert close(gv.read_image(), img)
+    return gf
+
+
+@pytest.mark.gpu
+def test_foveated_reconstruction_matches_oracle(H, O, ctx, scene2, feb, screen):
+    # two frames with the gaze moving: the second blends a reprojected history
+    A = 4
+    gv = H.GpuFoveatedView(ctx, feb, 540, 600, A)
+    gs = H.GpuScene(ctx, scene2)
+    hist = H.foveated_history()
+    prev = np.zeros((600, 540), np.uint32)
+    for k, g in enumerate([_gaze(0.0, 0.0), _gaze(6.0, 3.0)]):
+        fr = H.foveated_frame(screen.camera_to_world, g, screen.sample_to_camera, screen.sample_bounds, hist)
+        gv.render_foveated(gs, fr)
+        assert np.array_equal(gv.read_foveated("previous"), prev)
+        prev = _stage_checks(O, gv, feb, fr, prev)
+    # the second frame did use history: reprojected pixels differ from the no-history answer
+    assert np.any(gv.read_foveated("filtered") != gv.read_foveated("previous"))
