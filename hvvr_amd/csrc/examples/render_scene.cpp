@@ -3,7 +3,10 @@
 // modelviewer/modelviewer.cpp: create models, lights and a camera, set samples and a
 // render target, call render() every frame).
 //
-//   render_scene <scene.bin> <out.rgba> [frames]
+//   render_scene <scene.bin> <out.rgba> [frames] [foveated]
+//
+// "foveated" renders in GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT mode with the feb2017
+// sample pattern (raycaster_spec.h:38-47), gaze straight ahead.
 //
 // scene.bin (little endian, packed):
 //   char[4] "HVSC", u32 version = 1
@@ -59,7 +62,7 @@ struct Reader {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s scene.bin out.rgba [frames]\n", argv[0]);
+        fprintf(stderr, "usage: %s scene.bin out.rgba [frames] [foveated]\n", argv[0]);
         return 2;
     }
     int frames = argc > 3 ? atoi(argv[3]) : 1;
@@ -111,6 +114,13 @@ int main(int argc, char** argv) {
 
         RayCasterSpecification spec;
         spec.msaaRate = msaa;
+        if (argc > 4 && std::string(argv[4]) == "foveated") {
+            spec.mode = RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT;
+            spec.foveatedSamplePattern.degreeTrackingError = 7.0f;
+            spec.foveatedSamplePattern.fovealMARDegrees = 1.0f / 20.0f;
+            spec.foveatedSamplePattern.marSlope = 0.015f;
+            spec.foveatedSamplePattern.maxFOVDegrees = 90.0f;
+        }
         Raycaster raycaster(spec);
         std::vector<Model*> handles;
         for (ModelIn& m : models) {

@@ -397,6 +397,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->tileStackScratch.resize(size_t(TILE_STACK_SIZE) * v->tileCount));
     step(v->tileCounts.resize(v->tileCount));
     step(v->overflowTiles.resize(v->tileCount));
+    step(v->largeQueue.resize(v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));

@@ -130,7 +130,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
     hvvr::DeviceBuffer<hvvr::StackEntry> tileStackScratch; // [TILE_STACK_SIZE][tileCount], slot-major
-    hvvr::DeviceBuffer<uint32_t> overflowTiles;         // [tileCount] 1 = did not fit the LDS traversal
+    hvvr::DeviceBuffer<uint32_t> overflowTiles;         // [tileCount] 1 = left for the global-stack traversal
+    hvvr::DeviceBuffer<uint32_t> largeQueue;            // [tileCount] tiles for the large LDS configuration
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length
     hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount] list start in triIndices
     hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage, tiles in allocation order
@@ -141,7 +142,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::vector2> lensTable;        // depth of field: [4][A][128] lens positions
     hvvr::DeviceBuffer<uint32_t> occRank;               // depth of field: rank among occupied tiles
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
-                                                        // [3] list entries allocated, [4] node visits
+                                                        // [3] list entries allocated, [4] node visits,
+                                                        // [5] tiles queued for the large configuration
     uint64_t listCapacity = 0;
     uint32_t waveStackLimit = 64; // register stack entries per tile traversal (<= 64)
     // foveated path (hvvr_gpu_view_set_foveated): cull planes and the t-max buffer apply to

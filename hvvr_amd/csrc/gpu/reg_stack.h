@@ -63,4 +63,80 @@ __device__ inline void stackInsert(RegStack& st, float key, uint32_t meta, uint3
     st.size++;
 }
 
+// ---------------------------------------------------------------------------------
+// Four-register variant (capacity 256) for tiles with deep stacks: position q lives in
+// lane q % 64 of register pair (t, m)[q / 64], bottom-up as above.
+// ---------------------------------------------------------------------------------
+enum : uint32_t { REG_STACK4_CAP = 256 };
+
+struct RegStack4 {
+    float t0, t1, t2, t3;
+    uint32_t m0, m1, m2, m3;
+    uint32_t size;
+};
+
+// key and meta at stack position q (q < size)
+__device__ inline void stackRead(const RegStack4& st, uint32_t q, float& key, uint32_t& meta) {
+    const uint32_t ln = q & 63u;
+    switch (q >> 6) {
+        case 0: key = bitsf(readLane(fbits(st.t0), ln)); meta = readLane(st.m0, ln); break;
+        case 1: key = bitsf(readLane(fbits(st.t1), ln)); meta = readLane(st.m1, ln); break;
+        case 2: key = bitsf(readLane(fbits(st.t2), ln)); meta = readLane(st.m2, ln); break;
+        default: key = bitsf(readLane(fbits(st.t3), ln)); meta = readLane(st.m3, ln); break;
+    }
+}
+
+__device__ inline uint32_t stackPop(RegStack4& st) {
+    float key;
+    uint32_t meta;
+    stackRead(st, st.size - 1u, key, meta);
+    st.size--;
+    return meta;
+}
+
+__device__ inline float stackTopKey(const RegStack4& st) {
+    float key;
+    uint32_t meta;
+    stackRead(st, st.size - 1u, key, meta);
+    return key;
+}
+
+// one register's part of an insert at position p: entries at positions >= p move up by
+// one (lane 0 takes lane 63 of the register below, `carryT`/`carryM`), position p takes
+// the new entry
+__device__ inline void shiftInsertReg(float& t, uint32_t& m, uint32_t base, uint32_t lane, uint32_t p, float key,
+                                      uint32_t meta, uint32_t carryT, uint32_t carryM) {
+    const float ut = bitsf(laneShiftUp(fbits(t), carryT));
+    const uint32_t um = laneShiftUp(m, carryM);
+    const uint32_t q = base + lane;
+    t = q < p ? t : (q == p ? key : ut);
+    m = q < p ? m : (q == p ? meta : um);
+}
+
+// Inserts (key, meta) one above the highest entry whose key is not smaller; the caller
+// keeps size < 256.
+__device__ inline void stackInsert(RegStack4& st, float key, uint32_t meta, uint32_t lane) {
+    uint32_t p = 0;
+#pragma unroll
+    for (int reg = 3; reg >= 0; reg--) {
+        if (p == 0u && 64u * uint32_t(reg) < st.size) {
+            const float t = reg == 0 ? st.t0 : (reg == 1 ? st.t1 : (reg == 2 ? st.t2 : st.t3));
+            const uint64_t b = ballot(64u * uint32_t(reg) + lane < st.size && !(t < key));
+            if (b)
+                p = 64u * uint32_t(reg) + (64u - uint32_t(__builtin_clzll(b)));
+        }
+    }
+    const uint32_t lo = p >> 6, hi = st.size >> 6; // registers touched (the new top is at position size)
+    // highest register first, so every carry is read before its source register changes
+    if (lo <= 3u && hi >= 3u)
+        shiftInsertReg(st.t3, st.m3, 192u, lane, p, key, meta, readLane(fbits(st.t2), 63), readLane(st.m2, 63));
+    if (lo <= 2u && hi >= 2u)
+        shiftInsertReg(st.t2, st.m2, 128u, lane, p, key, meta, readLane(fbits(st.t1), 63), readLane(st.m1, 63));
+    if (lo <= 1u && hi >= 1u)
+        shiftInsertReg(st.t1, st.m1, 64u, lane, p, key, meta, readLane(fbits(st.t0), 63), readLane(st.m0, 63));
+    if (lo == 0u)
+        shiftInsertReg(st.t0, st.m0, 0u, lane, p, key, meta, 0u, 0u);
+    st.size++;
+}
+
 } // namespace hvvr

@@ -11,6 +11,8 @@
 //   tileListKernel            one wave64 per tile, in three phases (see below); writes the
 //                             tile's list at an offset claimed with one atomic add, so no
 //                             count pass and no scan are needed
+//   tileListLargeKernel       the same for the tiles tileListKernel queued, with 2.7x the
+//                             records and 4x the stack
 //   tileTraverseKernel        one thread per tile with the full 4096-entry stack in global
 //                             memory; only runs the tiles tileListKernel could not hold in
 //                             LDS (count pass, allocation, emit pass)
@@ -211,28 +213,74 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
 // dropped without moving any other pop. A NaN key breaks that order (it inserts on top
 // of anything), so a tile with one, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
+//
+// Two configurations share the code. The standard one (768 records, a one-register
+// 64-entry stack, 10.5 KB of LDS) runs every tile; a tile it cannot hold is queued, and
+// the large one (2048 records, a four-register 256-entry stack, 28 KB of LDS) takes the
+// queue with a few hundred persistent workgroups. Only what neither holds goes to the
+// one-thread-per-tile global-stack kernel.
 // ---------------------------------------------------------------------------------
-enum : uint32_t { TL_REC_CAP = 768, TL_STACK_CAP = REG_STACK_CAP };
-
-struct TileListLds {
-    uint32_t recNode[TL_REC_CAP];
-    uint32_t recKey[TL_REC_CAP];   // float bits of the record's distance key; in phase C, the
-                                   // list length of emitted record i
-    uint32_t recMeta[TL_REC_CAP];  // packMeta(first child record, child mask, leaf mask,
-                                   // stack-entry leaf mask)
-    uint16_t order[TL_REC_CAP];    // records with emitted leaves, in emission order
-};
-
-// Record meta: first child record (bits 0..9), child mask (20..23), leaf mask (24..27),
-// stack-entry leaf mask (28..31). On the traversal stack bits 10..19 carry the record
-// index itself, so a popped entry needs no LDS read.
-__device__ inline uint32_t metaFirst(uint32_t m) { return m & 0x3ffu; }
-__device__ inline uint32_t metaRec(uint32_t m) { return (m >> 10) & 0x3ffu; }
+// Record meta: first child record (bits 0..11), child mask (20..23), leaf mask (24..27),
+// stack-entry leaf mask (28..31).
 __device__ inline uint32_t metaMask(uint32_t m) { return (m >> 20) & 0xfu; }
 __device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 24) & 0xfu; }
 __device__ inline uint32_t metaEntry(uint32_t m) { return m >> 28; }
 __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf, uint32_t entry) {
     return first | (mask << 20) | (leaf << 24) | (entry << 28);
+}
+
+// The standard configuration keeps the record index in bits 10..19 of a stack entry's
+// meta (records < 1024), so a pop needs no LDS read; the large one stacks record indices.
+struct TileListSmall {
+    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1 };
+    using Stack = RegStack;
+};
+struct TileListLarge {
+    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0 };
+    using Stack = RegStack4;
+};
+
+template <class Cfg>
+struct TileListLds {
+    uint32_t recNode[Cfg::RecCap];
+    uint32_t recKey[Cfg::RecCap];  // float bits of the record's distance key; in phase C, the
+                                   // list length of emitted record i
+    uint32_t recMeta[Cfg::RecCap]; // packMeta(first child record, child mask, leaf mask,
+                                   // stack-entry leaf mask)
+    uint16_t order[Cfg::RecCap];   // records with emitted leaves, in emission order
+};
+
+template <class Cfg>
+__device__ inline uint32_t metaFirst(uint32_t m) {
+    return Cfg::Packed ? (m & 0x3ffu) : (m & 0xfffu);
+}
+// stack entry of record r with meta m, and back
+template <class Cfg>
+__device__ inline uint32_t stackEntry(uint32_t r, uint32_t m) {
+    return Cfg::Packed ? (m | (r << 10)) : r;
+}
+template <class Cfg>
+__device__ inline void fromStackEntry(uint32_t e, const TileListLds<Cfg>& sh, uint32_t& rec, uint32_t& meta) {
+    if (Cfg::Packed) {
+        rec = (e >> 10) & 0x3ffu;
+        meta = e;
+    } else {
+        rec = e;
+        meta = uniform(sh.recMeta[e]);
+    }
+}
+
+__device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size) {
+    st.t = t;
+    st.m = m;
+    st.size = size;
+}
+__device__ inline void stackInit(RegStack4& st, float t, uint32_t m, uint32_t size) {
+    st.t0 = t;
+    st.m0 = m;
+    st.t1 = st.t2 = st.t3 = 0.0f;
+    st.m1 = st.m2 = st.m3 = 0u;
+    st.size = size;
 }
 
 // One child box of a node, as one lane of a 4-lane node group loads it.
@@ -272,28 +320,38 @@ __device__ inline uint32_t leafListLength(const BVHNode* __restrict__ nodes, uin
     return n;
 }
 
-__global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__ nodes,
-                                                     const FrustumPre* __restrict__ tilePre,
-                                                     const StackEntry* __restrict__ blockStacks,
-                                                     const uint32_t* __restrict__ blockStackSize,
-                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
-                                                     uint32_t* __restrict__ triOut, uint64_t capacity,
-                                                     uint32_t* __restrict__ overflowTiles,
-                                                     unsigned long long* __restrict__ counters, uint32_t stackLimit) {
-    __shared__ TileListLds sh;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
+struct TileListArgs {
+    const BVHNode* nodes;
+    const FrustumPre* tilePre;
+    const StackEntry* blockStacks;
+    const uint32_t* blockStackSize;
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint32_t* triOut;
+    uint64_t capacity;
+    uint32_t* overflowTiles; // 1 = left for the global-stack kernel
+    uint32_t* largeQueue;    // tiles the standard configuration could not hold
+    unsigned long long* counters;
+    uint32_t stackLimit;
+};
+
+// Builds one tile's list; false when the tile does not fit this configuration (its count
+// and offset are then zero).
+template <class Cfg>
+__device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32_t tile, uint32_t lane) {
+    using Stack = typename Cfg::Stack;
+    const BVHNode* __restrict__ nodes = g.nodes;
     const uint32_t block = tile / TILES_PER_BLOCK;
-    const FrustumPre* __restrict__ f = tilePre + tile;
-    const uint32_t n0 = blockStackSize[block];
+    const FrustumPre* __restrict__ f = g.tilePre + tile;
+    const uint32_t n0 = g.blockStackSize[block];
     if (n0 == 0 || f->degenerate) {
         if (lane == 0) {
-            counts[tile] = 0u;
-            offsets[tile] = 0u;
+            g.counts[tile] = 0u;
+            g.offsets[tile] = 0u;
         }
-        return;
+        return true;
     }
-    const uint32_t limit = stackLimit < TL_STACK_CAP ? stackLimit : TL_STACK_CAP;
+    const uint32_t limit = g.stackLimit < uint32_t(Cfg::StackCap) ? g.stackLimit : uint32_t(Cfg::StackCap);
     bool overflow = n0 > limit;
     uint32_t deepest = n0; // deepest stack size seen
     uint32_t nEmit = 0;
@@ -301,7 +359,7 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
 
     // ---- phase A: node tests, breadth-first ----
     if (!overflow) {
-        const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
+        const StackEntry* bs = g.blockStacks + size_t(block) * BLOCK_STACK_SIZE;
         bool nanKey = false;
         if (lane < n0) {
             StackEntry e = bs[lane];
@@ -351,12 +409,12 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
             const uint32_t cntA = uint32_t(__builtin_popcountll(biA));
             const uint32_t slotA = nrec + lanesBelow(biA);
             const uint32_t slotB = nrec + cntA + lanesBelow(biB);
-            if (innerA && slotA < TL_REC_CAP) {
+            if (innerA && slotA < Cfg::RecCap) {
                 sh.recNode[slotA] = nodeA + A.childOffset;
                 sh.recKey[slotA] = fbits(A.key);
                 sh.recMeta[slotA] = 0u;
             }
-            if (innerB && slotB < TL_REC_CAP) {
+            if (innerB && slotB < Cfg::RecCap) {
                 sh.recNode[slotB] = nodeB + B.childOffset;
                 sh.recKey[slotB] = fbits(B.key);
                 sh.recMeta[slotB] = 0u;
@@ -372,7 +430,7 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
             nrec += cntA + uint32_t(__builtin_popcountll(biB));
             head = end;
             waveLdsSync();
-            if (nrec > TL_REC_CAP || ballot((innerA && A.key != A.key) || (innerB && B.key != B.key)) != 0ull) {
+            if (nrec > Cfg::RecCap || ballot((innerA && A.key != A.key) || (innerB && B.key != B.key)) != 0ull) {
                 overflow = true;
                 break;
             }
@@ -381,28 +439,30 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
 
     // ---- phase B: the reference's stack walk over the live records, stack in VGPRs ----
     if (!overflow) {
-        RegStack st;
+        Stack st;
         {
             // the block stack's live entries, compacted in order (entry q at position q)
             const bool valid = lane < n0;
             const uint32_t key = valid ? sh.recKey[lane] : 0u;
-            const uint32_t meta = valid ? (sh.recMeta[lane] | (lane << 10)) : 0u;
+            const uint32_t meta = valid ? sh.recMeta[lane] : 0u;
             const uint64_t live = ballot(valid && metaMask(meta) != 0u);
-            st.size = uint32_t(__builtin_popcountll(live));
-            const uint32_t dest = ((live >> lane) & 1u) ? lanesBelow(live) : st.size + lanesBelow(~live);
-            st.t = bitsf(uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(key))));
-            st.m = uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(meta)));
-            const float above = __shfl_down(st.t, 1);
-            if (ballot(lane + 1u < st.size && st.t < above) != 0ull)
+            const uint32_t size = uint32_t(__builtin_popcountll(live));
+            const uint32_t dest = ((live >> lane) & 1u) ? lanesBelow(live) : size + lanesBelow(~live);
+            const float t = bitsf(uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(key))));
+            const uint32_t m =
+                uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(stackEntry<Cfg>(lane, meta))));
+            stackInit(st, t, m, size);
+            const float above = __shfl_down(t, 1);
+            if (ballot(lane + 1u < size && t < above) != 0ull)
                 overflow = true;
-            deepest = st.size;
+            deepest = size;
         }
         uint32_t emitBuf = 0; // emission order, 64 records at a time
         uint32_t steps = 0;
-        uint32_t cur = 0;
+        uint32_t curRec = 0, curMeta = 0;
         bool walking = !overflow && st.size != 0u;
         if (walking)
-            cur = stackPop(st);
+            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
         while (walking) {
             // every record is reached at most once; the bound keeps the loop finite whatever
             // the records hold
@@ -410,21 +470,22 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
                 overflow = true;
                 break;
             }
-            const uint32_t mask = metaMask(cur), leafMask = metaLeaf(cur);
+            const uint32_t mask = metaMask(curMeta), leafMask = metaLeaf(curMeta);
             if (mask & leafMask) {
-                emitBuf = writeLane(emitBuf, metaRec(cur), nEmit & 63u, lane);
+                emitBuf = writeLane(emitBuf, curRec, nEmit & 63u, lane);
                 nEmit++;
                 if ((nEmit & 63u) == 0u)
                     sh.order[nEmit - 64u + lane] = uint16_t(emitBuf);
             }
             const uint32_t inner = mask & ~leafMask;
-            if (inner != 0u && metaEntry(cur) == 0u) {
-                const uint32_t first = metaFirst(cur);
+            if (inner != 0u && metaEntry(curMeta) == 0u) {
+                const uint32_t first = metaFirst<Cfg>(curMeta);
                 if ((inner & (inner - 1u)) == 0u) {
                     // single hit child: test it next without touching the stack
                     const uint32_t child = uniform(sh.recMeta[first]);
                     if (metaMask(child) != 0u) {
-                        cur = child | (first << 10);
+                        curRec = first;
+                        curMeta = child;
                         continue;
                     }
                 } else {
@@ -433,7 +494,7 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
                     const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
                     const bool isChild = lane < nChildren;
                     const uint32_t childKey = isChild ? sh.recKey[first + lane] : 0u;
-                    const uint32_t childMeta = isChild ? (sh.recMeta[first + lane] | ((first + lane) << 10)) : 0u;
+                    const uint32_t childMeta = isChild ? sh.recMeta[first + lane] : 0u;
                     const uint32_t liveBits = uint32_t(ballot(isChild && metaMask(childMeta) != 0u));
                     if (liveBits != 0u) {
                         const uint32_t nLive = uint32_t(__builtin_popcount(liveBits));
@@ -456,32 +517,36 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
                             }
                         }
                         const bool take = st.size == 0u || bestKey <= stackTopKey(st);
+                        const uint32_t entry = stackEntry<Cfg>(first + lane, childMeta);
                         for (uint32_t j = 0; j < nChildren; j++)
                             if (((liveBits >> j) & 1u) && !(take && j == best))
-                                stackInsert(st, bitsf(readLane(childKey, j)), readLane(childMeta, j), lane);
+                                stackInsert(st, bitsf(readLane(childKey, j)), readLane(entry, j), lane);
                         const uint32_t depth = st.size + (take ? 1u : 0u);
                         deepest = depth > deepest ? depth : deepest;
-                        cur = take ? readLane(childMeta, best) : stackPop(st);
+                        if (take) {
+                            curRec = first + best;
+                            curMeta = readLane(childMeta, best);
+                        } else {
+                            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
+                        }
                         continue;
                     }
                 }
             }
             if (st.size == 0u)
                 break;
-            cur = stackPop(st);
+            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
         }
         if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
             sh.order[(nEmit & ~63u) + lane] = uint16_t(emitBuf);
         waveLdsSync();
     }
     if (overflow) {
-        // re-traversed by the global-stack kernel
         if (lane == 0) {
-            overflowTiles[tile] = 1u;
-            counts[tile] = 0u;
-            offsets[tile] = 0u;
+            g.counts[tile] = 0u;
+            g.offsets[tile] = 0u;
         }
-        return;
+        return false;
     }
 
     // ---- phase C: emit ----
@@ -499,19 +564,19 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
     }
     unsigned long long start = 0;
     if (lane == 0)
-        start = atomicAdd(&counters[3], (unsigned long long)total);
+        start = atomicAdd(&g.counters[3], (unsigned long long)total);
     start = __shfl(start, 0);
-    const bool fits = start + total <= capacity;
+    const bool fits = start + total <= g.capacity;
     if (lane == 0) {
-        counts[tile] = total;
-        offsets[tile] = uint32_t(start);
+        g.counts[tile] = total;
+        g.offsets[tile] = uint32_t(start);
         if (!fits)
-            atomicOr(&counters[1], 1ull);
-        atomicMax(&counters[2], (unsigned long long)deepest);
-        atomicAdd(&counters[4], (unsigned long long)nrec); // node visits
+            atomicOr(&g.counters[1], 1ull);
+        atomicMax(&g.counters[2], (unsigned long long)deepest);
+        atomicAdd(&g.counters[4], (unsigned long long)nrec); // node visits
     }
     if (!fits)
-        return;
+        return true;
     uint32_t run = uint32_t(start);
     for (uint32_t base = 0; base < nEmit; base += 64) {
         const uint32_t i = base + lane;
@@ -529,9 +594,36 @@ __global__ __launch_bounds__(64) void tileListKernel(const BVHNode* __restrict__
                 const uint32_t t0 = d[k], t1 = d[k + 1];
                 if (leaf & (1u << k))
                     for (uint32_t t = t0; t < t1; t++)
-                        triOut[at++] = t;
+                        g.triOut[at++] = t;
             }
         }
+    }
+    return true;
+}
+
+// every tile, standard configuration; a tile it cannot hold is queued for the large one
+__global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
+    __shared__ TileListLds<TileListSmall> sh;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    if (!tileListBody<TileListSmall>(g, sh, tile, lane) && lane == 0) {
+        g.overflowTiles[tile] = 1u;
+        const unsigned long long slot = atomicAdd(&g.counters[5], 1ull);
+        g.largeQueue[slot] = tile;
+    }
+}
+
+// the queued tiles, large configuration, persistent workgroups; what it cannot hold stays
+// flagged for the global-stack kernel
+__global__ __launch_bounds__(64) void tileListLargeKernel(TileListArgs g) {
+    __shared__ TileListLds<TileListLarge> sh;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t queued = uniform(uint32_t(g.counters[5]));
+    for (uint32_t i = blockIdx.x; i < queued; i += gridDim.x) {
+        const uint32_t tile = uniform(g.largeQueue[i]);
+        waveLdsSync(); // the previous tile's LDS reads are done
+        if (tileListBody<TileListLarge>(g, sh, tile, lane) && lane == 0)
+            g.overflowTiles[tile] = 0u;
     }
 }
 
@@ -711,11 +803,26 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
 }
 
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
-    tileListKernel<<<v->tileCount, 64, 0, stream>>>(s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr,
-                                                    v->blockStackSize.ptr, v->tileCounts.ptr, v->tileOffsets.ptr,
-                                                    v->triIndices.ptr, v->listCapacity, v->overflowTiles.ptr, counters,
-                                                    v->waveStackLimit);
+    TileListArgs g;
+    g.nodes = s->nodes.ptr;
+    g.tilePre = v->tilePre.ptr;
+    g.blockStacks = v->blockStacks.ptr;
+    g.blockStackSize = v->blockStackSize.ptr;
+    g.counts = v->tileCounts.ptr;
+    g.offsets = v->tileOffsets.ptr;
+    g.triOut = v->triIndices.ptr;
+    g.capacity = v->listCapacity;
+    g.overflowTiles = v->overflowTiles.ptr;
+    g.largeQueue = v->largeQueue.ptr;
+    g.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    g.stackLimit = v->waveStackLimit;
+    tileListKernel<<<v->tileCount, 64, 0, stream>>>(g);
+    // the large configuration's stack limit is the view's limit scaled past the standard
+    // one (a test limit below 64 keeps both paths short)
+    TileListArgs big = g;
+    big.stackLimit = v->waveStackLimit >= REG_STACK_CAP ? REG_STACK4_CAP : v->waveStackLimit;
+    const uint32_t groups = v->tileCount < 512u ? v->tileCount : 512u;
+    tileListLargeKernel<<<groups, 64, 0, stream>>>(big);
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {

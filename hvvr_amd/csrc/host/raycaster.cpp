@@ -141,6 +141,11 @@ void Camera::rebuildView() {
         if (s.pixelLocation.x >= _renderTarget.width || s.pixelLocation.y >= _renderTarget.height)
             throw RaycasterError("Camera::setSamples: sample pixel location outside the render target");
 
+    if (_raycaster._spec.mode == RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT) {
+        rebuildFoveatedView();
+        return;
+    }
+
     SampleSetup setup;
     buildSampleSetup(_samples.data(), uint32_t(_samples.size()), _split, _renderTarget.width,
                      _renderTarget.height, getSampleToCamera(), _lens, setup);
@@ -163,6 +168,74 @@ void Camera::rebuildView() {
     _viewDirty = false;
 }
 
+// polarSpaceFoveatedSetup (foveated.cpp:86-113) + GPUCamera::registerPolarFoveatedSamples
+// (gpu_camera.cu:194-230): the spec's polar pattern as an eye-space view whose image is
+// the render target
+void Camera::rebuildFoveatedView() {
+    const auto& sp = _raycaster._spec.foveatedSamplePattern;
+    FoveatedSamplePattern p;
+    p.degreeTrackingError = sp.degreeTrackingError;
+    p.maxFOVDegrees = sp.maxFOVDegrees;
+    p.marSlope = sp.marSlope;
+    p.fovealMARDegrees = sp.fovealMARDegrees;
+    FoveatedSetup setup;
+    std::string err;
+    if (!buildFoveatedSetup(p, setup, &err))
+        throw RaycasterError("Camera: " + err);
+    hvvr_gpu_view_desc d = {};
+    d.msaaRate = _raycaster._spec.msaaRate;
+    d.sampleCount = setup.sampleCount;
+    d.validSampleCount = setup.validSampleCount;
+    d.beams = setup.beams.data();
+    d.tileFrusta = setup.tileFrusta.data();
+    d.blockFrusta = setup.blockFrusta.data();
+    d.remap = nullptr;
+    d.width = _renderTarget.width;
+    d.height = _renderTarget.height;
+    d.split = 1;
+    d.lensRadius = _lens.radius;
+    d.focalDistance = _lens.focalDistance;
+    check(hvvr_gpu_view_create(_raycaster._ctx, &d, &_view), "Camera: foveated view creation failed");
+    hvvr_gpu_foveated_desc f = {};
+    f.polarRemap = (const uint32_t*)setup.polarRemap.data();
+    f.samplesPerRing = setup.samplesPerRing;
+    f.ringCount = setup.ringCount;
+    const EccentricityMap& e = setup.emap;
+    float emap[6] = {e.m, e.w0, e.S, e.invLogA, e.invW0, e.invM};
+    memcpy(f.eccentricityMap, emap, sizeof(emap));
+    f.maxEccentricity = setup.maxEccentricity;
+    f.taaAlpha = 0.05f;
+    f.taaStddevMultiplier = 1.0f;
+    f.contrastEnhancement = 1;
+    f.contrastFe = 1.0f;
+    check(hvvr_gpu_view_set_foveated(_view, &f), "Camera: foveated setup failed");
+    _validSampleCount = setup.validSampleCount;
+    _worldToEyePrevious = matrix4x4(vector4(0.0f), vector4(0.0f), vector4(0.0f), vector4(0.0f));
+    _eyePreviousToSamplePrevious = matrix3x3::identity();
+    _viewDirty = false;
+}
+
+// the frame's eye transforms, screen cull planes and reprojection (render.cpp:454-488)
+void Camera::foveatedFrame(hvvr_gpu_foveated_frame& out) {
+    // sample bounds of the camera's own samples (camera.cpp:36-42)
+    vector2 lo(1.0f, 1.0f), hi(0.0f, 0.0f);
+    for (const Sample& s : _samples) {
+        lo = vector2(std::min(lo.x, s.position.x), std::min(lo.y, s.position.y));
+        hi = vector2(std::max(hi.x, s.position.x), std::max(hi.y, s.position.y));
+    }
+    FoveatedHistory h;
+    h.worldToEyePrevious = _worldToEyePrevious;
+    h.eyePreviousToSamplePrevious = _eyePreviousToSamplePrevious;
+    FoveatedFrame fr;
+    hvvr::foveatedFrame(getCameraToWorld(), _eyeDir, getSampleToCamera(), lo, hi, h, fr);
+    _worldToEyePrevious = h.worldToEyePrevious;
+    _eyePreviousToSamplePrevious = h.eyePreviousToSamplePrevious;
+    memcpy(out.eyeToWorld, &fr.eyeToWorld, sizeof(out.eyeToWorld));
+    memcpy(out.cullPlanes, fr.cullPlanes, sizeof(out.cullPlanes));
+    memcpy(out.sampleToEye, &fr.sampleToEye, sizeof(out.sampleToEye));
+    memcpy(out.eyeToSamplePrevious, &fr.eyeToSamplePrevious, sizeof(out.eyeToSamplePrevious));
+}
+
 void Camera::extractImage() {
     if (!_view || !_renderTarget.data)
         return;
@@ -181,8 +254,6 @@ vector2 Camera::nextJitter() {
 // Raycaster
 // ---------------------------------------------------------------------------------
 Raycaster::Raycaster(const RayCasterSpecification& spec) : _spec(spec) {
-    if (_spec.mode != RayCasterGPUMode::GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE)
-        throw RaycasterError("Raycaster: only GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE is implemented");
     if (_spec.outputTo3DApi)
         throw RaycasterError("Raycaster: 3D-API output (D3D11/GL interop) is not supported; use CPU render targets");
     if (!validMsaaRate(_spec.msaaRate))
@@ -200,8 +271,6 @@ Raycaster::~Raycaster() {
 
 void Raycaster::reinit(RayCasterGPUMode mode) {
     // raycaster.cpp:57-76: drops cameras and the GPU scene, keeps models and lights
-    if (mode != RayCasterGPUMode::GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE)
-        throw RaycasterError("Raycaster::reinit: only GPU_INTERSECT_AND_RECONSTRUCT_DEFERRED_MSAA_RESOLVE is implemented");
     if (_ctx)
         hvvr_gpu_context_synchronize(_ctx);
     _cameras.clear();
@@ -333,6 +402,19 @@ void Raycaster::render(double elapsedTime) {
     }
     if (active.empty())
         return;
+
+    if (_spec.mode == RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT) {
+        // renderFoveatedPolarSpaceCudaReconstruct (render.cpp:454-488), camera by camera
+        for (Camera* c : active) {
+            hvvr_gpu_foveated_frame fr;
+            c->foveatedFrame(fr);
+            vector2 j = c->nextJitter();
+            check(hvvr_gpu_view_render_foveated(c->_view, _scene, &fr, &j.x), "Raycaster::render");
+        }
+        for (Camera* c : active)
+            c->extractImage();
+        return;
+    }
 
     std::vector<hvvr_gpu_view*> views;
     std::vector<matrix4x4> cameraToWorld;

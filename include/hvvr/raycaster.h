@@ -8,8 +8,12 @@
 //    compiles it in as COLOR_MODE_MSAA_RATE = 16, raycaster_common.h:27);
 //  * errors throw hvvr::RaycasterError instead of printing and exiting;
 //  * render targets are CPU memory (ImageResourceDescriptor::MemoryType::CPU_MEMORY);
-//    D3D11/GL interop, textures and the foveated mode are outside this build;
-//  * depth of field follows the camera's lens (setLens / setAperture / setFocalDepth).
+//    D3D11/GL interop and textures are outside this build;
+//  * depth of field follows the camera's lens (setLens / setAperture / setFocalDepth);
+//  * in the foveated mode (GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT) each camera traces
+//    the spec's polar sample pattern around its eye direction (setEyeDir, camera space)
+//    and reconstructs its render target; the camera's own samples set the screen
+//    rectangle tiles are culled to, as in the reference.
 
 #include "types.h"
 
@@ -21,6 +25,7 @@
 struct hvvr_gpu_context;
 struct hvvr_gpu_scene;
 struct hvvr_gpu_view;
+struct hvvr_gpu_foveated_frame;
 
 namespace hvvr {
 
@@ -146,6 +151,8 @@ public:
 
 protected:
     void rebuildView();
+    void rebuildFoveatedView();
+    void foveatedFrame(hvvr_gpu_foveated_frame& out);
     vector2 nextJitter();
 
     Raycaster& _raycaster;
@@ -162,6 +169,9 @@ protected:
     hvvr_gpu_view* _view = nullptr;
     std::vector<vector2> _frameJitters;
     uint32_t _frameCount = 0;
+    // foveated mode: previous-frame eye transforms (camera.h:121-122)
+    matrix4x4 _worldToEyePrevious = matrix4x4(vector4(0.0f), vector4(0.0f), vector4(0.0f), vector4(0.0f));
+    matrix3x3 _eyePreviousToSamplePrevious = matrix3x3::identity();
 };
 
 class Raycaster {

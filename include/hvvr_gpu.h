@@ -143,7 +143,7 @@ typedef struct {
 } hvvr_gpu_foveated_desc;
 int hvvr_gpu_view_set_foveated(hvvr_gpu_view* view, const hvvr_gpu_foveated_desc* desc);
 /* Per-frame transforms, as hvvr_host_foveated_frame computes them. */
-typedef struct {
+typedef struct hvvr_gpu_foveated_frame {
     float eyeToWorld[16];          /* eye (beam) space to world; the frame's camera-to-world */
     float cullPlanes[16];          /* 4 world-space planes (normal xyz, dist); tiles wholly on the
                                       positive side of one are skipped; dist[0] = +inf: no culling */

@@ -1,4 +1,4 @@
-// GPU unit probe for the VGPR-resident traversal stack (hvvr_amd/csrc/gpu/reg_stack.h):
+// GPU unit probe for the VGPR-resident traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h):
 // one wave runs a host-given sequence of inserts and pops, lane 0 records every popped
 // record, and the final register contents are written out. Test infrastructure only.
 
@@ -8,17 +8,42 @@
 
 namespace {
 
-// ops[i]: 0 = pop (popped[i] = record), 1 = insert (keys[i], record i as the entry's meta).
-// finalT/finalS[q] = key/meta at stack position q (0 = bottom).
-__global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const float* keys, uint32_t n,
-                                                   uint32_t* popped, float* finalT, uint32_t* finalS,
-                                                   uint32_t* finalSize) {
-    using namespace hvvr;
-    const uint32_t lane = threadIdx.x;
-    RegStack st;
+using namespace hvvr;
+
+__device__ void probeInit(RegStack& st) {
     st.t = 0.0f;
     st.m = 0u;
     st.size = 0;
+}
+__device__ void probeInit(RegStack4& st) {
+    st.t0 = st.t1 = st.t2 = st.t3 = 0.0f;
+    st.m0 = st.m1 = st.m2 = st.m3 = 0u;
+    st.size = 0;
+}
+__device__ void probeStore(const RegStack& st, uint32_t lane, float* finalT, uint32_t* finalS) {
+    finalT[lane] = st.t;
+    finalS[lane] = st.m;
+}
+__device__ void probeStore(const RegStack4& st, uint32_t lane, float* finalT, uint32_t* finalS) {
+    finalT[lane] = st.t0;
+    finalT[64 + lane] = st.t1;
+    finalT[128 + lane] = st.t2;
+    finalT[192 + lane] = st.t3;
+    finalS[lane] = st.m0;
+    finalS[64 + lane] = st.m1;
+    finalS[128 + lane] = st.m2;
+    finalS[192 + lane] = st.m3;
+}
+
+// ops[i]: 0 = pop (popped[i] = record), 1 = insert (keys[i], record i as the entry's meta).
+// finalT/finalS[q] = key/meta at stack position q (0 = bottom).
+template <class Stack, uint32_t Cap>
+__global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const float* keys, uint32_t n,
+                                                   uint32_t* popped, float* finalT, uint32_t* finalS,
+                                                   uint32_t* finalSize) {
+    const uint32_t lane = threadIdx.x;
+    Stack st;
+    probeInit(st);
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t op = uniform(ops[i]);
         if (op == 0u) {
@@ -27,38 +52,40 @@ __global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const f
             const uint32_t rec = stackPop(st);
             if (lane == 0)
                 popped[i] = rec;
-        } else if (st.size < REG_STACK_CAP) {
+        } else if (st.size < Cap) {
             stackInsert(st, bitsf(uniform(fbits(keys[i]))), i, lane);
         }
     }
-    finalT[lane] = st.t;
-    finalS[lane] = st.m;
+    probeStore(st, lane, finalT, finalS);
     if (lane == 0)
         *finalSize = st.size;
 }
 
-} // namespace
-
-extern "C" int reg_stack_probe_run(const uint32_t* ops, const float* keys, uint32_t n, uint32_t* popped,
-                                   float* finalT, uint32_t* finalS, uint32_t* finalSize) {
+// capacity 64 (RegStack) or 256 (RegStack4)
+int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n, uint32_t* popped, float* finalT,
+             uint32_t* finalS, uint32_t* finalSize) {
+    if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP)
+        return -5;
     uint32_t *dOps = nullptr, *dPopped = nullptr, *dS = nullptr, *dSize = nullptr;
     float *dKeys = nullptr, *dT = nullptr;
     int rc = 0;
     if (hipMalloc(&dOps, n * 4) || hipMalloc(&dKeys, n * 4) || hipMalloc(&dPopped, n * 4) ||
-        hipMalloc(&dT, 64 * 4) || hipMalloc(&dS, 64 * 4) || hipMalloc(&dSize, 4))
+        hipMalloc(&dT, capacity * 4) || hipMalloc(&dS, capacity * 4) || hipMalloc(&dSize, 4))
         rc = -1;
     if (!rc && (hipMemcpy(dOps, ops, n * 4, hipMemcpyHostToDevice) ||
-                hipMemcpy(dKeys, keys, n * 4, hipMemcpyHostToDevice) ||
-                hipMemset(dPopped, 0xff, n * 4)))
+                hipMemcpy(dKeys, keys, n * 4, hipMemcpyHostToDevice) || hipMemset(dPopped, 0xff, n * 4)))
         rc = -2;
     if (!rc) {
-        regStackProbe<<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
+        if (capacity == REG_STACK_CAP)
+            regStackProbe<RegStack, REG_STACK_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
+        else
+            regStackProbe<RegStack4, REG_STACK4_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         if (hipDeviceSynchronize() != hipSuccess)
             rc = -3;
     }
     if (!rc && (hipMemcpy(popped, dPopped, n * 4, hipMemcpyDeviceToHost) ||
-                hipMemcpy(finalT, dT, 64 * 4, hipMemcpyDeviceToHost) ||
-                hipMemcpy(finalS, dS, 64 * 4, hipMemcpyDeviceToHost) ||
+                hipMemcpy(finalT, dT, capacity * 4, hipMemcpyDeviceToHost) ||
+                hipMemcpy(finalS, dS, capacity * 4, hipMemcpyDeviceToHost) ||
                 hipMemcpy(finalSize, dSize, 4, hipMemcpyDeviceToHost)))
         rc = -4;
     (void)hipFree(dOps);
@@ -68,4 +95,11 @@ extern "C" int reg_stack_probe_run(const uint32_t* ops, const float* keys, uint3
     (void)hipFree(dS);
     (void)hipFree(dSize);
     return rc;
+}
+
+} // namespace
+
+extern "C" int reg_stack_probe_run(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n,
+                                   uint32_t* popped, float* finalT, uint32_t* finalS, uint32_t* finalSize) {
+    return probeRun(capacity, ops, keys, n, popped, finalT, finalS, finalSize);
 }

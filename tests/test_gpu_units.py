@@ -1,9 +1,11 @@
 """GPU unit probes of wave-level building blocks, against Python models.
 
-The VGPR-resident traversal stack (hvvr_amd/csrc/gpu/reg_stack.h): random sequences of
-sorted inserts and pops that keep it shallow, half full, and at its 64-entry capacity; every popped record and the final contents must match a list model of the
-reference's insertion rule (traverse_avx.h:499-520: a new entry goes below the run of
-top entries whose key is smaller)."""
+The VGPR-resident traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h), one register (64
+entries) and four registers (256): random sequences of sorted inserts and pops that keep
+them shallow, past each register boundary and at capacity; every popped record and the
+final contents must match a list model of the reference's insertion rule
+(traverse_avx.h:499-520: a new entry goes below the run of top entries whose key is
+smaller)."""
 
 import ctypes
 import os
@@ -16,14 +18,14 @@ from conftest import ROOT
 LIB = os.path.join(ROOT, "tests", "gpu_units", "_build", "libhvvr_gpu_units.so")
 
 
-def _model(ops, keys):
+def _model(ops, keys, cap):
     stack = []  # index 0 = top
     popped = np.full(len(ops), 0xFFFFFFFF, np.uint32)
     for i, (op, k) in enumerate(zip(ops, keys)):
         if op == 0:
             if stack:
                 popped[i] = stack.pop(0)[1]
-        elif len(stack) < 64:
+        elif len(stack) < cap:
             r = 0
             while r < len(stack) and stack[r][0] < k:
                 r += 1
@@ -41,22 +43,31 @@ def test_probe_library_built():
     assert os.path.exists(LIB), f"{LIB} missing: run __graft_entry__.build()"
 
 
+CASES = {"small": (300, 0.55), "deep": (400, 0.6), "full": (1500, 0.75), "ties": (1200, 0.7)}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["small", "deep", "full", "ties"])
-def test_register_stack_matches_model(case):
+@pytest.mark.parametrize("cap", [64, 256])
+@pytest.mark.parametrize("case", list(CASES))
+def test_register_stack_matches_model(case, cap):
     lib = ctypes.CDLL(LIB)
-    rng = np.random.default_rng(["small", "deep", "full", "ties"].index(case))
-    n, bias = {"small": (300, 0.55), "deep": (400, 0.6), "full": (1500, 0.75), "ties": (1200, 0.7)}[case]
+    rng = np.random.default_rng(list(CASES).index(case) + cap)
+    n, bias = CASES[case]
+    if cap == 256:
+        n *= 3  # enough pushes to cross every register boundary
     pool = np.arange(8, dtype=np.float32) if case == "ties" else rng.normal(size=4096).astype(np.float32)
     ops, keys = _sequence(rng, n, bias, pool)
     popped = np.zeros(n, np.uint32)
-    ft = np.zeros(64, np.float32)
-    fs = np.zeros(64, np.uint32)
+    ft = np.zeros(cap, np.float32)
+    fs = np.zeros(cap, np.uint32)
     size = np.zeros(1, np.uint32)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    rc = lib.reg_stack_probe_run(p(ops), p(keys), ctypes.c_uint32(n), p(popped), p(ft), p(fs), p(size))
+    rc = lib.reg_stack_probe_run(ctypes.c_uint32(cap), p(ops), p(keys), ctypes.c_uint32(n), p(popped), p(ft), p(fs),
+                                 p(size))
     assert rc == 0
-    want_popped, want_stack = _model(ops, keys)
+    want_popped, want_stack = _model(ops, keys, cap)
+    if case != "small":
+        assert max(_depths(ops, cap)) > cap * 3 // 4
     assert int(size[0]) == len(want_stack)
     pops = ops == 0
     assert np.array_equal(popped[pops], want_popped[pops]), "popped records differ"
@@ -64,3 +75,11 @@ def test_register_stack_matches_model(case):
     top = len(want_stack) - 1
     got = [(float(ft[top - d]), int(fs[top - d])) for d in range(len(want_stack))]
     assert got == [(float(k), r) for k, r in want_stack], "final stack differs"
+
+
+def _depths(ops, cap):
+    d, out = 0, []
+    for o in ops:
+        d = max(d - 1, 0) if o == 0 else min(d + 1, cap)
+        out.append(d)
+    return out

@@ -145,3 +145,33 @@ def test_cpp_api_matches_oracle(H, O, tmp_path):
     assert (want0 != 0xFF000000).mean() > 0.3, "scene should cover a good part of the frame"
     assert rgba_delta(imgs[0], want0).max() <= 1
     assert rgba_delta(imgs[1], want1).max() <= 1
+
+
+@pytest.mark.gpu
+def test_cpp_api_foveated_matches_c_abi(H, ctx, tmp_path):
+    # the Raycaster's foveated mode (eye transforms, screen cull planes, history) against
+    # the same frames driven through the C ABI: bit-identical images
+    models, lights = _scene(H)
+    path, out = tmp_path / "scene.bin", tmp_path / "out.rgba"
+    _write_scene(path, models, lights)
+    r = subprocess.run([BIN, str(path), str(out), "2", "foveated"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(out, np.uint32).reshape(H_, W)
+    meshes, mat_offset, all_mats = [], 0, []
+    for (v, tris, mats), materials, (t, rot, s) in models:
+        meshes.append((v, tris, mats + mat_offset, H.transform_matrix(t, rot, s)))
+        all_mats.append(materials)
+        mat_offset += len(materials)
+    scene = H.HostScene.from_meshes(meshes, np.concatenate(all_mats), lights)
+    screen = H.HostView.from_samples(H.grid_samples(W, H_), W, H_, VIEWPORT)
+    fov = H.HostFoveated(H.HostFoveated.FEB2017)
+    gv = H.GpuFoveatedView(ctx, fov, W, H_, MSAA)
+    gs = H.GpuScene(ctx, scene)
+    hist = H.foveated_history()
+    m = H.transform_matrix(*CAMERA)
+    for _ in range(2):
+        fr = H.foveated_frame(m, (0.0, 0.0, -1.0), screen.sample_to_camera, screen.sample_bounds, hist)
+        gv.render_foveated(gs, fr)
+    want = gv.read_image()
+    assert (want != 0).mean() > 0.9
+    assert np.array_equal(img, want)
