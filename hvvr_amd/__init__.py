@@ -450,6 +450,9 @@ class GpuView:
     def set_traversal_stack_limit(self, entries):
         _check_gpu(self.lib.hvvr_gpu_view_set_traversal_stack_limit(self.handle, c_uint32(entries)))
 
+    def set_shading_precision(self, bits):
+        _check_gpu(self.lib.hvvr_gpu_view_set_shading_precision(self.handle, c_int(bits)))
+
     def set_jitter_spin(self, enable):
         _check_gpu(self.lib.hvvr_gpu_view_set_jitter_spin(self.handle, c_int(1 if enable else 0)))
 

@@ -442,6 +442,15 @@ int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* v, uint32_t entries) 
     return HVVR_GPU_OK;
 }
 
+int hvvr_gpu_view_set_shading_precision(hvvr_gpu_view* v, int bits) {
+    if (!v)
+        return fail("null view");
+    if (bits != 16 && bits != 32)
+        return fail("shading precision must be 16 or 32 bits");
+    v->halfShading = bits == 16;
+    return HVVR_GPU_OK;
+}
+
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
     if (!v)
         return fail("null view");
@@ -666,6 +675,8 @@ int hvvr_gpu_render_views(hvvr_gpu_scene* s, uint32_t count, hvvr_gpu_view* cons
             return fail("null view");
         if (views[i]->ctx != s->ctx)
             return fail("view and scene belong to different contexts");
+        if (views[i]->foveated)
+            return fail("foveated views render with hvvr_gpu_view_render_foveated");
     }
     if (count == 0)
         return HVVR_GPU_OK;

@@ -121,6 +121,7 @@ struct hvvr_gpu_view {
     hvvr::ThinLens lens = {0.0f, 1.0f};
     bool dof = false;                                   // lens.radius > 0
     bool jitterSpin = false;
+    bool halfShading = false;                           // GGX colour terms in packed FP16
     hvvr::vector2 offsets[32];
     hvvr::matrix4x4 lastCameraToWorld = hvvr::matrix4x4::identity();
 

@@ -165,13 +165,95 @@ __device__ inline vector3 ggxShade(const SimpleMaterial& mat, vector3 pos, vecto
     return vector3(fmaxv(rad.x, 0.0f), fmaxv(rad.y, 0.0f), fmaxv(rad.z, 0.0f));
 }
 
+// ---- reduced-precision shading: the colour terms of each light in packed FP16 ----
+// PBR_BRDF1 = diffuse + lightColor * spec with diffuse = base * lightColor * (1 - spec) *
+// NdotL and spec = F * s, s = D * vis / 4, which is lightColor * (base NdotL) +
+// lightColor * F * (1 - base NdotL) * s. The scalar terms (D, vis, Fresnel weight) keep
+// FP32: D * vis spans many decades (up to ~1e7 for a glossy highlight seen edge-on). The
+// two colour vectors, bounded by the light colour, are formed with packed FP16 operations
+// (v_pk_mul_f16 / v_pk_add_f16); the radiance sum stays FP32.
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ inline half4v toHalf4(const vector3& v) {
+    return __builtin_convertvector((float4v{v.x, v.y, v.z, 0.0f}), half4v);
+}
+
+__device__ inline void pbrBrdfHalf(const vector3& L, const vector3& N, const vector3& V, half4v base,
+                                   float roughness, half4v F0, half4v lightColor, float4v& rad) {
+    float alpha = roughness * roughness;
+    vector3 H = normalize(L + V);
+    float alpha2 = alpha * alpha;
+    float NdotH = saturate(dot(N, H));
+    float q = NdotH * NdotH * (alpha2 - 1.0f) + 1.0f;
+    float D = alpha2 / (3.141592653589f * (q * q));
+    float LdotH = saturate(dot(L, H));
+    float x = 1.0f - LdotH;
+    float x2 = x * x;
+    float p5 = x2 * x2 * x;
+    float NdotL = saturate(dot(N, L));
+    float NdotV = saturate(dot(N, V));
+    float k = alpha * 0.5f;
+    float vis = (1.0f / (NdotL * (1.0f - k) + k)) * (1.0f / (NdotV * (1.0f - k) + k));
+    float sc = (D * vis) * 0.25f;
+    const half4v one = (_Float16)1.0f;
+    const half4v F = F0 + (one - F0) * (_Float16)p5;
+    const half4v a = base * (_Float16)NdotL;
+    const half4v u = lightColor * a;
+    const half4v w = (lightColor * F) * (one - a);
+    rad += __builtin_convertvector(u, float4v) + __builtin_convertvector(w, float4v) * sc;
+}
+
+__device__ inline vector3 ggxShadeHalf(const SimpleMaterial& mat, vector3 pos, vector3 nrm, vector3 camPos,
+                                       const LightingEnvironment& env) {
+    vector3 normal = normalize(nrm);
+    uint32_t model = uint32_t(mat.textureIDsAndShadingModel & 0xffffu);
+    if (model == ShadingModelEmissive)
+        return mat.emissive.xyz();
+    const half4v base = toHalf4(mat.diffuse.xyz());
+    vector3 V = normalize(camPos - pos);
+    const half4v F0 = toHalf4(mat.specular.xyz());
+    float roughness = 1.0f - mat.glossiness;
+    vector3 amb = mat.emissive.xyz() + mat.diffuse.xyz() * vector3(0.15f, 0.15f, 0.165f);
+    float4v rad = {amb.x, amb.y, amb.z, 0.0f};
+    for (int i = 0; i < MAX_DIRECTIONAL_LIGHTS; i++) {
+        if (i >= env.directionalLightCount)
+            break;
+        pbrBrdfHalf(env.directionalLights[i].Direction, normal, V, base, roughness, F0,
+                    toHalf4(env.directionalLights[i].Power), rad);
+    }
+    for (int i = 0; i < MAX_POINT_LIGHTS; i++) {
+        if (i >= env.pointLightCount)
+            break;
+        const LightPoint& l = env.pointLights[i];
+        vector3 delta = l.Position - pos;
+        float r = length(delta);
+        vector3 L = normalize(delta);
+        float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
+        pbrBrdfHalf(L, normal, V, base, roughness, F0, toHalf4(l.Color * fdist), rad);
+    }
+    for (int i = 0; i < MAX_SPOT_LIGHTS; i++) {
+        if (i >= env.spotLightCount)
+            break;
+        const LightSpot& l = env.spotLights[i];
+        vector3 delta = l.Position - pos;
+        float r = length(delta);
+        vector3 L = normalize(delta);
+        float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
+        float cosA = dot(l.Direction, -L);
+        float fang = saturate((cosA - l.CosOuterAngle) * l.CosAngleScale);
+        pbrBrdfHalf(L, normal, V, base, roughness, F0, toHalf4(l.Color * (fdist * fang)), rad);
+    }
+    return vector3(fmaxv(rad.x, 0.0f), fmaxv(rad.y, 0.0f), fmaxv(rad.z, 0.0f));
+}
+
 __device__ inline float acesFilm(float x) {
     // ACESFilm (shading_helpers.h:158-170)
     const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
     return saturate((x * (a * x + b)) / (x * (c * x + d) + e));
 }
 
-template <int A, bool DoF>
+template <int A, bool DoF, bool Half>
 __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
     __shared__ DofTile dofTile;
     const uint32_t tile = blockIdx.x;
@@ -250,7 +332,8 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
         vector3 n1(halfToFloat(V1.normal[0]), halfToFloat(V1.normal[1]), halfToFloat(V1.normal[2]));
         vector3 n2(halfToFloat(V2.normal[0]), halfToFloat(V2.normal[1]), halfToFloat(V2.normal[2]));
         vector3 nrm = n0 * bx + n1 * by + n2 * bz;
-        vector3 c = ggxShade(args.materials[S.material], pos, nrm, cam, args.env);
+        vector3 c = Half ? ggxShadeHalf(args.materials[S.material], pos, nrm, cam, args.env)
+                         : ggxShade(args.materials[S.material], pos, nrm, cam, args.env);
         result += c * float(__popc(mask));
     }
     result = result * (1.0f / float(A));
@@ -328,13 +411,18 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
     a.tMax = v->foveated ? v->tMax.ptr : nullptr;
     dim3 grid(v->tileCount), block(128);
     const bool dof = v->dof;
+    const bool half = v->halfShading;
     switch (v->msaaRate) {
 #define HVVR_RESOLVE_CASE(A)                                                          \
     case A:                                                                           \
-        if (dof)                                                                      \
-            resolveKernel<A, true><<<grid, block, 0, stream>>>(a);                    \
+        if (dof && half)                                                              \
+            resolveKernel<A, true, true><<<grid, block, 0, stream>>>(a);              \
+        else if (dof)                                                                 \
+            resolveKernel<A, true, false><<<grid, block, 0, stream>>>(a);             \
+        else if (half)                                                                \
+            resolveKernel<A, false, true><<<grid, block, 0, stream>>>(a);             \
         else                                                                          \
-            resolveKernel<A, false><<<grid, block, 0, stream>>>(a);                   \
+            resolveKernel<A, false, false><<<grid, block, 0, stream>>>(a);            \
         break;
         HVVR_RESOLVE_CASE(1)
         HVVR_RESOLVE_CASE(2)

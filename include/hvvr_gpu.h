@@ -121,6 +121,10 @@ int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* 
  * stack. The result is the same either way; the limit only moves work between the two
  * paths. */
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entries);
+/* 32 (default): FP32 shading, the oracle's arithmetic. 16: the GGX colour terms of each
+ * light in packed FP16 (the scalar D / visibility / Fresnel weights and the radiance sum
+ * stay FP32); error stated against the FP64 oracle. */
+int hvvr_gpu_view_set_shading_precision(hvvr_gpu_view* view, int bits);
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
 
