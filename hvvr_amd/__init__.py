@@ -203,7 +203,16 @@ class HostView:
         self.sample_to_camera = np.zeros(9, np.float32)
         self.sample_bounds = np.zeros(4, np.float32)
         _check_host(lib.hvvr_host_view_sample_space(h, _ptr(self.sample_to_camera), _ptr(self.sample_bounds)))
-        lib.hvvr_host_view_destroy(h)
+        self.handle = h  # kept for the serving loop (hvvr_server_add_shape)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                load_host().hvvr_host_view_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
 
     @classmethod
     def from_samples(cls, samples, width, height, viewport, split=1, lens_radius=0.0, focal_distance=1.0):
@@ -245,8 +254,19 @@ class HostFoveated:
             mx = c_float(0.0)
             _check_host(lib.hvvr_host_foveated_params(h, _ptr(self.emap), byref(mx)))
             self.max_eccentricity = float(mx.value)
-        finally:
+        except Exception:
             lib.hvvr_host_foveated_destroy(h)
+            raise
+        self.handle = h  # kept for the serving loop (hvvr_server_add_foveated_shape)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                load_host().hvvr_host_foveated_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
 
 
 def foveated_history():
@@ -547,3 +567,88 @@ class GpuFoveatedView(GpuView):
         _check_gpu(self.lib.hvvr_gpu_view_read_foveated(self.handle, c_int(FOVEATED_BUFFERS[name]), _ptr(out),
                                                         c_uint64(out.nbytes)))
         return out
+
+
+# ---------------------------------------------------------------------------------------
+# serving loop (include/hvvr_server.h)
+# ---------------------------------------------------------------------------------------
+class _ServerDesc(ctypes.Structure):
+    _fields_ = [("deviceBudgetBytes", c_uint64), ("maxInFlight", c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("queued", c_uint64), ("inFlight", c_uint64),
+                ("deviceBytes", c_uint64), ("peakDeviceBytes", c_uint64), ("viewsCreated", c_uint32),
+                ("viewsEvicted", c_uint32), ("peakInFlight", c_uint32), ("p50Ms", ctypes.c_double),
+                ("p99Ms", ctypes.c_double), ("meanMs", ctypes.c_double)]
+
+
+class Server:
+    """Frame-request serving loop: resident scenes, view shapes admitted by HBM footprint,
+    requests overlapping on the device (hvvr_server_*)."""
+
+    def __init__(self, ctx: GpuContext, budget_bytes, max_in_flight=8):
+        self.ctx = ctx
+        self.lib = load_host()
+        self.lib.hvvr_server_create.argtypes = [c_void_p, c_void_p, c_void_p]
+        self.handle = c_void_p()
+        d = _ServerDesc(int(budget_bytes), int(max_in_flight))
+        _check_host(self.lib.hvvr_server_create(ctx.handle, byref(d), byref(self.handle)))
+        self.shapes = []
+        self._images = {}
+
+    def add_shape(self, view: HostView, msaa_rate):
+        sid = c_uint32()
+        _check_host(self.lib.hvvr_server_add_shape(self.handle, view.handle, c_uint32(msaa_rate), byref(sid)))
+        self.shapes.append((view.height, view.width))
+        return sid.value
+
+    def add_foveated_shape(self, fov: HostFoveated, screen: HostView, msaa_rate):
+        sid = c_uint32()
+        _check_host(self.lib.hvvr_server_add_foveated_shape(self.handle, fov.handle, screen.handle,
+                                                            c_uint32(msaa_rate), byref(sid)))
+        self.shapes.append((screen.height, screen.width))
+        return sid.value
+
+    def shape_bytes(self, shape):
+        b = c_uint64()
+        _check_host(self.lib.hvvr_server_shape_bytes(self.handle, c_uint32(shape), byref(b)))
+        return b.value
+
+    def submit(self, scene: GpuScene, shape, camera_to_world, eye_dir=None):
+        """Queues a frame; returns (request id, the image array it will be written to)."""
+        img = np.zeros(self.shapes[shape], np.uint32)
+        m = np.ascontiguousarray(camera_to_world, np.float32).reshape(16)
+        e = None if eye_dir is None else np.ascontiguousarray(eye_dir, np.float32)
+        rid = c_uint64()
+        _check_host(self.lib.hvvr_server_submit(self.handle, scene.handle, c_uint32(shape), _ptr(m),
+                                                _ptr(e) if e is not None else None, _ptr(img), byref(rid)))
+        self._images[rid.value] = img
+        return rid.value, img
+
+    def poll(self):
+        return _check_host(self.lib.hvvr_server_poll(self.handle))
+
+    def drain(self):
+        _check_host(self.lib.hvvr_server_drain(self.handle))
+
+    def done(self, rid):
+        lat = ctypes.c_double()
+        r = _check_host(self.lib.hvvr_server_request_done(self.handle, c_uint64(rid), byref(lat)))
+        return (r == 1), lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check_host(self.lib.hvvr_server_get_stats(self.handle, byref(st)))
+        return st
+
+    def close(self):
+        if self.handle:
+            self.lib.hvvr_server_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -217,6 +217,10 @@ int hvvr_gpu_scene_create(hvvr_gpu_context* ctx, const hvvr_gpu_scene_desc* d, h
         return rc;
 
     hvvr_gpu_scene* s = new hvvr_gpu_scene();
+    if (hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) != hipSuccess) {
+        delete s;
+        return fail("scene event creation failed", HVVR_GPU_ERR_HIP);
+    }
     s->ctx = ctx;
     s->vertexCount = d->vertexCount;
     s->triangleCount = d->triangleCount;
@@ -329,6 +333,7 @@ int hvvr_gpu_scene_update(hvvr_gpu_scene* s) {
     HVVR_HIP_CHECK(hipSetDevice(s->ctx->device));
     launchSceneRefit(s, s->ctx->stream);
     HVVR_HIP_CHECK(hipGetLastError());
+    HVVR_HIP_CHECK(hipEventRecord(s->ready, s->ctx->stream));
     s->dirty = false;
     s->everUpdated = true;
     return HVVR_GPU_OK;
@@ -510,7 +515,17 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     HVVR_HIP_CHECK(hipGetLastError());
     v->stagesRecorded = prof;
     v->rendered = true;
+    v->lastStream = st;
     return HVVR_GPU_OK;
+}
+
+// the stream a single-view render goes to: the view's own (after the scene's last
+// update) or the context's
+static hipStream_t renderStream(hvvr_gpu_view* v, hvvr_gpu_scene* s) {
+    if (!v->ownStream)
+        return v->ctx->stream;
+    (void)hipStreamWaitEvent(v->ownStream, s->ready, 0);
+    return v->ownStream;
 }
 
 int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter) {
@@ -523,7 +538,7 @@ int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, 
         if (rc)
             return rc;
     }
-    return renderOne(v, s, m16, jitter, v->ctx->stream);
+    return renderOne(v, s, m16, jitter, renderStream(v, s));
 }
 
 static int viewSync(hvvr_gpu_view* v);
@@ -632,7 +647,7 @@ int hvvr_gpu_view_render_foveated(hvvr_gpu_view* v, hvvr_gpu_scene* s, const hvv
         if (rc)
             return rc;
     }
-    return renderOne(v, s, fr->eyeToWorld, jitter, v->ctx->stream, fr);
+    return renderOne(v, s, fr->eyeToWorld, jitter, renderStream(v, s), fr);
 }
 
 int hvvr_gpu_view_read_foveated(hvvr_gpu_view* v, int buffer, void* dst, uint64_t bytes) {
@@ -720,12 +735,119 @@ static int viewSync(hvvr_gpu_view* v) {
         return fail("null view");
     if (!v->rendered)
         return fail("view has not been rendered", HVVR_GPU_ERR_STATE);
-    HVVR_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    hipStream_t st = v->lastStream ? v->lastStream : v->ctx->stream;
+    HVVR_HIP_CHECK(hipStreamSynchronize(st));
+    if (st != v->ctx->stream)
+        HVVR_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
     uint64_t counters[4];
-    HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+    HVVR_HIP_CHECK(hipMemcpyAsync(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, st));
+    HVVR_HIP_CHECK(hipStreamSynchronize(st));
     if (counters[1])
         return fail("tile traversal exceeded its stack or triangle-list budget", HVVR_GPU_ERR_OVERFLOW);
     return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_set_own_stream(hvvr_gpu_view* v, int enable) {
+    if (!v)
+        return fail("null view");
+    if (enable && !v->ownStream) {
+        HVVR_HIP_CHECK(hipSetDevice(v->ctx->device));
+        HVVR_HIP_CHECK(hipStreamCreateWithFlags(&v->ownStream, hipStreamNonBlocking));
+        HVVR_HIP_CHECK(hipEventCreateWithFlags(&v->done, hipEventDisableTiming));
+    } else if (!enable && v->ownStream) {
+        HVVR_HIP_CHECK(hipStreamSynchronize(v->ownStream));
+        (void)hipEventDestroy(v->done);
+        (void)hipStreamDestroy(v->ownStream);
+        v->done = nullptr;
+        v->ownStream = nullptr;
+        v->lastStream = nullptr;
+    }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_image_async(hvvr_gpu_view* v, uint32_t* image, uint32_t stride) {
+    if (!v || !image)
+        return fail("null argument");
+    if (!v->ownStream)
+        return fail("asynchronous reads need the view's own stream", HVVR_GPU_ERR_STATE);
+    if (!v->rendered)
+        return fail("view has not been rendered", HVVR_GPU_ERR_STATE);
+    if (stride < v->width)
+        return fail("stride smaller than image width");
+    HVVR_HIP_CHECK(hipMemcpy2DAsync(image, size_t(stride) * 4, v->image.ptr, size_t(v->width) * 4,
+                                    size_t(v->width) * 4, v->height, hipMemcpyDeviceToHost, v->ownStream));
+    HVVR_HIP_CHECK(hipEventRecord(v->done, v->ownStream));
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_query(hvvr_gpu_view* v) {
+    if (!v)
+        return fail("null view");
+    if (!v->ownStream)
+        return fail("query needs the view's own stream", HVVR_GPU_ERR_STATE);
+    hipError_t e = hipEventQuery(v->done);
+    if (e == hipSuccess)
+        return 1;
+    if (e == hipErrorNotReady)
+        return 0;
+    return fail(std::string("hipEventQuery: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+}
+
+int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_foveated_desc* f, uint64_t* bytes) {
+    if (!d || !bytes)
+        return fail("null argument");
+    if (d->sampleCount == 0 || d->sampleCount % BLOCK_SIZE != 0 || !validRate(d->msaaRate))
+        return fail("invalid view description");
+    // the allocations of hvvr_gpu_view_create and hvvr_gpu_view_set_foveated
+    const uint64_t S = d->sampleCount, T = S / TILE_SIZE, B = S / BLOCK_SIZE, A = d->msaaRate;
+    const uint64_t pixels = uint64_t(d->width) * d->height;
+    const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
+    uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
+                 T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
+                 B * 4 + uint64_t(TILE_STACK_SIZE) * T * sizeof(StackEntry) + 4 * T * 4 + listCapacity * 4 +
+                 S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 8 * 8;
+    if (d->remap)
+        b += pixels * d->split * 4;
+    if (d->lensRadius > 0.0f)
+        b += 4ull * TILE_SIZE * A * sizeof(vector2) + T * 4; // lens table rows, occupied-tile ranks
+    if (f) {
+        const uint64_t polar = uint64_t(f->samplesPerRing) * f->ringCount;
+        b += uint64_t(d->validSampleCount) * sizeof(vector2ui) + polar * (4 + 4 + 8 + 8) + 4 * pixels * 4 + 256 * 4 +
+             256 + S * 4;
+    }
+    *bytes = b;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_device_bytes(hvvr_gpu_view* v, uint64_t* bytes) {
+    if (!v || !bytes)
+        return fail("null argument");
+    uint64_t b = v->beams.bytes() + v->tileFrustaCam.bytes() + v->blockFrustaCam.bytes() +
+                 v->tileFrustaWorld.bytes() + v->tilePre.bytes() + v->blockPre.bytes() + v->blockStacks.bytes() +
+                 v->blockStackSize.bytes() + v->tileStackScratch.bytes() + v->overflowTiles.bytes() +
+                 v->largeQueue.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
+                 v->gbuffer.bytes() + v->sampleResults.bytes() + v->remap.bytes() + v->image.bytes() +
+                 v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes();
+    const FoveatedState& f = v->fov;
+    b += f.polarRemap.bytes() + f.polarColor.bytes() + f.polarDepth.bytes() + f.moment1.bytes() +
+         f.moment2.bytes() + f.history[0].bytes() + f.history[1].bytes() + f.filtered[0].bytes() +
+         f.filtered[1].bytes() + f.srgbToLinear.bytes() + f.srgbRoundTrip.bytes();
+    *bytes = b;
+    return HVVR_GPU_OK;
+}
+
+void* hvvr_gpu_host_alloc(uint64_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        setLastError("hipHostMalloc failed");
+        return nullptr;
+    }
+    return p;
+}
+
+void hvvr_gpu_host_free(void* p) {
+    if (p)
+        (void)hipHostFree(p);
 }
 
 int hvvr_gpu_view_read_image(hvvr_gpu_view* v, uint32_t* image, uint32_t stride) {

@@ -111,6 +111,11 @@ struct hvvr_gpu_scene {
     hvvr::LightingEnvironment lighting = {};
     bool dirty = true;
     bool everUpdated = false;
+    hipEvent_t ready = nullptr; // recorded on the context stream after each update
+    ~hvvr_gpu_scene() {
+        if (ready)
+            (void)hipEventDestroy(ready);
+    }
 };
 
 struct hvvr_gpu_view {
@@ -156,10 +161,19 @@ struct hvvr_gpu_view {
     hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
     bool stagesRecorded = false;
     bool rendered = false;
+    // own-stream mode (hvvr_gpu_view_set_own_stream): renders and reads go to `ownStream`,
+    // after the scene's `ready` event; `done` marks the end of the last queued work
+    hipStream_t ownStream = nullptr;
+    hipEvent_t done = nullptr;
+    hipStream_t lastStream = nullptr; // the stream of the last render
     ~hvvr_gpu_view() {
         for (auto& ev : stageEvents)
             if (ev)
                 (void)hipEventDestroy(ev);
+        if (done)
+            (void)hipEventDestroy(done);
+        if (ownStream)
+            (void)hipStreamDestroy(ownStream);
     }
 };
 

@@ -32,22 +32,21 @@ struct hvvr_host_scene {
     uint32_t modelCount = 0;
 };
 
-struct hvvr_host_view {
-    SampleSetup setup;
-    matrix4x4 cameraToWorld = matrix4x4::identity();
-    ThinLens lens = {0.0f, 1.0f};
-    matrix3x3 sampleToCam = matrix3x3::identity();
-};
 
-struct hvvr_host_foveated {
-    FoveatedSetup setup;
-};
 
 extern "C" {
 
 const char* hvvr_host_last_error(void) {
     return gLastError.c_str();
 }
+
+} // extern "C"
+
+void hvvrSetHostError(const std::string& msg) {
+    gLastError = msg;
+}
+
+extern "C" {
 
 static int finishScene(std::vector<MeshData>& meshes, const std::vector<matrix4x4>& xforms,
                        const std::vector<LightUnion>& lights, hvvr_host_scene** out) {

@@ -160,3 +160,18 @@ CameraPreset heightfieldCamera();
 CameraPreset roomEyeCamera(uint32_t eye, uint32_t width, uint32_t height);
 
 } // namespace hvvr
+
+// sets the message hvvr_host_last_error() returns on this thread
+void hvvrSetHostError(const std::string& msg);
+
+// the host C API's handles (include/hvvr_host.h), shared with the serving loop
+struct hvvr_host_view {
+    hvvr::SampleSetup setup;
+    hvvr::matrix4x4 cameraToWorld = hvvr::matrix4x4::identity();
+    hvvr::ThinLens lens = {0.0f, 1.0f};
+    hvvr::matrix3x3 sampleToCam = hvvr::matrix3x3::identity();
+};
+
+struct hvvr_host_foveated {
+    hvvr::FoveatedSetup setup;
+};

@@ -173,6 +173,25 @@ enum {
 /* Synchronous read of a foveated intermediate of the last frame; bytes must match. */
 int hvvr_gpu_view_read_foveated(hvvr_gpu_view* view, int buffer, void* dst, uint64_t bytes);
 
+/* ---- overlapping views (the serving loop) ----
+ * With its own stream a view renders on a private non-blocking HIP stream, ordered after
+ * the scene's last update, so several views' frames overlap on the device. Image reads
+ * can then be queued behind the frame and polled. */
+int hvvr_gpu_view_set_own_stream(hvvr_gpu_view* view, int enable);
+/* Queue a copy of the last frame's image to host memory (pinned memory from
+ * hvvr_gpu_host_alloc for a truly asynchronous copy); needs the view's own stream. */
+int hvvr_gpu_view_read_image_async(hvvr_gpu_view* view, uint32_t* image, uint32_t strideInPixels);
+/* 1 when everything queued on the view's own stream up to its last
+ * hvvr_gpu_view_read_image_async has finished, 0 when not yet, < 0 on error. */
+int hvvr_gpu_view_query(hvvr_gpu_view* view);
+/* Device memory the view holds, in bytes; and what a view of this description (and
+ * foveated description, or NULL) would hold, before creating it. */
+int hvvr_gpu_view_device_bytes(hvvr_gpu_view* view, uint64_t* bytes);
+int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* desc, const hvvr_gpu_foveated_desc* fov, uint64_t* bytes);
+/* Page-locked host memory (NULL on failure). */
+void* hvvr_gpu_host_alloc(uint64_t bytes);
+void hvvr_gpu_host_free(void* ptr);
+
 /* Synchronous reads of the last rendered frame. */
 int hvvr_gpu_view_read_image(hvvr_gpu_view* view, uint32_t* image, uint32_t strideInPixels);
 int hvvr_gpu_view_read_sample_colors(hvvr_gpu_view* view, uint32_t* colors); /* [sampleCount] RGBA8 */
