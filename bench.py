@@ -140,6 +140,99 @@ def roofline(stage_ms, survivors, entries, visits):
                     f"{int(entries)} list entries per launch"}
 
 
+def serving_bench(args):
+    """Serving mix: Poisson arrivals of view requests over resident scenes of 100k, 262k, 1M
+    and 2M triangles; each request picks a scene, a resolution (log-uniform over five sizes
+    from 256x256 to 1080x1200), an MSAA rate (1/2/4/8) and, for half of them, polar
+    foveation (the feb2017 pattern reconstructed at that size). Admission by a device
+    budget, FP16 shading. Reports throughput and submit-to-image latency."""
+    rng = np.random.default_rng(args.seed)
+    sizes = [(256, 256), (360, 400), (540, 600), (760, 840), (1080, 1200)]
+    rates = [1, 2, 4, 8]
+    t0 = time.perf_counter()
+    scenes = [H.HostScene(H.SCENE_ROOM, k, n) for k, n in enumerate((391, 1024, 4096, 8192))]
+    screens = [H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, w, h) for w, h in sizes]
+    fov = H.HostFoveated(H.HostFoveated.FEB2017)
+    log(f"serving setup: scenes {[len(s.triShade) for s in scenes]}, {time.perf_counter() - t0:.1f} s")
+    ctx = H.GpuContext(0)
+    gscenes = [H.GpuScene(ctx, s) for s in scenes]
+    for g in gscenes:
+        g.update()
+    ctx.synchronize()
+    log(f"scenes resident ({time.perf_counter() - t0:.1f} s)")
+    srv = H.Server(ctx, int(args.budget_gib * (1 << 30)), args.slots, shading_bits=16)
+    shape = {}
+    for i, v in enumerate(screens):
+        for a in rates:
+            shape[(i, a, False)] = srv.add_shape(v, a)
+            shape[(i, a, True)] = srv.add_foveated_shape(fov, v, a)
+    log(f"{len(shape)} shapes ({time.perf_counter() - t0:.1f} s)")
+    n = args.requests
+    gap = rng.exponential(1.0 / args.rate, n)
+    arrival = np.cumsum(gap)
+    # log-uniform resolution over the five sizes: index uniform in log area
+    areas = np.log([w * h for w, h in sizes])
+    pick = rng.uniform(areas[0], areas[-1], n)
+    size_idx = np.abs(pick[:, None] - areas[None, :]).argmin(1)
+    reqs = [(int(rng.integers(4)), int(size_idx[k]), rates[int(rng.integers(4))], bool(rng.random() < 0.5))
+            for k in range(n)]
+    cams = [screens[0].camera_to_world for _ in range(n)]
+
+    def submit(k):
+        sc, si, a, fv = reqs[k]
+        eye = None
+        if fv:
+            y, p = np.radians(rng.uniform(-20, 20, 2))
+            eye = np.array([np.sin(y) * np.cos(p), np.sin(p), -np.cos(y) * np.cos(p)], np.float32)
+        return srv.submit(gscenes[sc], shape[(si, a, fv)], cams[k], eye)
+
+    # warm-up: one request per shape creates the first views
+    for key in sorted(shape):
+        srv.submit(gscenes[0], shape[key], cams[0], None)
+        srv.drain()
+    warm = srv.stats()
+    log(f"warm-up: {warm.viewsCreated} views, {warm.peakDeviceBytes / (1 << 30):.2f} GiB "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t_start = time.perf_counter()
+    k = 0
+    next_log = 1.0
+    while k < n:
+        now = time.perf_counter() - t_start
+        while k < n and arrival[k] <= now:
+            submit(k)
+            k += 1
+        srv.poll()
+        if now > next_log:
+            st = srv.stats()
+            log(f"t={now:.1f} s: submitted {k}, completed {st.completed - warm.completed}, in flight "
+                f"{st.inFlight}, queued {st.queued}")
+            next_log += 1.0
+    srv.drain()
+    elapsed = time.perf_counter() - t_start
+    st = srv.stats()
+    lat = []
+    done = st.completed - warm.completed
+    subsamples = 0
+    for sc, si, a, fv in reqs:
+        w, h = sizes[si]
+        subsamples += (fov.valid_sample_count if fv else w * h) * a
+    out = {
+        "metric": "serving mix: requests/s and submit-to-image latency (ms)",
+        "value": round(done / elapsed, 2), "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+        "requests": int(done), "offered_rate": args.rate, "elapsed_s": round(elapsed, 3),
+        "msamples_per_s": round(subsamples / elapsed / 1e6, 1),
+        "p50_ms": round(st.p50Ms, 3), "p99_ms": round(st.p99Ms, 3), "mean_ms": round(st.meanMs, 3),
+        "peak_in_flight": st.peakInFlight, "views_created": st.viewsCreated, "views_evicted": st.viewsEvicted,
+        "peak_device_gib": round(st.peakDeviceBytes / (1 << 30), 2), "dtype": "f32 intersect, f16 shade",
+        "data": "synthetic: seeded room scenes of 100k/262k/1M/2M triangles",
+        "config": {"workload": "serving mix", "slots": args.slots, "budget_gib": args.budget_gib,
+                   "sizes": sizes, "msaa": rates, "foveated_fraction": 0.5},
+    }
+    del lat
+    srv.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +246,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
                          "workload for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--workload", choices=["stereo", "serving"], default="stereo",
+                    help="stereo: the headline stereo frame; serving: the request-serving mix")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: requests in the timed run")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=8, help="serving: requests on the device at once")
+    ap.add_argument("--budget-gib", type=float, default=64.0, help="serving: device budget for views")
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -162,6 +262,9 @@ def main():
                                    "(eye / request) with no per-frame exchange, so N>1 is N independent "
                                    "copies; run with --gpus 1"}))
         sys.exit(2)
+    if args.workload == "serving":
+        serving_bench(args)
+        return
 
     t0 = time.perf_counter()
     scene = H.HostScene(H.SCENE_ROOM, 0, MESHES)

@@ -339,7 +339,8 @@ class _ViewDesc(ctypes.Structure):
 class ViewStats(ctypes.Structure):
     _fields_ = [("listEntries", c_uint64), ("survivorPairs", c_uint64), ("occupiedTiles", c_uint32),
                 ("tileCount", c_uint32), ("maxTileStack", c_uint32), ("overflow", c_uint32),
-                ("nodeVisits", c_uint64)]
+                ("nodeVisits", c_uint64), ("largeTiles", c_uint32), ("hugeTiles", c_uint32),
+                ("walkTiles", c_uint32)]
 
 
 class GpuContext:
@@ -473,6 +474,15 @@ class GpuView:
     def set_shading_precision(self, bits):
         _check_gpu(self.lib.hvvr_gpu_view_set_shading_precision(self.handle, c_int(bits)))
 
+    def list_demand(self):
+        """(entries the last frame's triangle lists needed, entries the view holds, overflowed)"""
+        need, cap, of = c_uint64(), c_uint64(), c_int()
+        _check_gpu(self.lib.hvvr_gpu_view_list_demand(self.handle, byref(need), byref(cap), byref(of)))
+        return need.value, cap.value, bool(of.value)
+
+    def reserve_lists(self, entries):
+        _check_gpu(self.lib.hvvr_gpu_view_reserve_lists(self.handle, c_uint64(entries)))
+
     def set_jitter_spin(self, enable):
         _check_gpu(self.lib.hvvr_gpu_view_set_jitter_spin(self.handle, c_int(1 if enable else 0)))
 
@@ -573,26 +583,27 @@ class GpuFoveatedView(GpuView):
 # serving loop (include/hvvr_server.h)
 # ---------------------------------------------------------------------------------------
 class _ServerDesc(ctypes.Structure):
-    _fields_ = [("deviceBudgetBytes", c_uint64), ("maxInFlight", c_uint32)]
+    _fields_ = [("deviceBudgetBytes", c_uint64), ("maxInFlight", c_uint32), ("shadingPrecision", c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("queued", c_uint64), ("inFlight", c_uint64),
                 ("deviceBytes", c_uint64), ("peakDeviceBytes", c_uint64), ("viewsCreated", c_uint32),
                 ("viewsEvicted", c_uint32), ("peakInFlight", c_uint32), ("p50Ms", ctypes.c_double),
-                ("p99Ms", ctypes.c_double), ("meanMs", ctypes.c_double)]
+                ("p99Ms", ctypes.c_double), ("meanMs", ctypes.c_double), ("failed", c_uint64),
+                ("listGrowths", c_uint32)]
 
 
 class Server:
     """Frame-request serving loop: resident scenes, view shapes admitted by HBM footprint,
     requests overlapping on the device (hvvr_server_*)."""
 
-    def __init__(self, ctx: GpuContext, budget_bytes, max_in_flight=8):
+    def __init__(self, ctx: GpuContext, budget_bytes, max_in_flight=8, shading_bits=32):
         self.ctx = ctx
         self.lib = load_host()
         self.lib.hvvr_server_create.argtypes = [c_void_p, c_void_p, c_void_p]
         self.handle = c_void_p()
-        d = _ServerDesc(int(budget_bytes), int(max_in_flight))
+        d = _ServerDesc(int(budget_bytes), int(max_in_flight), int(shading_bits))
         _check_host(self.lib.hvvr_server_create(ctx.handle, byref(d), byref(self.handle)))
         self.shapes = []
         self._images = {}

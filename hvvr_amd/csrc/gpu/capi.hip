@@ -399,10 +399,8 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->blockPre.resize(v->blockCount));
     step(v->blockStacks.resize(size_t(v->blockCount) * BLOCK_STACK_SIZE));
     step(v->blockStackSize.resize(v->blockCount));
-    step(v->tileStackScratch.resize(size_t(TILE_STACK_SIZE) * v->tileCount));
     step(v->tileCounts.resize(v->tileCount));
-    step(v->overflowTiles.resize(v->tileCount));
-    step(v->largeQueue.resize(v->tileCount));
+    step(v->tileQueues.resize(size_t(3) * v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
@@ -486,7 +484,6 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
             (void)hipEventRecord(v->stageEvents[i], st);
     };
     HVVR_HIP_CHECK(hipMemsetAsync(v->counters.ptr, 0, v->counters.bytes(), st));
-    HVVR_HIP_CHECK(hipMemsetAsync(v->overflowTiles.ptr, 0, v->overflowTiles.bytes(), st));
     mark(0);
     launchFrustumSetup(v, m, st);
     mark(1);
@@ -793,6 +790,66 @@ int hvvr_gpu_view_query(hvvr_gpu_view* v) {
     return fail(std::string("hipEventQuery: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
 }
 
+// waits for the view's last frame (its own stream and the context's)
+static int viewWait(hvvr_gpu_view* v) {
+    hipStream_t st = v->lastStream ? v->lastStream : v->ctx->stream;
+    HVVR_HIP_CHECK(hipStreamSynchronize(st));
+    if (st != v->ctx->stream)
+        HVVR_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_list_demand(hvvr_gpu_view* v, uint64_t* entries, uint64_t* capacity, int* overflowed) {
+    if (!v || !entries || !capacity || !overflowed)
+        return fail("null argument");
+    if (!v->rendered)
+        return fail("view has not been rendered", HVVR_GPU_ERR_STATE);
+    int rc = viewWait(v);
+    if (rc)
+        return rc;
+    uint64_t counters[4];
+    HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+    *entries = counters[3];
+    *capacity = v->listCapacity;
+    *overflowed = counters[1] ? 1 : 0;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_reserve_lists(hvvr_gpu_view* v, uint64_t entries) {
+    if (!v)
+        return fail("null view");
+    if (entries > 0xffffffffull)
+        return fail("list storage is indexed by 32-bit offsets: at most 2^32 - 1 entries");
+    if (entries <= v->listCapacity)
+        return HVVR_GPU_OK;
+    if (v->rendered) {
+        int rc = viewWait(v);
+        if (rc)
+            return rc;
+    }
+    HVVR_HIP_CHECK(hipSetDevice(v->ctx->device));
+    hipError_t e = v->triIndices.resize(entries);
+    if (e != hipSuccess) {
+        // keep a usable view: back to the capacity it had
+        (void)v->triIndices.resize(v->listCapacity);
+        return fail(std::string("list storage allocation failed: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+    }
+    v->listCapacity = entries;
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_foveated_rewind(hvvr_gpu_view* v) {
+    if (!v)
+        return fail("null view");
+    if (!v->foveated || !v->rendered)
+        return fail("view has no foveated frame to rewind", HVVR_GPU_ERR_STATE);
+    int rc = viewWait(v);
+    if (rc)
+        return rc;
+    v->fov.historyIndex ^= 1u;
+    return HVVR_GPU_OK;
+}
+
 int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_foveated_desc* f, uint64_t* bytes) {
     if (!d || !bytes)
         return fail("null argument");
@@ -804,7 +861,7 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + uint64_t(TILE_STACK_SIZE) * T * sizeof(StackEntry) + 4 * T * 4 + listCapacity * 4 +
+                 B * 4 + 5 * T * 4 + listCapacity * 4 +
                  S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 8 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
@@ -824,8 +881,7 @@ int hvvr_gpu_view_device_bytes(hvvr_gpu_view* v, uint64_t* bytes) {
         return fail("null argument");
     uint64_t b = v->beams.bytes() + v->tileFrustaCam.bytes() + v->blockFrustaCam.bytes() +
                  v->tileFrustaWorld.bytes() + v->tilePre.bytes() + v->blockPre.bytes() + v->blockStacks.bytes() +
-                 v->blockStackSize.bytes() + v->tileStackScratch.bytes() + v->overflowTiles.bytes() +
-                 v->largeQueue.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
+                 v->blockStackSize.bytes() + v->tileQueues.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
                  v->gbuffer.bytes() + v->sampleResults.bytes() + v->remap.bytes() + v->image.bytes() +
                  v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes();
     const FoveatedState& f = v->fov;
@@ -966,6 +1022,9 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     st->overflow = counters[1] ? 1u : 0u;
     st->maxTileStack = uint32_t(counters[2]);
     st->nodeVisits = counters[4];
+    st->largeTiles = uint32_t(counters[5]);
+    st->hugeTiles = uint32_t(counters[6]);
+    st->walkTiles = uint32_t(counters[7]);
     st->tileCount = v->tileCount;
     for (uint32_t c : counts)
         st->occupiedTiles += c ? 1u : 0u;

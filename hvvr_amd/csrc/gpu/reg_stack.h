@@ -139,4 +139,87 @@ __device__ inline void stackInsert(RegStack4& st, float key, uint32_t meta, uint
     st.size++;
 }
 
+// ---------------------------------------------------------------------------------
+// LDS variant for the deepest stacks: position q at t[q] / m[q] of two LDS arrays owned
+// by the caller (capacity = their length, checked by the caller), bottom-up as above.
+// An insert searches the top 64 positions with one ballot per chunk, going down until a
+// chunk holds an entry whose key is not smaller, then the wave moves the entries above
+// the insertion point up by one, 64 at a time, highest chunk first.
+// ---------------------------------------------------------------------------------
+struct LdsStack {
+    float* t;
+    uint32_t* m;
+    uint32_t size;
+};
+
+__device__ inline void stackBindLds(LdsStack& st, float* t, uint32_t* m) {
+    st.t = t;
+    st.m = m;
+    st.size = 0;
+}
+
+// positions 0..size-1 from lanes 0..size-1 of (t, m); size <= 64
+__device__ inline void stackInit(LdsStack& st, float t, uint32_t m, uint32_t size, uint32_t lane) {
+    if (lane < size) {
+        st.t[lane] = t;
+        st.m[lane] = m;
+    }
+    waveLdsSync();
+    st.size = size;
+}
+
+__device__ inline uint32_t stackPop(LdsStack& st) {
+    st.size--;
+    return uniform(st.m[st.size]);
+}
+
+__device__ inline float stackTopKey(const LdsStack& st) {
+    return bitsf(uniform(fbits(st.t[st.size - 1u])));
+}
+
+// entry at position q (q < size)
+__device__ inline uint32_t stackAt(const LdsStack& st, uint32_t q) {
+    return uniform(st.m[q]);
+}
+
+// Inserts (key, meta) one above the highest entry whose key is not smaller (position 0
+// when there is none); the caller keeps size below the arrays' length.
+__device__ inline void stackInsert(LdsStack& st, float key, uint32_t meta, uint32_t lane) {
+    int32_t p = 0;
+    for (int32_t base = int32_t(st.size) - 64;; base -= 64) {
+        const int32_t q = base + int32_t(lane);
+        const bool in = q >= 0;
+        const float t = st.t[in ? q : 0];
+        const uint64_t b = ballot(in && !(t < key));
+        if (b) {
+            p = base + 64 - int32_t(__builtin_clzll(b));
+            break;
+        }
+        if (base <= 0)
+            break;
+    }
+    for (int32_t base = int32_t(st.size) - 64; base + 64 > p; base -= 64) {
+        const int32_t q = base + int32_t(lane);
+        const bool move = q >= p;
+        float t = 0.0f;
+        uint32_t m = 0u;
+        if (move) {
+            t = st.t[q];
+            m = st.m[q];
+        }
+        waveLdsSync(); // every lane has read its entry before any lane overwrites one
+        if (move) {
+            st.t[q + 1] = t;
+            st.m[q + 1] = m;
+        }
+        waveLdsSync();
+    }
+    if (lane == 0) {
+        st.t[p] = key;
+        st.m[p] = meta;
+    }
+    waveLdsSync();
+    st.size++;
+}
+
 } // namespace hvvr

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
 // queue with a few hundred persistent workgroups. Only what neither holds goes to the
 // one-thread-per-tile global-stack kernel.
 // ---------------------------------------------------------------------------------
-// Record meta: first child record (bits 0..11), child mask (20..23), leaf mask (24..27),
+// Record meta: first child record (bits 0..19), child mask (20..23), leaf mask (24..27),
 // stack-entry leaf mask (28..31).
 __device__ inline uint32_t metaMask(uint32_t m) { return (m >> 20) & 0xfu; }
 __device__ inline uint32_t metaLeaf(uint32_t m) { return (m >> 24) & 0xfu; }
@@ -230,14 +230,28 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 }
 
 // The standard configuration keeps the record index in bits 10..19 of a stack entry's
-// meta (records < 1024), so a pop needs no LDS read; the large one stacks record indices.
+// meta (records < 1024), so a pop needs no LDS read; the others stack record indices.
+// The huge one keeps its stack in LDS beside the records (the stack store).
+enum : uint32_t { LDS_STACK_CAP = 2048 };
+struct NoStackStore {};
+struct LdsStackStore {
+    float t[LDS_STACK_CAP];
+    uint32_t m[LDS_STACK_CAP];
+};
 struct TileListSmall {
     enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1 };
     using Stack = RegStack;
+    using StackStore = NoStackStore;
 };
 struct TileListLarge {
     enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0 };
     using Stack = RegStack4;
+    using StackStore = NoStackStore;
+};
+struct TileListHuge {
+    enum : uint32_t { RecCap = 8192, StackCap = LDS_STACK_CAP, Packed = 0 };
+    using Stack = LdsStack;
+    using StackStore = LdsStackStore;
 };
 
 template <class Cfg>
@@ -248,11 +262,12 @@ struct TileListLds {
     uint32_t recMeta[Cfg::RecCap]; // packMeta(first child record, child mask, leaf mask,
                                    // stack-entry leaf mask)
     uint16_t order[Cfg::RecCap];   // records with emitted leaves, in emission order
+    typename Cfg::StackStore stack;
 };
 
 template <class Cfg>
 __device__ inline uint32_t metaFirst(uint32_t m) {
-    return Cfg::Packed ? (m & 0x3ffu) : (m & 0xfffu);
+    return Cfg::Packed ? (m & 0x3ffu) : (m & 0xfffffu);
 }
 // stack entry of record r with meta m, and back
 template <class Cfg>
@@ -270,12 +285,16 @@ __device__ inline void fromStackEntry(uint32_t e, const TileListLds<Cfg>& sh, ui
     }
 }
 
-__device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size) {
+__device__ inline void stackBind(RegStack&, NoStackStore&) {}
+__device__ inline void stackBind(RegStack4&, NoStackStore&) {}
+__device__ inline void stackBind(LdsStack& st, LdsStackStore& s) { stackBindLds(st, s.t, s.m); }
+
+__device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size, uint32_t) {
     st.t = t;
     st.m = m;
     st.size = size;
 }
-__device__ inline void stackInit(RegStack4& st, float t, uint32_t m, uint32_t size) {
+__device__ inline void stackInit(RegStack4& st, float t, uint32_t m, uint32_t size, uint32_t) {
     st.t0 = t;
     st.m0 = m;
     st.t1 = st.t2 = st.t3 = 0.0f;
@@ -329,8 +348,10 @@ struct TileListArgs {
     uint32_t* offsets;
     uint32_t* triOut;
     uint64_t capacity;
-    uint32_t* overflowTiles; // 1 = left for the global-stack kernel
-    uint32_t* largeQueue;    // tiles the standard configuration could not hold
+    const uint32_t* queue;   // the tiles this launch takes (queued kernels)
+    uint32_t* nextQueue;     // where a tile this configuration cannot hold goes
+    uint32_t queueCounter;   // counters[] slot holding the length of `queue`
+    uint32_t nextCounter;    // and of `nextQueue`
     unsigned long long* counters;
     uint32_t stackLimit;
 };
@@ -440,6 +461,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
     // ---- phase B: the reference's stack walk over the live records, stack in VGPRs ----
     if (!overflow) {
         Stack st;
+        stackBind(st, sh.stack);
         {
             // the block stack's live entries, compacted in order (entry q at position q)
             const bool valid = lane < n0;
@@ -451,7 +473,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             const float t = bitsf(uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(key))));
             const uint32_t m =
                 uint32_t(__builtin_amdgcn_ds_permute(int(dest << 2), int(stackEntry<Cfg>(lane, meta))));
-            stackInit(st, t, m, size);
+            stackInit(st, t, m, size, lane);
             const float above = __shfl_down(t, 1);
             if (ballot(lane + 1u < size && t < above) != 0ull)
                 overflow = true;
@@ -607,153 +629,165 @@ __global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
     const uint32_t tile = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     if (!tileListBody<TileListSmall>(g, sh, tile, lane) && lane == 0) {
-        g.overflowTiles[tile] = 1u;
-        const unsigned long long slot = atomicAdd(&g.counters[5], 1ull);
-        g.largeQueue[slot] = tile;
+        const unsigned long long slot = atomicAdd(&g.counters[g.nextCounter], 1ull);
+        g.nextQueue[slot] = tile;
     }
 }
 
-// the queued tiles, large configuration, persistent workgroups; what it cannot hold stays
-// flagged for the global-stack kernel
-__global__ __launch_bounds__(64) void tileListLargeKernel(TileListArgs g) {
-    __shared__ TileListLds<TileListLarge> sh;
+// the tiles a smaller configuration queued, persistent workgroups; a tile this one cannot
+// hold either is queued for the next
+template <class Cfg>
+__global__ __launch_bounds__(64) void tileListQueueKernel(TileListArgs g) {
+    __shared__ TileListLds<Cfg> sh;
     const uint32_t lane = threadIdx.x;
-    const uint32_t queued = uniform(uint32_t(g.counters[5]));
+    const uint32_t queued = uniform(uint32_t(g.counters[g.queueCounter]));
     for (uint32_t i = blockIdx.x; i < queued; i += gridDim.x) {
-        const uint32_t tile = uniform(g.largeQueue[i]);
+        const uint32_t tile = uniform(g.queue[i]);
         waveLdsSync(); // the previous tile's LDS reads are done
-        if (tileListBody<TileListLarge>(g, sh, tile, lane) && lane == 0)
-            g.overflowTiles[tile] = 0u;
+        if (!tileListBody<Cfg>(g, sh, tile, lane) && lane == 0) {
+            const unsigned long long slot = atomicAdd(&g.counters[g.nextCounter], 1ull);
+            g.nextQueue[slot] = tile;
+        }
     }
 }
 
-// traverseTiles (traverse_avx.h:412-527)
-__global__ __launch_bounds__(64) void tileTraverseKernel(const BVHNode* __restrict__ nodes,
-                                                         const FrustumPre* __restrict__ tilePre, uint32_t tileCount,
-                                                         const StackEntry* __restrict__ blockStacks,
-                                                         const uint32_t* __restrict__ blockStackSize,
-                                                         StackEntry* __restrict__ scratch, uint32_t* __restrict__ counts,
-                                                         const uint32_t* __restrict__ offsets,
-                                                         uint32_t* __restrict__ triOut, uint64_t capacity,
-                                                         const uint32_t* __restrict__ onlyTiles,
-                                                         unsigned long long* __restrict__ counters, int emit) {
-    uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= tileCount)
-        return;
-    if (onlyTiles && !onlyTiles[tile])
-        return;
-    uint32_t block = tile / TILES_PER_BLOCK;
-    uint32_t n0 = blockStackSize[block];
-    const FrustumPre& f = tilePre[tile];
-    if (n0 == 0 || f.degenerate) {
-        if (!emit)
-            counts[tile] = 0;
-        return;
-    }
-    auto S = [&](uint32_t slot) -> StackEntry& { return scratch[size_t(slot) * tileCount + tile]; };
-    const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
-    for (uint32_t i = 0; i < n0; i++)
-        S(i) = bs[i];
-    uint64_t outBase = emit ? offsets[tile] : 0;
-    uint32_t count = 0;
-    uint32_t top = n0 - 1;
-    uint32_t maxTop = top;
-    uint32_t node = S(top).nodeMask & STACK_NODE_MASK;
-    uint32_t entryMask = S(top).nodeMask >> 28;
-    bool overflow = false;
-    int axis = f.distanceAxis;
-    uint32_t visits = 0;
-    for (;;) {
-        const BVHNode& n = nodes[node];
-        visits++;
-        uint32_t mask = testChildren(f, n);
-        bool pop = true;
-        if (mask) {
-            uint32_t leaf = mask & n.leafMask;
-            if (leaf) {
-                mask = testChildrenRefine(f, n, mask);
-                leaf = mask & n.leafMask;
-                for (int k = 0; k < 4; k++) {
-                    if (!(leaf & (1u << k)))
-                        continue;
-                    uint32_t t0 = n.data[k], t1 = n.data[k + 1];
-                    if (emit) {
-                        for (uint32_t t = t0; t < t1; t++) {
-                            uint64_t at = outBase + count + (t - t0);
-                            if (at < capacity)
-                                triOut[at] = t;
-                            else
-                                overflow = true;
-                        }
-                    }
-                    count += t1 > t0 ? t1 - t0 : 0;
-                }
+// ---------------------------------------------------------------------------------
+// The reference's tile walk as written (traverseTiles, traverse_avx.h:412-527), one wave
+// per tile, for the tiles no LDS configuration holds and those with a NaN key or an
+// unsorted block stack. The walk is wave-uniform (one node per step, tested by every
+// lane alike); the distance-sorted stack, up to TILE_STACK_SIZE entries, lives in LDS and
+// the wave inserts into it together (LdsStack: the sequential insertion's position, one
+// ballot per 64 entries, then a 64-wide move). The current node's entry stays on top
+// while it is tested, as in the reference: a node with several passing internal
+// children replaces its entry by them, each at its sorted position; a single one is
+// visited directly. Count pass (emit = 0) and emit pass (emit = 1) walk alike.
+// ---------------------------------------------------------------------------------
+struct TileWalkLds {
+    float t[TILE_STACK_SIZE];
+    uint32_t m[TILE_STACK_SIZE];
+};
+
+__global__ __launch_bounds__(64) void tileWalkKernel(const BVHNode* __restrict__ nodes,
+                                                     const FrustumPre* __restrict__ tilePre,
+                                                     const StackEntry* __restrict__ blockStacks,
+                                                     const uint32_t* __restrict__ blockStackSize,
+                                                     const uint32_t* __restrict__ queue, uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     uint32_t* __restrict__ triOut, uint64_t capacity,
+                                                     unsigned long long* __restrict__ counters, int emit) {
+    __shared__ TileWalkLds sh;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t queued = uniform(uint32_t(counters[7]));
+    for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
+        const uint32_t tile = uniform(queue[qi]);
+        const uint32_t block = tile / TILES_PER_BLOCK;
+        const uint32_t n0 = uniform(blockStackSize[block]);
+        const FrustumPre& f = tilePre[tile];
+        if (n0 == 0 || f.degenerate) {
+            if (!emit && lane == 0)
+                counts[tile] = 0;
+            continue;
+        }
+        waveLdsSync(); // the previous tile's LDS reads are done
+        LdsStack st;
+        stackBindLds(st, sh.t, sh.m);
+        {
+            const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
+            float t0 = 0.0f;
+            uint32_t m0 = 0u;
+            if (lane < n0) {
+                t0 = bs[lane].tMin;
+                m0 = bs[lane].nodeMask;
             }
-            mask &= ~n.leafMask;
-            if (mask && entryMask == 0) {
-                if ((mask & (mask - 1)) == 0) {
-                    node = node + n.data[1 + __builtin_ctz(mask)];
-                    continue;
+            stackInit(st, t0, m0, n0, lane);
+        }
+        const uint64_t outBase = emit ? offsets[tile] : 0;
+        uint32_t count = 0, visits = 0, maxSize = n0;
+        bool listOverflow = false, stackOverflow = false;
+        const int axis = f.distanceAxis;
+        uint32_t topEntry = stackAt(st, n0 - 1u);
+        uint32_t node = topEntry & STACK_NODE_MASK;
+        uint32_t entryMask = topEntry >> 28;
+        for (;;) {
+            const BVHNode& n = nodes[node];
+            visits++;
+            uint32_t mask = testChildren(f, n);
+            bool pop = true;
+            if (mask) {
+                uint32_t leaf = mask & n.leafMask;
+                if (leaf) {
+                    mask = testChildrenRefine(f, n, mask);
+                    leaf = mask & n.leafMask;
+                    for (int k = 0; k < 4; k++) {
+                        if (!(leaf & (1u << k)))
+                            continue;
+                        const uint32_t t0 = n.data[k], t1 = n.data[k + 1];
+                        if (emit) {
+                            for (uint32_t t = t0 + lane; t < t1; t += 64) {
+                                const uint64_t at = outBase + count + (t - t0);
+                                if (at < capacity)
+                                    triOut[at] = t;
+                                else
+                                    listOverflow = true;
+                            }
+                        }
+                        count += t1 > t0 ? t1 - t0 : 0;
+                    }
                 }
-                bool first = true;
-                for (int k = 0; k < 4; k++) {
-                    if (!(mask & (1u << k)))
+                mask &= ~n.leafMask;
+                if (mask && entryMask == 0) {
+                    if ((mask & (mask - 1)) == 0) {
+                        node = node + n.data[1 + __builtin_ctz(mask)];
                         continue;
-                    if (!first)
-                        top++;
-                    first = false;
-                    if (top >= TILE_STACK_SIZE) {
-                        overflow = true;
-                        top = TILE_STACK_SIZE - 1;
+                    }
+                    const uint32_t nc = uint32_t(__builtin_popcount(mask));
+                    if (st.size - 1u + nc > TILE_STACK_SIZE) {
+                        stackOverflow = true;
                         break;
                     }
-                    float dist = -n.bounds[axis][k];
-                    uint32_t i = top;
-                    while (i > 0 && S(i - 1).tMin < dist) {
-                        S(i) = S(i - 1);
-                        i--;
-                    }
-                    StackEntry e;
-                    e.tMin = dist;
-                    e.nodeMask = (node + n.data[1 + k]) & STACK_NODE_MASK;
-                    S(i) = e;
+                    st.size--; // the children take the current entry's place
+                    for (int k = 0; k < 4; k++)
+                        if (mask & (1u << k))
+                            stackInsert(st, -n.bounds[axis][k], (node + n.data[1 + k]) & STACK_NODE_MASK, lane);
+                    maxSize = st.size > maxSize ? st.size : maxSize;
+                    // the closest entry may be an older one (possibly a block-level leaf entry)
+                    topEntry = stackAt(st, st.size - 1u);
+                    node = topEntry & STACK_NODE_MASK;
+                    entryMask = topEntry >> 28;
+                    pop = false;
                 }
-                if (overflow)
+            }
+            if (pop) {
+                if (st.size == 1u)
                     break;
-                maxTop = top > maxTop ? top : maxTop;
-                // the closest entry may be an older one (possibly a block-level leaf entry)
-                StackEntry e = S(top);
-                node = e.nodeMask & STACK_NODE_MASK;
-                entryMask = e.nodeMask >> 28;
-                pop = false;
+                st.size--;
+                topEntry = stackAt(st, st.size - 1u);
+                node = topEntry & STACK_NODE_MASK;
+                entryMask = topEntry >> 28;
             }
         }
-        if (pop) {
-            if (top == 0)
-                break;
-            top--;
-            StackEntry e = S(top);
-            node = e.nodeMask & STACK_NODE_MASK;
-            entryMask = e.nodeMask >> 28;
+        const bool overflow = stackOverflow || ballot(listOverflow) != 0ull;
+        if (lane == 0) {
+            if (!emit)
+                counts[tile] = count;
+            if (overflow)
+                atomicOr(&counters[1], 1ull);
+            atomicMax(&counters[2], (unsigned long long)maxSize);
+            if (!emit)
+                atomicAdd(&counters[4], (unsigned long long)visits); // node visits
         }
     }
-    if (!emit)
-        counts[tile] = count;
-    if (overflow)
-        atomicOr(&counters[1], 1ull);
-    atomicMax(&counters[2], (unsigned long long)(maxTop + 1));
-    if (!emit)
-        atomicAdd(&counters[4], (unsigned long long)visits); // node visits
 }
 
-// List space for the tiles the global-stack kernel re-traverses, after its count pass.
-__global__ void fallbackAllocKernel(const uint32_t* __restrict__ overflowTiles, const uint32_t* __restrict__ counts,
-                                    uint32_t* __restrict__ offsets, uint32_t tileCount, uint64_t capacity,
-                                    unsigned long long* __restrict__ counters) {
-    uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= tileCount || !overflowTiles[tile])
+// List space for the tiles the walk kernel took, after its count pass.
+__global__ void walkAllocKernel(const uint32_t* __restrict__ queue, const uint32_t* __restrict__ counts,
+                                uint32_t* __restrict__ offsets, uint64_t capacity,
+                                unsigned long long* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= uint32_t(counters[7]))
         return;
-    unsigned long long start = atomicAdd(&counters[3], (unsigned long long)counts[tile]);
+    const uint32_t tile = queue[i];
+    const unsigned long long start = atomicAdd(&counters[3], (unsigned long long)counts[tile]);
     offsets[tile] = uint32_t(start);
     if (start + counts[tile] > capacity)
         atomicOr(&counters[1], 1ull);
@@ -803,6 +837,8 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
 }
 
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
+    uint32_t* queues = v->tileQueues.ptr; // [3][tileCount]: large, huge, walk
+    const uint32_t T = v->tileCount;
     TileListArgs g;
     g.nodes = s->nodes.ptr;
     g.tilePre = v->tilePre.ptr;
@@ -812,31 +848,46 @@ void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stre
     g.offsets = v->tileOffsets.ptr;
     g.triOut = v->triIndices.ptr;
     g.capacity = v->listCapacity;
-    g.overflowTiles = v->overflowTiles.ptr;
-    g.largeQueue = v->largeQueue.ptr;
+    g.queue = nullptr;
+    g.nextQueue = queues;
+    g.queueCounter = 0;
+    g.nextCounter = 5;
     g.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
     g.stackLimit = v->waveStackLimit;
-    tileListKernel<<<v->tileCount, 64, 0, stream>>>(g);
-    // the large configuration's stack limit is the view's limit scaled past the standard
-    // one (a test limit below 64 keeps both paths short)
-    TileListArgs big = g;
-    big.stackLimit = v->waveStackLimit >= REG_STACK_CAP ? REG_STACK4_CAP : v->waveStackLimit;
-    const uint32_t groups = v->tileCount < 512u ? v->tileCount : 512u;
-    tileListLargeKernel<<<groups, 64, 0, stream>>>(big);
+    tileListKernel<<<T, 64, 0, stream>>>(g);
+    // the larger configurations' stack limit is the view's limit scaled past the standard
+    // one (a test limit below 64 keeps every configuration short)
+    const bool full = v->waveStackLimit >= REG_STACK_CAP;
+    TileListArgs large = g;
+    large.stackLimit = full ? REG_STACK4_CAP : v->waveStackLimit;
+    large.queue = queues;
+    large.queueCounter = 5;
+    large.nextQueue = queues + T;
+    large.nextCounter = 6;
+    tileListQueueKernel<TileListLarge><<<T < 512u ? T : 512u, 64, 0, stream>>>(large);
+    // 128 KB of LDS: one workgroup per CU
+    TileListArgs huge = large;
+    huge.stackLimit = full ? LDS_STACK_CAP : v->waveStackLimit;
+    huge.queue = queues + T;
+    huge.queueCounter = 6;
+    huge.nextQueue = queues + 2 * T;
+    huge.nextCounter = 7;
+    tileListQueueKernel<TileListHuge><<<T < 256u ? T : 256u, 64, 0, stream>>>(huge);
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
-    const uint32_t groups = (v->tileCount + 63) / 64;
+    const uint32_t T = v->tileCount;
+    const uint32_t* walkQueue = v->tileQueues.ptr + 2 * size_t(T);
+    const uint32_t groups = T < 1024u ? T : 1024u; // 32 KB of LDS each
     for (int emit = 0; emit < 2; emit++) {
         if (emit)
-            fallbackAllocKernel<<<groups, 64, 0, stream>>>(v->overflowTiles.ptr, v->tileCounts.ptr,
-                                                           v->tileOffsets.ptr, v->tileCount, v->listCapacity,
-                                                           counters);
-        tileTraverseKernel<<<groups, 64, 0, stream>>>(
-            s->nodes.ptr, v->tilePre.ptr, v->tileCount, v->blockStacks.ptr, v->blockStackSize.ptr,
-            v->tileStackScratch.ptr, v->tileCounts.ptr, v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity,
-            v->overflowTiles.ptr, counters, emit);
+            walkAllocKernel<<<(T + 63) / 64, 64, 0, stream>>>(walkQueue, v->tileCounts.ptr, v->tileOffsets.ptr,
+                                                              v->listCapacity, counters);
+        tileWalkKernel<<<groups, 64, 0, stream>>>(s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr,
+                                                  v->blockStackSize.ptr, walkQueue, v->tileCounts.ptr,
+                                                  v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity, counters,
+                                                  emit);
     }
 }
 

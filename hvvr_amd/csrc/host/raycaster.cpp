@@ -16,6 +16,7 @@
 // submission; a camera with a CPU render target gets its image copied back at the end of
 // render() (the reference copies only to 3D-API targets, render.cpp:259-268).
 
+#include <algorithm>
 #include "host_internal.h"
 
 #include "hvvr_gpu.h"
@@ -405,14 +406,22 @@ void Raycaster::render(double elapsedTime) {
 
     if (_spec.mode == RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT) {
         // renderFoveatedPolarSpaceCudaReconstruct (render.cpp:454-488), camera by camera
-        for (Camera* c : active) {
-            hvvr_gpu_foveated_frame fr;
-            c->foveatedFrame(fr);
-            vector2 j = c->nextJitter();
-            check(hvvr_gpu_view_render_foveated(c->_view, _scene, &fr, &j.x), "Raycaster::render");
+        std::vector<hvvr_gpu_foveated_frame> frames(active.size());
+        std::vector<vector2> jitter(active.size());
+        for (size_t i = 0; i < active.size(); i++) {
+            active[i]->foveatedFrame(frames[i]);
+            jitter[i] = active[i]->nextJitter();
+            check(hvvr_gpu_view_render_foveated(active[i]->_view, _scene, &frames[i], &jitter[i].x),
+                  "Raycaster::render");
         }
-        for (Camera* c : active)
-            c->extractImage();
+        for (size_t i = 0; i < active.size(); i++) {
+            if (growLists(active[i]->_view)) {
+                check(hvvr_gpu_view_foveated_rewind(active[i]->_view), "Raycaster::render");
+                check(hvvr_gpu_view_render_foveated(active[i]->_view, _scene, &frames[i], &jitter[i].x),
+                      "Raycaster::render");
+            }
+            active[i]->extractImage();
+        }
         return;
     }
 
@@ -427,8 +436,26 @@ void Raycaster::render(double elapsedTime) {
     check(hvvr_gpu_render_views(_scene, uint32_t(views.size()), views.data(), (const float*)cameraToWorld.data(),
                                 (const float*)jitter.data()),
           "Raycaster::render");
-    for (Camera* c : active)
-        c->extractImage();
+    for (size_t i = 0; i < active.size(); i++) {
+        if (growLists(views[i]))
+            check(hvvr_gpu_view_render(views[i], _scene, (const float*)&cameraToWorld[i], &jitter[i].x),
+                  "Raycaster::render");
+        active[i]->extractImage();
+    }
+}
+
+// After a frame: true when its triangle lists overflowed the view's storage, which has
+// then grown to what the frame needed (plus a quarter), for the caller to render again.
+// (The reference stops with a message instead, traverse_avx.h:463.)
+bool Raycaster::growLists(hvvr_gpu_view* view) {
+    uint64_t need = 0, cap = 0;
+    int overflowed = 0;
+    check(hvvr_gpu_view_list_demand(view, &need, &cap, &overflowed), "Raycaster::render");
+    if (!overflowed || need <= cap)
+        return false;
+    check(hvvr_gpu_view_reserve_lists(view, std::min<uint64_t>(need + need / 4, 0xffffffffull)),
+          "Raycaster::render");
+    return true;
 }
 
 } // namespace hvvr

@@ -50,7 +50,9 @@ struct Shape {
     FoveatedSetup fov;
     matrix3x3 screenSampleToCam = matrix3x3::identity();
     vector2 boundsLower, boundsUpper;
-    uint64_t bytes = 0; // device bytes of one view (hvvr_gpu_view_estimate_bytes)
+    uint64_t bytes = 0;    // device bytes of one new view (hvvr_gpu_view_estimate_bytes, then
+                           // what a view grown to listHint holds)
+    uint64_t listHint = 0; // list entries a view of this shape has needed (0 = the default)
 };
 
 struct PooledView {
@@ -71,6 +73,7 @@ struct Request {
     uint32_t* image;
     double submitted = 0.0, completed = 0.0;
     PooledView* pv = nullptr;
+    uint32_t launches = 0;
 };
 
 } // namespace
@@ -79,6 +82,7 @@ struct hvvr_server {
     hvvr_gpu_context* ctx = nullptr;
     uint64_t budget = 0, used = 0, peakUsed = 0;
     uint32_t maxInFlight = 1, peakInFlight = 0;
+    int shadingBits = 32;
     uint32_t viewsCreated = 0, viewsEvicted = 0;
     uint64_t nextId = 1, submitted = 0;
     std::vector<Shape> shapes;
@@ -86,7 +90,9 @@ struct hvvr_server {
     std::deque<Request*> queue;
     std::vector<Request*> inFlight;
     std::unordered_map<uint64_t, double> done; // request id -> latency (ms)
+    std::unordered_map<uint64_t, std::string> failed; // request id -> why
     std::vector<double> latencies;
+    uint32_t listGrowths = 0;
 
     ~hvvr_server() {
         for (Request* r : inFlight)
@@ -150,6 +156,10 @@ struct hvvr_server {
         if (!rc)
             rc = hvvr_gpu_view_set_own_stream(pv->view, 1);
         if (!rc)
+            rc = hvvr_gpu_view_set_shading_precision(pv->view, shadingBits);
+        if (!rc && sh.listHint)
+            rc = hvvr_gpu_view_reserve_lists(pv->view, sh.listHint);
+        if (!rc)
             rc = hvvr_gpu_view_device_bytes(pv->view, &pv->bytes);
         if (!rc) {
             pv->staging = (uint32_t*)hvvr_gpu_host_alloc(uint64_t(sh.width) * sh.height * 4);
@@ -196,6 +206,7 @@ struct hvvr_server {
     int launch(Request* r) {
         PooledView* pv = r->pv;
         const Shape& sh = shapes[r->shape];
+        r->launches++;
         int rc;
         if (sh.foveated) {
             FoveatedHistory h; // every request is a first frame: no history to reproject
@@ -215,6 +226,27 @@ struct hvvr_server {
         return rc ? gpuFail("render", rc) : HVVR_GPU_OK;
     }
 
+    int growLists(PooledView* pv, uint64_t entries) {
+        Shape& sh = shapes[pv->shape];
+        entries = std::min<uint64_t>(entries, 0xffffffffull);
+        int rc = hvvr_gpu_view_reserve_lists(pv->view, entries);
+        uint64_t bytes = 0;
+        if (!rc)
+            rc = hvvr_gpu_view_device_bytes(pv->view, &bytes);
+        if (rc)
+            return gpuFail("list storage", rc);
+        used = used - pv->bytes + bytes;
+        peakUsed = std::max(peakUsed, used);
+        pv->bytes = bytes;
+        if (entries > sh.listHint) {
+            // views of one shape differ only in list storage: new ones start at this size
+            sh.listHint = entries;
+            sh.bytes = bytes;
+        }
+        listGrowths++;
+        return HVVR_GPU_OK;
+    }
+
     int poll() {
         int completed = 0;
         for (size_t i = 0; i < inFlight.size();) {
@@ -226,14 +258,35 @@ struct hvvr_server {
                 i++;
                 continue;
             }
-            const Shape& sh = shapes[r->shape];
-            memcpy(r->image, r->pv->staging, size_t(sh.width) * sh.height * 4);
+            Shape& sh = shapes[r->shape];
+            uint64_t need = 0, cap = 0;
+            int overflowed = 0;
+            int rc = hvvr_gpu_view_list_demand(r->pv->view, &need, &cap, &overflowed);
+            if (rc)
+                return gpuFail("list demand", rc);
+            if (overflowed && need > cap && r->launches == 1) {
+                // the lists did not fit: grow the view's storage (and that of the shape's
+                // new views) and render the frame again
+                rc = growLists(r->pv, need + need / 4);
+                if (rc)
+                    return rc;
+                rc = launch(r);
+                if (rc)
+                    return rc;
+                i++;
+                continue;
+            }
             r->completed = nowMs();
             r->pv->busy = false;
             r->pv->lastUsed = r->completed;
-            double lat = r->completed - r->submitted;
-            done[r->id] = lat;
-            latencies.push_back(lat);
+            if (overflowed) {
+                failed[r->id] = "the frame overflowed a tile's traversal stack";
+            } else {
+                memcpy(r->image, r->pv->staging, size_t(sh.width) * sh.height * 4);
+                double lat = r->completed - r->submitted;
+                done[r->id] = lat;
+                latencies.push_back(lat);
+            }
             delete r;
             inFlight.erase(inFlight.begin() + long(i));
             completed++;
@@ -268,7 +321,10 @@ extern "C" {
 int hvvr_server_create(hvvr_gpu_context* ctx, const hvvr_server_desc* d, hvvr_server** out) {
     if (!ctx || !d || !out || d->maxInFlight < 1 || d->maxInFlight > 64 || d->deviceBudgetBytes == 0)
         return serverFail("invalid server description");
+    if (d->shadingPrecision != 0 && d->shadingPrecision != 16 && d->shadingPrecision != 32)
+        return serverFail("shading precision must be 16 or 32");
     hvvr_server* s = new hvvr_server();
+    s->shadingBits = d->shadingPrecision ? int(d->shadingPrecision) : 32;
     s->ctx = ctx;
     s->budget = d->deviceBudgetBytes;
     s->maxInFlight = d->maxInFlight;
@@ -397,6 +453,9 @@ int hvvr_server_drain(hvvr_server* s) {
 int hvvr_server_request_done(hvvr_server* s, uint64_t id, double* latencyMs) {
     if (!s)
         return serverFail("null server");
+    auto f = s->failed.find(id);
+    if (f != s->failed.end())
+        return serverFail("request " + std::to_string(id) + " failed: " + f->second, HVVR_GPU_ERR_OVERFLOW);
     auto it = s->done.find(id);
     if (it == s->done.end())
         return 0;
@@ -418,6 +477,8 @@ int hvvr_server_get_stats(hvvr_server* s, hvvr_server_stats* st) {
     st->viewsCreated = s->viewsCreated;
     st->viewsEvicted = s->viewsEvicted;
     st->peakInFlight = s->peakInFlight;
+    st->failed = s->failed.size();
+    st->listGrowths = s->listGrowths;
     if (!s->latencies.empty()) {
         std::vector<double> l = s->latencies;
         std::sort(l.begin(), l.end());

@@ -201,6 +201,7 @@ public:
 
 protected:
     void updateScene();
+    bool growLists(hvvr_gpu_view* view);
     void transformsChanged() { _transformsDirty = true; }
 
     RayCasterSpecification _spec;

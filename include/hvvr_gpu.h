@@ -188,6 +188,21 @@ int hvvr_gpu_view_query(hvvr_gpu_view* view);
  * foveated description, or NULL) would hold, before creating it. */
 int hvvr_gpu_view_device_bytes(hvvr_gpu_view* view, uint64_t* bytes);
 int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* desc, const hvvr_gpu_foveated_desc* fov, uint64_t* bytes);
+/* Triangle-list storage. The reference holds a fixed MAX_TRI_INDICES_TO_INTERSECT
+ * (raycaster/magic_constants.h:19) and stops with a message when a frame needs more; a
+ * view here starts at 1024 entries per tile and can grow. After a frame has finished
+ * (waits for it): *entries = the list entries it needed, *capacity = what the view holds,
+ * *overflowed = 1 when the frame overflowed its lists or a tile's traversal stack (its
+ * reads then fail with HVVR_GPU_ERR_OVERFLOW). *entries > *capacity: reserve *entries and
+ * render the frame again; otherwise the overflow was a traversal stack's, which more list
+ * storage does not cure. */
+int hvvr_gpu_view_list_demand(hvvr_gpu_view* view, uint64_t* entries, uint64_t* capacity, int* overflowed);
+/* Grows the list storage to `entries` (at most 2^32 - 1; never shrinks); waits for the
+ * view's work first. */
+int hvvr_gpu_view_reserve_lists(hvvr_gpu_view* view, uint64_t entries);
+/* Undoes the reconstruction history's advance of the last foveated frame, so that the
+ * same frame can be rendered again (after its lists grew) from the same history. */
+int hvvr_gpu_view_foveated_rewind(hvvr_gpu_view* view);
 /* Page-locked host memory (NULL on failure). */
 void* hvvr_gpu_host_alloc(uint64_t bytes);
 void hvvr_gpu_host_free(void* ptr);
@@ -210,6 +225,9 @@ typedef struct {
     uint32_t maxTileStack;     /* deepest tile traversal stack seen */
     uint32_t overflow;         /* nonzero if a budget overflowed */
     uint64_t nodeVisits;       /* BVH nodes tested by the tile traversal (4 child boxes each) */
+    uint32_t largeTiles;       /* tiles the standard LDS configuration passed to the large one */
+    uint32_t hugeTiles;        /* ... the large one to the huge one */
+    uint32_t walkTiles;        /* ... the huge one to the LDS-stack walk */
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled

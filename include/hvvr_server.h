@@ -23,6 +23,7 @@ typedef struct hvvr_server hvvr_server;
 typedef struct {
     uint64_t deviceBudgetBytes; /* device memory the server may hold in views (scenes are the caller's) */
     uint32_t maxInFlight;       /* requests on the device at once, 1..64 */
+    uint32_t shadingPrecision;  /* 32 (default when 0) or 16: hvvr_gpu_view_set_shading_precision */
 } hvvr_server_desc;
 
 int hvvr_server_create(hvvr_gpu_context* ctx, const hvvr_server_desc* desc, hvvr_server** out);
@@ -52,7 +53,10 @@ int hvvr_server_poll(hvvr_server* server);
 /* Polls until nothing is queued or in flight. */
 int hvvr_server_drain(hvvr_server* server);
 /* 1 once the request has completed, 0 if not yet; *latencyMs (may be NULL) = submit to
- * image written, in milliseconds. */
+ * image written, in milliseconds. A request whose frame overflowed a tile's traversal
+ * stack fails: HVVR_GPU_ERR_OVERFLOW (its image is not written). A frame whose triangle
+ * lists overflowed is rendered again once, after the view's list storage has grown to
+ * what it needed (and new views of its shape start at that size). */
 int hvvr_server_request_done(hvvr_server* server, uint64_t requestId, double* latencyMs);
 
 typedef struct {
@@ -61,6 +65,8 @@ typedef struct {
     uint64_t peakDeviceBytes;
     uint32_t viewsCreated, viewsEvicted, peakInFlight;
     double p50Ms, p99Ms, meanMs; /* submit-to-image latency over completed requests */
+    uint64_t failed;             /* requests that failed (not in completed) */
+    uint32_t listGrowths;        /* frames rendered again after their list storage grew */
 } hvvr_server_stats;
 int hvvr_server_get_stats(hvvr_server* server, hvvr_server_stats* stats);
 

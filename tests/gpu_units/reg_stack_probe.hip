@@ -61,10 +61,40 @@ __global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const f
         *finalSize = st.size;
 }
 
-// capacity 64 (RegStack) or 256 (RegStack4)
+// the same over the LDS stack (capacity LDS_PROBE_CAP)
+enum : uint32_t { LDS_PROBE_CAP = 2048 };
+__global__ __launch_bounds__(64) void ldsStackProbe(const uint32_t* ops, const float* keys, uint32_t n,
+                                                   uint32_t* popped, float* finalT, uint32_t* finalS,
+                                                   uint32_t* finalSize) {
+    __shared__ float t[LDS_PROBE_CAP];
+    __shared__ uint32_t m[LDS_PROBE_CAP];
+    const uint32_t lane = threadIdx.x;
+    LdsStack st;
+    stackBindLds(st, t, m);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t op = uniform(ops[i]);
+        if (op == 0u) {
+            if (st.size == 0u)
+                continue;
+            const uint32_t rec = stackPop(st);
+            if (lane == 0)
+                popped[i] = rec;
+        } else if (st.size < LDS_PROBE_CAP) {
+            stackInsert(st, bitsf(uniform(fbits(keys[i]))), i, lane);
+        }
+    }
+    for (uint32_t q = lane; q < st.size; q += 64) {
+        finalT[q] = t[q];
+        finalS[q] = m[q];
+    }
+    if (lane == 0)
+        *finalSize = st.size;
+}
+
+// capacity 64 (RegStack), 256 (RegStack4) or 2048 (LdsStack)
 int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n, uint32_t* popped, float* finalT,
              uint32_t* finalS, uint32_t* finalSize) {
-    if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP)
+    if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP && capacity != LDS_PROBE_CAP)
         return -5;
     uint32_t *dOps = nullptr, *dPopped = nullptr, *dS = nullptr, *dSize = nullptr;
     float *dKeys = nullptr, *dT = nullptr;
@@ -78,8 +108,10 @@ int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t
     if (!rc) {
         if (capacity == REG_STACK_CAP)
             regStackProbe<RegStack, REG_STACK_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
-        else
+        else if (capacity == REG_STACK4_CAP)
             regStackProbe<RegStack4, REG_STACK4_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
+        else
+            ldsStackProbe<<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         if (hipDeviceSynchronize() != hipSuccess)
             rc = -3;
     }

@@ -60,9 +60,10 @@ def test_room_262k_reduced_eye(H, O, ctx, scene2, view2_small):
 
 @pytest.mark.parametrize("limit", [1, 60, 64])
 def test_deep_stack_fallback_same_lists(H, O, ctx, scene2, view2_small, limit):
-    # a register stack budget below a tile's need sends that tile down the global-memory
-    # traversal path: 1 sends every tile, 60 about a third of them (block stacks hold ~60
-    # entries); at 64 only tiles with more than 768 visited nodes go there
+    # a stack budget below a tile's need sends that tile past every LDS configuration to
+    # the LDS-stack walk: 1 sends every tile, 60 about a third of them (block stacks hold
+    # ~60 entries); at 64 the large configuration takes tiles with more than 768 visited
+    # nodes
     gs, gv = _render(H, ctx, scene2, view2_small, 4, limit=limit)
     _check_frame(O, gs, gv, scene2, view2_small, 4, colors=False)
 

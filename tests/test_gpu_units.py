@@ -1,7 +1,7 @@
 """GPU unit probes of wave-level building blocks, against Python models.
 
-The VGPR-resident traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h), one register (64
-entries) and four registers (256): random sequences of sorted inserts and pops that keep
+The traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h), one register (64 entries), four
+registers (256) and LDS (2048 here): random sequences of sorted inserts and pops that keep
 them shallow, past each register boundary and at capacity; every popped record and the
 final contents must match a list model of the reference's insertion rule
 (traverse_avx.h:499-520: a new entry goes below the run of top entries whose key is
@@ -19,18 +19,22 @@ LIB = os.path.join(ROOT, "tests", "gpu_units", "_build", "libhvvr_gpu_units.so")
 
 
 def _model(ops, keys, cap):
-    stack = []  # index 0 = top
+    # stored bottom-up (the end of the lists is the top); the insertion rule reads top-down:
+    # the new entry goes below the run of top entries whose key is smaller
+    tk, tr = [], []
     popped = np.full(len(ops), 0xFFFFFFFF, np.uint32)
     for i, (op, k) in enumerate(zip(ops, keys)):
         if op == 0:
-            if stack:
-                popped[i] = stack.pop(0)[1]
-        elif len(stack) < cap:
-            r = 0
-            while r < len(stack) and stack[r][0] < k:
-                r += 1
-            stack.insert(r, (k, i))
-    return popped, stack
+            if tk:
+                tk.pop()
+                popped[i] = tr.pop()
+        elif len(tk) < cap:
+            q = len(tk)
+            while q > 0 and tk[q - 1] < k:
+                q -= 1
+            tk.insert(q, float(k))
+            tr.insert(q, i)
+    return popped, list(zip(tk[::-1], tr[::-1]))  # index 0 = top
 
 
 def _sequence(rng, n, push_bias, key_pool):
@@ -47,7 +51,7 @@ CASES = {"small": (300, 0.55), "deep": (400, 0.6), "full": (1500, 0.75), "ties":
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cap", [64, 256])
+@pytest.mark.parametrize("cap", [64, 256, 2048])
 @pytest.mark.parametrize("case", list(CASES))
 def test_register_stack_matches_model(case, cap):
     lib = ctypes.CDLL(LIB)
@@ -55,6 +59,8 @@ def test_register_stack_matches_model(case, cap):
     n, bias = CASES[case]
     if cap == 256:
         n *= 3  # enough pushes to cross every register boundary
+    elif cap == 2048:
+        n *= 24  # the LDS stack: deep enough for many 64-entry chunks
     pool = np.arange(8, dtype=np.float32) if case == "ties" else rng.normal(size=4096).astype(np.float32)
     ops, keys = _sequence(rng, n, bias, pool)
     popped = np.zeros(n, np.uint32)
@@ -71,7 +77,7 @@ def test_register_stack_matches_model(case, cap):
     assert int(size[0]) == len(want_stack)
     pops = ops == 0
     assert np.array_equal(popped[pops], want_popped[pops]), "popped records differ"
-    # the register stack is bottom-up: the model's top (index 0) is position size-1
+    # the stacks are bottom-up: the model's top (index 0) is position size-1
     top = len(want_stack) - 1
     got = [(float(ft[top - d]), int(fs[top - d])) for d in range(len(want_stack))]
     assert got == [(float(k), r) for k, r in want_stack], "final stack differs"
