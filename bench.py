@@ -142,16 +142,19 @@ def roofline(stage_ms, survivors, entries, visits):
 
 def serving_bench(args):
     """Serving mix: Poisson arrivals of view requests over resident scenes of 100k, 262k, 1M
-    and 2M triangles; each request picks a scene, a resolution (log-uniform over five sizes
-    from 256x256 to 1080x1200), an MSAA rate (1/2/4/8) and, for half of them, polar
+    and 2M triangles (the room generator at 391, 1,024, 4,096 and 8,192 meshes); each
+    request picks a scene, a resolution log-uniform in area between 256x256 and a
+    2x1080x1200 stereo pair (six size classes; the stereo class is two eye views submitted
+    together, done when both are), an MSAA rate (1/2/4/8) and, for half of them, polar
     foveation (the feb2017 pattern reconstructed at that size). Admission by a device
-    budget, FP16 shading. Reports throughput and submit-to-image latency."""
+    budget, FP16 shading. Reports completed requests/s and submit-to-image latency."""
     rng = np.random.default_rng(args.seed)
     sizes = [(256, 256), (360, 400), (540, 600), (760, 840), (1080, 1200)]
+    classes = sizes + [(1080, 1200)]  # the last class: a stereo pair
     rates = [1, 2, 4, 8]
     t0 = time.perf_counter()
     scenes = [H.HostScene(H.SCENE_ROOM, k, n) for k, n in enumerate((391, 1024, 4096, 8192))]
-    screens = [H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, w, h) for w, h in sizes]
+    screens = [[H.HostView(H.VIEW_PRESET_ROOM_EYE, e, w, h) for e in range(2)] for w, h in sizes]
     fov = H.HostFoveated(H.HostFoveated.FEB2017)
     log(f"serving setup: scenes {[len(s.triShade) for s in scenes]}, {time.perf_counter() - t0:.1f} s")
     ctx = H.GpuContext(0)
@@ -162,73 +165,86 @@ def serving_bench(args):
     log(f"scenes resident ({time.perf_counter() - t0:.1f} s)")
     srv = H.Server(ctx, int(args.budget_gib * (1 << 30)), args.slots, shading_bits=16)
     shape = {}
-    for i, v in enumerate(screens):
-        for a in rates:
-            shape[(i, a, False)] = srv.add_shape(v, a)
-            shape[(i, a, True)] = srv.add_foveated_shape(fov, v, a)
+    for i in range(len(sizes)):
+        for e in range(2):
+            for a in rates:
+                shape[(i, e, a, False)] = srv.add_shape(screens[i][e], a)
+                shape[(i, e, a, True)] = srv.add_foveated_shape(fov, screens[i][e], a)
     log(f"{len(shape)} shapes ({time.perf_counter() - t0:.1f} s)")
     n = args.requests
-    gap = rng.exponential(1.0 / args.rate, n)
-    arrival = np.cumsum(gap)
-    # log-uniform resolution over the five sizes: index uniform in log area
-    areas = np.log([w * h for w, h in sizes])
+    arrival = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    areas = np.log([w * h * (2 if c == len(sizes) else 1) for c, (w, h) in enumerate(classes)])
     pick = rng.uniform(areas[0], areas[-1], n)
-    size_idx = np.abs(pick[:, None] - areas[None, :]).argmin(1)
-    reqs = [(int(rng.integers(4)), int(size_idx[k]), rates[int(rng.integers(4))], bool(rng.random() < 0.5))
+    cls = np.abs(pick[:, None] - areas[None, :]).argmin(1)
+    reqs = [(int(rng.integers(4)), int(cls[k]), rates[int(rng.integers(4))], bool(rng.random() < 0.5))
             for k in range(n)]
-    cams = [screens[0].camera_to_world for _ in range(n)]
+
+    def views_of(c):
+        return [(len(sizes) - 1, 0), (len(sizes) - 1, 1)] if c == len(sizes) else [(c, 0)]
 
     def submit(k):
-        sc, si, a, fv = reqs[k]
+        sc, c, a, fv = reqs[k]
         eye = None
         if fv:
             y, p = np.radians(rng.uniform(-20, 20, 2))
             eye = np.array([np.sin(y) * np.cos(p), np.sin(p), -np.cos(y) * np.cos(p)], np.float32)
-        return srv.submit(gscenes[sc], shape[(si, a, fv)], cams[k], eye)
+        return [srv.submit(gscenes[sc], shape[(i, e, a, fv)], screens[i][e].camera_to_world, eye)[0]
+                for i, e in views_of(c)]
 
-    # warm-up: one request per shape creates the first views
+    # warm-up: one request per shape and scene creates the first views and sizes their lists
     for key in sorted(shape):
-        srv.submit(gscenes[0], shape[key], cams[0], None)
+        for g in gscenes:
+            srv.submit(g, shape[key], screens[key[0]][key[1]].camera_to_world, None)
         srv.drain()
     warm = srv.stats()
-    log(f"warm-up: {warm.viewsCreated} views, {warm.peakDeviceBytes / (1 << 30):.2f} GiB "
-        f"({time.perf_counter() - t0:.1f} s)")
+    log(f"warm-up: {warm.viewsCreated} views, {warm.listGrowths} list growths, "
+        f"{warm.peakDeviceBytes / (1 << 30):.2f} GiB ({time.perf_counter() - t0:.1f} s)")
+    ids = []
     t_start = time.perf_counter()
     k = 0
+    views_submitted = views_completed = 0
     next_log = 1.0
-    while k < n:
+    while True:
         now = time.perf_counter() - t_start
         while k < n and arrival[k] <= now:
-            submit(k)
+            ids.append(submit(k))
+            views_submitted += len(ids[-1])
             k += 1
-        srv.poll()
+        views_completed += srv.poll()
+        if k == n and views_completed == views_submitted:
+            break
         if now > next_log:
             st = srv.stats()
-            log(f"t={now:.1f} s: submitted {k}, completed {st.completed - warm.completed}, in flight "
+            log(f"t={now:.1f} s: requests submitted {k}, views completed {views_completed}, in flight "
                 f"{st.inFlight}, queued {st.queued}")
             next_log += 1.0
-    srv.drain()
     elapsed = time.perf_counter() - t_start
     st = srv.stats()
     lat = []
-    done = st.completed - warm.completed
+    for rids in ids:
+        ls = [srv.done(r) for r in rids]
+        assert all(d for d, _ in ls)
+        lat.append(max(l for _, l in ls))
+    lat = np.array(lat)
     subsamples = 0
-    for sc, si, a, fv in reqs:
-        w, h = sizes[si]
-        subsamples += (fov.valid_sample_count if fv else w * h) * a
+    for sc, c, a, fv in reqs:
+        for i, e in views_of(c):
+            w, h = sizes[i]
+            subsamples += (fov.valid_sample_count if fv else w * h) * a
     out = {
-        "metric": "serving mix: requests/s and submit-to-image latency (ms)",
-        "value": round(done / elapsed, 2), "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
-        "requests": int(done), "offered_rate": args.rate, "elapsed_s": round(elapsed, 3),
+        "metric": "serving mix: completed requests/s and submit-to-image latency (ms)",
+        "value": round(n / elapsed, 2), "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+        "requests": n, "offered_rate": args.rate, "elapsed_s": round(elapsed, 3),
         "msamples_per_s": round(subsamples / elapsed / 1e6, 1),
-        "p50_ms": round(st.p50Ms, 3), "p99_ms": round(st.p99Ms, 3), "mean_ms": round(st.meanMs, 3),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3),
         "peak_in_flight": st.peakInFlight, "views_created": st.viewsCreated, "views_evicted": st.viewsEvicted,
+        "list_growths": st.listGrowths, "failed": st.failed,
         "peak_device_gib": round(st.peakDeviceBytes / (1 << 30), 2), "dtype": "f32 intersect, f16 shade",
         "data": "synthetic: seeded room scenes of 100k/262k/1M/2M triangles",
         "config": {"workload": "serving mix", "slots": args.slots, "budget_gib": args.budget_gib,
-                   "sizes": sizes, "msaa": rates, "foveated_fraction": 0.5},
+                   "sizes": sizes, "stereo_class": "2x1080x1200", "msaa": rates, "foveated_fraction": 0.5},
     }
-    del lat
     srv.close()
     print(json.dumps(out))
 

@@ -400,7 +400,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->blockStacks.resize(size_t(v->blockCount) * BLOCK_STACK_SIZE));
     step(v->blockStackSize.resize(v->blockCount));
     step(v->tileCounts.resize(v->tileCount));
-    step(v->tileQueues.resize(size_t(3) * v->tileCount));
+    step(v->tileQueues.resize(size_t(4) * v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
@@ -413,7 +413,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         step(upload(v->lensTable, table.data(), table.size(), st));
         step(v->occRank.resize(v->tileCount));
     }
-    step(v->counters.resize(8));
+    step(v->counters.resize(16));
     for (auto& ev : v->stageEvents)
         step(hipEventCreate(&ev));
     if (e == hipSuccess)
@@ -861,8 +861,8 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + 5 * T * 4 + listCapacity * 4 +
-                 S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 8 * 8;
+                 B * 4 + 6 * T * 4 + listCapacity * 4 +
+                 S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 16 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
     if (d->lensRadius > 0.0f)
@@ -1009,7 +1009,7 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     int rc = viewSync(v);
     if (rc && rc != HVVR_GPU_ERR_OVERFLOW)
         return rc;
-    uint64_t counters[8];
+    uint64_t counters[16];
     HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
     std::vector<uint32_t> counts(v->tileCount);
     HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
@@ -1024,7 +1024,8 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     st->nodeVisits = counters[4];
     st->largeTiles = uint32_t(counters[5]);
     st->hugeTiles = uint32_t(counters[6]);
-    st->walkTiles = uint32_t(counters[7]);
+    st->giantTiles = uint32_t(counters[7]);
+    st->walkTiles = uint32_t(counters[8]);
     st->tileCount = v->tileCount;
     for (uint32_t c : counts)
         st->occupiedTiles += c ? 1u : 0u;

@@ -135,8 +135,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::FrustumPre> tilePre, blockPre;
     hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
-    hvvr::DeviceBuffer<uint32_t> tileQueues;            // [3][tileCount] tiles queued for the large and
-                                                        // huge LDS configurations and the LDS-stack walk
+    hvvr::DeviceBuffer<uint32_t> tileQueues;            // [4][tileCount] tiles queued for the large, huge
+                                                        // and giant LDS configurations and the walk
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length
     hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount] list start in triIndices
     hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage, tiles in allocation order
@@ -148,8 +148,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> occRank;               // depth of field: rank among occupied tiles
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
                                                         // [3] list entries allocated, [4] node visits,
-                                                        // [5..7] tiles queued for the large and huge
-                                                        // configurations and for the walk
+                                                        // [5..8] tiles queued for the large, huge and
+                                                        // giant configurations and for the walk
     uint64_t listCapacity = 0;
     uint32_t waveStackLimit = 64; // register stack entries per tile traversal (<= 64)
     // foveated path (hvvr_gpu_view_set_foveated): cull planes and the t-max buffer apply to

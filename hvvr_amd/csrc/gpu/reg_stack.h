@@ -64,29 +64,48 @@ __device__ inline void stackInsert(RegStack& st, float key, uint32_t meta, uint3
 }
 
 // ---------------------------------------------------------------------------------
-// Four-register variant (capacity 256) for tiles with deep stacks: position q lives in
-// lane q % 64 of register pair (t, m)[q / 64], bottom-up as above.
+// N-register variant (capacity 64 N) for tiles with deep stacks: position q lives in lane
+// q % 64 of register pair (t, m)[q / 64], bottom-up as above. The arrays are indexed by
+// compile-time constants only (unrolled loops), so they stay in VGPRs.
 // ---------------------------------------------------------------------------------
-enum : uint32_t { REG_STACK4_CAP = 256 };
-
-struct RegStack4 {
-    float t0, t1, t2, t3;
-    uint32_t m0, m1, m2, m3;
+template <int N>
+struct RegStackN {
+    float t[N];
+    uint32_t m[N];
     uint32_t size;
 };
+using RegStack4 = RegStackN<4>;
+using RegStack8 = RegStackN<8>;
+using RegStack16 = RegStackN<16>;
+enum : uint32_t { REG_STACK4_CAP = 256, REG_STACK8_CAP = 512, REG_STACK16_CAP = 1024 };
+
+template <int N>
+__device__ inline void stackInit(RegStackN<N>& st, float t, uint32_t m, uint32_t size, uint32_t) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        st.t[k] = k == 0 ? t : 0.0f;
+        st.m[k] = k == 0 ? m : 0u;
+    }
+    st.size = size;
+}
 
 // key and meta at stack position q (q < size)
-__device__ inline void stackRead(const RegStack4& st, uint32_t q, float& key, uint32_t& meta) {
-    const uint32_t ln = q & 63u;
-    switch (q >> 6) {
-        case 0: key = bitsf(readLane(fbits(st.t0), ln)); meta = readLane(st.m0, ln); break;
-        case 1: key = bitsf(readLane(fbits(st.t1), ln)); meta = readLane(st.m1, ln); break;
-        case 2: key = bitsf(readLane(fbits(st.t2), ln)); meta = readLane(st.m2, ln); break;
-        default: key = bitsf(readLane(fbits(st.t3), ln)); meta = readLane(st.m3, ln); break;
+template <int N>
+__device__ inline void stackRead(const RegStackN<N>& st, uint32_t q, float& key, uint32_t& meta) {
+    const uint32_t ln = q & 63u, r = q >> 6;
+    key = 0.0f;
+    meta = 0u;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (uint32_t(k) == r) {
+            key = bitsf(readLane(fbits(st.t[k]), ln));
+            meta = readLane(st.m[k], ln);
+        }
     }
 }
 
-__device__ inline uint32_t stackPop(RegStack4& st) {
+template <int N>
+__device__ inline uint32_t stackPop(RegStackN<N>& st) {
     float key;
     uint32_t meta;
     stackRead(st, st.size - 1u, key, meta);
@@ -94,7 +113,8 @@ __device__ inline uint32_t stackPop(RegStack4& st) {
     return meta;
 }
 
-__device__ inline float stackTopKey(const RegStack4& st) {
+template <int N>
+__device__ inline float stackTopKey(const RegStackN<N>& st) {
     float key;
     uint32_t meta;
     stackRead(st, st.size - 1u, key, meta);
@@ -114,28 +134,29 @@ __device__ inline void shiftInsertReg(float& t, uint32_t& m, uint32_t base, uint
 }
 
 // Inserts (key, meta) one above the highest entry whose key is not smaller; the caller
-// keeps size < 256.
-__device__ inline void stackInsert(RegStack4& st, float key, uint32_t meta, uint32_t lane) {
-    uint32_t p = 0;
+// keeps size < 64 N.
+template <int N>
+__device__ inline void stackInsert(RegStackN<N>& st, float key, uint32_t meta, uint32_t lane) {
+    uint32_t p = 0; // found positions are >= 1
 #pragma unroll
-    for (int reg = 3; reg >= 0; reg--) {
+    for (int reg = N - 1; reg >= 0; reg--) {
         if (p == 0u && 64u * uint32_t(reg) < st.size) {
-            const float t = reg == 0 ? st.t0 : (reg == 1 ? st.t1 : (reg == 2 ? st.t2 : st.t3));
-            const uint64_t b = ballot(64u * uint32_t(reg) + lane < st.size && !(t < key));
+            const uint64_t b = ballot(64u * uint32_t(reg) + lane < st.size && !(st.t[reg] < key));
             if (b)
                 p = 64u * uint32_t(reg) + (64u - uint32_t(__builtin_clzll(b)));
         }
     }
     const uint32_t lo = p >> 6, hi = st.size >> 6; // registers touched (the new top is at position size)
     // highest register first, so every carry is read before its source register changes
-    if (lo <= 3u && hi >= 3u)
-        shiftInsertReg(st.t3, st.m3, 192u, lane, p, key, meta, readLane(fbits(st.t2), 63), readLane(st.m2, 63));
-    if (lo <= 2u && hi >= 2u)
-        shiftInsertReg(st.t2, st.m2, 128u, lane, p, key, meta, readLane(fbits(st.t1), 63), readLane(st.m1, 63));
-    if (lo <= 1u && hi >= 1u)
-        shiftInsertReg(st.t1, st.m1, 64u, lane, p, key, meta, readLane(fbits(st.t0), 63), readLane(st.m0, 63));
-    if (lo == 0u)
-        shiftInsertReg(st.t0, st.m0, 0u, lane, p, key, meta, 0u, 0u);
+#pragma unroll
+    for (int reg = N - 1; reg >= 0; reg--) {
+        if (lo <= uint32_t(reg) && hi >= uint32_t(reg)) {
+            const int below = reg > 0 ? reg - 1 : 0;
+            const uint32_t ct = reg > 0 ? readLane(fbits(st.t[below]), 63) : 0u;
+            const uint32_t cm = reg > 0 ? readLane(st.m[below], 63) : 0u;
+            shiftInsertReg(st.t[reg], st.m[reg], 64u * uint32_t(reg), lane, p, key, meta, ct, cm);
+        }
+    }
     st.size++;
 }
 

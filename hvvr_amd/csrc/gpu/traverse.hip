@@ -214,11 +214,13 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
 // of anything), so a tile with one, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
-// Two configurations share the code. The standard one (768 records, a one-register
-// 64-entry stack, 10.5 KB of LDS) runs every tile; a tile it cannot hold is queued, and
-// the large one (2048 records, a four-register 256-entry stack, 28 KB of LDS) takes the
-// queue with a few hundred persistent workgroups. Only what neither holds goes to the
-// one-thread-per-tile global-stack kernel.
+// Four configurations share the code. The standard one (768 records, a one-register
+// 64-entry stack, 10.5 KB of LDS) runs every tile; a tile it cannot hold is queued for
+// the large one (2048 records, a four-register 256-entry stack, 28 KB of LDS), what that
+// cannot hold for the huge one (4608 records, eight registers, 512 entries, 63 KB), and
+// so on to the giant one (10240 records, sixteen registers, 1024 entries, 140 KB); each
+// takes its queue with persistent workgroups, as many as fit the chip at once. What none
+// holds goes to the wave-per-tile walk with its stack in LDS.
 // ---------------------------------------------------------------------------------
 // Record meta: first child record (bits 0..19), child mask (20..23), leaf mask (24..27),
 // stack-entry leaf mask (28..31).
@@ -231,27 +233,21 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 
 // The standard configuration keeps the record index in bits 10..19 of a stack entry's
 // meta (records < 1024), so a pop needs no LDS read; the others stack record indices.
-// The huge one keeps its stack in LDS beside the records (the stack store).
-enum : uint32_t { LDS_STACK_CAP = 2048 };
-struct NoStackStore {};
-struct LdsStackStore {
-    float t[LDS_STACK_CAP];
-    uint32_t m[LDS_STACK_CAP];
-};
 struct TileListSmall {
     enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1 };
     using Stack = RegStack;
-    using StackStore = NoStackStore;
 };
 struct TileListLarge {
     enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0 };
     using Stack = RegStack4;
-    using StackStore = NoStackStore;
 };
 struct TileListHuge {
-    enum : uint32_t { RecCap = 8192, StackCap = LDS_STACK_CAP, Packed = 0 };
-    using Stack = LdsStack;
-    using StackStore = LdsStackStore;
+    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0 };
+    using Stack = RegStack8;
+};
+struct TileListGiant {
+    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0 };
+    using Stack = RegStack16;
 };
 
 template <class Cfg>
@@ -262,7 +258,6 @@ struct TileListLds {
     uint32_t recMeta[Cfg::RecCap]; // packMeta(first child record, child mask, leaf mask,
                                    // stack-entry leaf mask)
     uint16_t order[Cfg::RecCap];   // records with emitted leaves, in emission order
-    typename Cfg::StackStore stack;
 };
 
 template <class Cfg>
@@ -285,23 +280,12 @@ __device__ inline void fromStackEntry(uint32_t e, const TileListLds<Cfg>& sh, ui
     }
 }
 
-__device__ inline void stackBind(RegStack&, NoStackStore&) {}
-__device__ inline void stackBind(RegStack4&, NoStackStore&) {}
-__device__ inline void stackBind(LdsStack& st, LdsStackStore& s) { stackBindLds(st, s.t, s.m); }
 
 __device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size, uint32_t) {
     st.t = t;
     st.m = m;
     st.size = size;
 }
-__device__ inline void stackInit(RegStack4& st, float t, uint32_t m, uint32_t size, uint32_t) {
-    st.t0 = t;
-    st.m0 = m;
-    st.t1 = st.t2 = st.t3 = 0.0f;
-    st.m1 = st.m2 = st.m3 = 0u;
-    st.size = size;
-}
-
 // One child box of a node, as one lane of a 4-lane node group loads it.
 struct ChildBox {
     float b0, b1, b2, b3, b4, b5; // bounds[k][child]
@@ -461,7 +445,6 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
     // ---- phase B: the reference's stack walk over the live records, stack in VGPRs ----
     if (!overflow) {
         Stack st;
-        stackBind(st, sh.stack);
         {
             // the block stack's live entries, compacted in order (entry q at position q)
             const bool valid = lane < n0;
@@ -677,7 +660,7 @@ __global__ __launch_bounds__(64) void tileWalkKernel(const BVHNode* __restrict__
                                                      unsigned long long* __restrict__ counters, int emit) {
     __shared__ TileWalkLds sh;
     const uint32_t lane = threadIdx.x;
-    const uint32_t queued = uniform(uint32_t(counters[7]));
+    const uint32_t queued = uniform(uint32_t(counters[8]));
     for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
         const uint32_t tile = uniform(queue[qi]);
         const uint32_t block = tile / TILES_PER_BLOCK;
@@ -784,7 +767,7 @@ __global__ void walkAllocKernel(const uint32_t* __restrict__ queue, const uint32
                                 uint32_t* __restrict__ offsets, uint64_t capacity,
                                 unsigned long long* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= uint32_t(counters[7]))
+    if (i >= uint32_t(counters[8]))
         return;
     const uint32_t tile = queue[i];
     const unsigned long long start = atomicAdd(&counters[3], (unsigned long long)counts[tile]);
@@ -837,7 +820,7 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
 }
 
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    uint32_t* queues = v->tileQueues.ptr; // [3][tileCount]: large, huge, walk
+    uint32_t* queues = v->tileQueues.ptr; // [4][tileCount]: large, huge, giant, walk
     const uint32_t T = v->tileCount;
     TileListArgs g;
     g.nodes = s->nodes.ptr;
@@ -864,21 +847,30 @@ void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stre
     large.queueCounter = 5;
     large.nextQueue = queues + T;
     large.nextCounter = 6;
-    tileListQueueKernel<TileListLarge><<<T < 512u ? T : 512u, 64, 0, stream>>>(large);
-    // 128 KB of LDS: one workgroup per CU
+    // 28 KB of LDS: five workgroups per CU
+    tileListQueueKernel<TileListLarge><<<T < 1280u ? T : 1280u, 64, 0, stream>>>(large);
+    // 63 KB of LDS: two workgroups per CU
     TileListArgs huge = large;
-    huge.stackLimit = full ? LDS_STACK_CAP : v->waveStackLimit;
+    huge.stackLimit = full ? REG_STACK8_CAP : v->waveStackLimit;
     huge.queue = queues + T;
     huge.queueCounter = 6;
     huge.nextQueue = queues + 2 * T;
     huge.nextCounter = 7;
-    tileListQueueKernel<TileListHuge><<<T < 256u ? T : 256u, 64, 0, stream>>>(huge);
+    tileListQueueKernel<TileListHuge><<<T < 512u ? T : 512u, 64, 0, stream>>>(huge);
+    // 140 KB of LDS: one workgroup per CU
+    TileListArgs giant = huge;
+    giant.stackLimit = full ? REG_STACK16_CAP : v->waveStackLimit;
+    giant.queue = queues + 2 * T;
+    giant.queueCounter = 7;
+    giant.nextQueue = queues + 3 * T;
+    giant.nextCounter = 8;
+    tileListQueueKernel<TileListGiant><<<T < 256u ? T : 256u, 64, 0, stream>>>(giant);
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
     const uint32_t T = v->tileCount;
-    const uint32_t* walkQueue = v->tileQueues.ptr + 2 * size_t(T);
+    const uint32_t* walkQueue = v->tileQueues.ptr + 3 * size_t(T);
     const uint32_t groups = T < 1024u ? T : 1024u; // 32 KB of LDS each
     for (int emit = 0; emit < 2; emit++) {
         if (emit)

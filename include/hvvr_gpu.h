@@ -227,7 +227,8 @@ typedef struct {
     uint64_t nodeVisits;       /* BVH nodes tested by the tile traversal (4 child boxes each) */
     uint32_t largeTiles;       /* tiles the standard LDS configuration passed to the large one */
     uint32_t hugeTiles;        /* ... the large one to the huge one */
-    uint32_t walkTiles;        /* ... the huge one to the LDS-stack walk */
+    uint32_t giantTiles;       /* ... the huge one to the giant one */
+    uint32_t walkTiles;        /* ... the giant one to the LDS-stack walk */
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled

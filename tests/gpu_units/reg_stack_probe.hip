@@ -15,24 +15,26 @@ __device__ void probeInit(RegStack& st) {
     st.m = 0u;
     st.size = 0;
 }
-__device__ void probeInit(RegStack4& st) {
-    st.t0 = st.t1 = st.t2 = st.t3 = 0.0f;
-    st.m0 = st.m1 = st.m2 = st.m3 = 0u;
+template <int N>
+__device__ void probeInit(RegStackN<N>& st) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        st.t[k] = 0.0f;
+        st.m[k] = 0u;
+    }
     st.size = 0;
 }
 __device__ void probeStore(const RegStack& st, uint32_t lane, float* finalT, uint32_t* finalS) {
     finalT[lane] = st.t;
     finalS[lane] = st.m;
 }
-__device__ void probeStore(const RegStack4& st, uint32_t lane, float* finalT, uint32_t* finalS) {
-    finalT[lane] = st.t0;
-    finalT[64 + lane] = st.t1;
-    finalT[128 + lane] = st.t2;
-    finalT[192 + lane] = st.t3;
-    finalS[lane] = st.m0;
-    finalS[64 + lane] = st.m1;
-    finalS[128 + lane] = st.m2;
-    finalS[192 + lane] = st.m3;
+template <int N>
+__device__ void probeStore(const RegStackN<N>& st, uint32_t lane, float* finalT, uint32_t* finalS) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        finalT[64 * k + lane] = st.t[k];
+        finalS[64 * k + lane] = st.m[k];
+    }
 }
 
 // ops[i]: 0 = pop (popped[i] = record), 1 = insert (keys[i], record i as the entry's meta).
@@ -91,10 +93,11 @@ __global__ __launch_bounds__(64) void ldsStackProbe(const uint32_t* ops, const f
         *finalSize = st.size;
 }
 
-// capacity 64 (RegStack), 256 (RegStack4) or 2048 (LdsStack)
+// capacity 64 (RegStack), 256 (RegStack4), 512 (RegStack8) or 2048 (LdsStack)
 int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n, uint32_t* popped, float* finalT,
              uint32_t* finalS, uint32_t* finalSize) {
-    if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP && capacity != LDS_PROBE_CAP)
+    if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP && capacity != REG_STACK8_CAP &&
+        capacity != LDS_PROBE_CAP)
         return -5;
     uint32_t *dOps = nullptr, *dPopped = nullptr, *dS = nullptr, *dSize = nullptr;
     float *dKeys = nullptr, *dT = nullptr;
@@ -110,6 +113,8 @@ int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t
             regStackProbe<RegStack, REG_STACK_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         else if (capacity == REG_STACK4_CAP)
             regStackProbe<RegStack4, REG_STACK4_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
+        else if (capacity == REG_STACK8_CAP)
+            regStackProbe<RegStack8, REG_STACK8_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         else
             ldsStackProbe<<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         if (hipDeviceSynchronize() != hipSuccess)

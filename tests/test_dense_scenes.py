@@ -57,14 +57,15 @@ def test_dense_scene_lists_grow_and_match(H, O, ctx, dense, dense_view, dense_re
     bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
     assert len(bad) == 0, f"{len(bad)} samples differ, first {bad[:5]}"
     st = gv.stats()
-    print(f"tiles {st.tileCount}: large {st.largeTiles}, huge {st.hugeTiles}, walk {st.walkTiles}, "
+    print(f"tiles {st.tileCount}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
+          f"walk {st.walkTiles}, "
           f"deepest stack {st.maxTileStack}", flush=True)
     assert st.overflow == 0 and st.survivorPairs == dense_ref["survivors"]
     if limit is None:
         # tiles past the large configuration exist and the huge one holds most of them
         assert st.hugeTiles > 0 and st.walkTiles < st.hugeTiles
     else:
-        assert st.walkTiles == st.largeTiles == st.hugeTiles > 0
+        assert st.walkTiles == st.largeTiles == st.hugeTiles == st.giantTiles > 0
     gv.close()
     gs.close()
 
