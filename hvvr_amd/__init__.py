@@ -434,6 +434,74 @@ def render_views(scene, views, cameras=None, jitters=None):
     _check_gpu(lib.hvvr_gpu_render_views(scene.handle, c_uint32(n), handles, _ptr(ms), _ptr(js)))
 
 
+class LaunchShape(ctypes.Structure):
+    _fields_ = [("largeGroups", c_uint32), ("hugeGroups", c_uint32), ("giantGroups", c_uint32),
+                ("walkGroups", c_uint32), ("xcdSwizzle", c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GpuBatch:
+    """Many views of one scene, each kernel of the frame launched once for all of them
+    (hvvr_gpu_batch_*); optionally captured as a HIP graph and replayed."""
+
+    def __init__(self, ctx: GpuContext, views):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.views = list(views)
+        handles = (c_void_p * len(self.views))(*[v.handle.value for v in self.views])
+        self.handle = c_void_p()
+        _check_gpu(self.lib.hvvr_gpu_batch_create(ctx.handle, handles, c_uint32(len(self.views)),
+                                                  byref(self.handle)))
+
+    def set_graph(self, enable):
+        _check_gpu(self.lib.hvvr_gpu_batch_set_graph(self.handle, c_int(1 if enable else 0)))
+
+    def set_refit(self, every_frame):
+        _check_gpu(self.lib.hvvr_gpu_batch_set_refit(self.handle, c_int(1 if every_frame else 0)))
+
+    def set_launch_shape(self, shape: LaunchShape):
+        _check_gpu(self.lib.hvvr_gpu_batch_set_launch_shape(self.handle, byref(shape)))
+
+    def launch_shape(self):
+        sh = LaunchShape()
+        _check_gpu(self.lib.hvvr_gpu_batch_get_launch_shape(self.handle, byref(sh)))
+        return sh
+
+    def _cameras(self, cameras):
+        return np.ascontiguousarray(np.stack([
+            np.asarray(v.view.camera_to_world if cameras is None or cameras[i] is None else cameras[i],
+                       np.float32).reshape(16) for i, v in enumerate(self.views)]), np.float32)
+
+    def render(self, scene: GpuScene, cameras=None):
+        m = self._cameras(cameras)
+        _check_gpu(self.lib.hvvr_gpu_batch_render(self.handle, scene.handle, _ptr(m)))
+
+    def synchronize(self):
+        ms = ctypes.c_float()
+        _check_gpu(self.lib.hvvr_gpu_batch_synchronize(self.handle, byref(ms)))
+        return ms.value
+
+    def autotune(self, scene: GpuScene, cameras=None, frames_per_trial=5):
+        m = self._cameras(cameras)
+        best, ms = LaunchShape(), ctypes.c_float()
+        _check_gpu(self.lib.hvvr_gpu_batch_autotune(self.handle, scene.handle, _ptr(m), c_uint32(frames_per_trial),
+                                                    byref(best), byref(ms)))
+        return best, ms.value
+
+    def close(self):
+        if self.handle:
+            self.lib.hvvr_gpu_batch_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class _FovDesc(ctypes.Structure):
     _fields_ = [("polarRemap", c_void_p), ("samplesPerRing", c_uint32), ("ringCount", c_uint32),
                 ("eccentricityMap", c_float * 6), ("maxEccentricity", c_float), ("taaAlpha", c_float),

@@ -20,29 +20,12 @@
 #include "device_common.h"
 #include "dof.h"
 #include "gpu_internal.h"
+#include "launch_args.h"
 
 #include <type_traits>
 
 namespace hvvr {
 
-struct IntersectArgs {
-    GBufferSubsample* gbuffer;
-    const DirectionalBeam* beams;
-    const uint32_t* tileCounts;
-    const uint32_t* tileOffsets;
-    const uint32_t* triIndices;
-    const SimpleRayFrustum* worldTiles;
-    const PrecomputedTriangleIntersect* tris;
-    unsigned long long* survivors;
-    uint64_t listCapacity;
-    matrix4x4 cameraToWorld;
-    vector2 offsets[32];
-    // depth of field: lens-position table [4][A][128], rank of each tile among the
-    // tiles with a non-empty list (its table row is rank % 4), thin lens
-    const vector2* lensTable;
-    const uint32_t* occRank;
-    float lensRadius, focalDistance;
-};
 
 struct TriTile {
     float t;
@@ -142,7 +125,7 @@ __device__ inline bool frustaUVWPass(const SimpleRayFrustum& frustum, const Prec
 }
 
 template <int A, bool DoF>
-__global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
+__device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
     using Cached = typename std::conditional<DoF, TriTileDoF, TriTile>::type;
     __shared__ Cached cache[128];
     __shared__ DofTile dofTile;
@@ -151,7 +134,6 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
     __shared__ SimpleRayFrustum frustum;
     __shared__ uint32_t waveSurvivors[2];
 
-    const uint32_t tile = blockIdx.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint32_t listCount = args.tileCounts[tile];
@@ -310,7 +292,22 @@ __global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
     emitGBuffer<A>(hit, waveBase, lane);
 }
 
-void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+template <int A, bool DoF>
+__global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
+    intersectBody<A, DoF>(args, blockIdx.x);
+}
+
+// every tile of the views whose units `start` lists (pinhole views of one MSAA rate)
+template <int A>
+__global__ __launch_bounds__(128) void intersectBatchKernel(const IntersectArgs* __restrict__ all,
+                                                            const uint32_t* __restrict__ start, uint32_t n,
+                                                            uint32_t swizzle) {
+    const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
+    const uint32_t v = batchView(start, n, unit);
+    intersectBody<A, false>(all[v], unit - start[v]);
+}
+
+IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m) {
     IntersectArgs a;
     a.gbuffer = v->gbuffer.ptr;
     a.beams = v->beams.ptr;
@@ -328,6 +325,11 @@ void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4&
     a.occRank = v->occRank.ptr;
     a.lensRadius = v->lens.radius;
     a.focalDistance = v->lens.focalDistance;
+    return a;
+}
+
+void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+    const IntersectArgs a = intersectArgs(v, s, m);
     dim3 grid(v->tileCount), block(128);
     const bool dof = v->dof;
     switch (v->msaaRate) {
@@ -345,6 +347,24 @@ void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4&
         HVVR_INTERSECT_CASE(16)
         HVVR_INTERSECT_CASE(32)
 #undef HVVR_INTERSECT_CASE
+        default: break;
+    }
+}
+
+void launchIntersectBatch(int msaaRate, const IntersectArgs* all, const uint32_t* start, uint32_t viewCount,
+                          uint32_t units, const LaunchShape& shape, hipStream_t stream) {
+    switch (msaaRate) {
+#define HVVR_INTERSECT_BATCH_CASE(A)                                                                          \
+    case A:                                                                                                   \
+        intersectBatchKernel<A><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle);          \
+        break;
+        HVVR_INTERSECT_BATCH_CASE(1)
+        HVVR_INTERSECT_BATCH_CASE(2)
+        HVVR_INTERSECT_BATCH_CASE(4)
+        HVVR_INTERSECT_BATCH_CASE(8)
+        HVVR_INTERSECT_BATCH_CASE(16)
+        HVVR_INTERSECT_BATCH_CASE(32)
+#undef HVVR_INTERSECT_BATCH_CASE
         default: break;
     }
 }

@@ -1,0 +1,179 @@
+#pragma once
+// Kernel arguments of the per-frame chain, one struct per kernel, and the launchers that
+// run a kernel over many views at once.
+//
+// A single view's launch passes its struct by value (kernel arguments). A batch keeps one
+// struct per view in device memory, rewritten from page-locked host memory every frame
+// (so a captured graph replays with new cameras), and launches each kernel once for all
+// views: grid-indexed kernels find their view from the prefix of per-view work units
+// (`start`, viewCount + 1 entries, start[viewCount] = all units), persistent kernels walk
+// every view's queue in turn.
+
+#include "gpu_internal.h"
+
+namespace hvvr {
+
+struct FrustumSetupArgs {
+    const SimpleRayFrustum* tileCam;
+    const SimpleRayFrustum* blockCam;
+    SimpleRayFrustum* tileWorld;
+    FrustumPre* tilePre;
+    FrustumPre* blockPre;
+    uint32_t tileCount, blockCount;
+    matrix4x4 cameraToWorld;
+    CullPlanes cull;
+};
+
+struct BlockTraverseArgs {
+    const BVHNode* nodes;
+    const FrustumPre* blockPre;
+    StackEntry* stacks;
+    uint32_t* stackSize;
+};
+
+struct TileListArgs {
+    const BVHNode* nodes;
+    const FrustumPre* tilePre;
+    const StackEntry* blockStacks;
+    const uint32_t* blockStackSize;
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint32_t* triOut;
+    uint64_t capacity;
+    const uint32_t* queue;   // the tiles this launch takes (queued kernels)
+    uint32_t* nextQueue;     // where a tile this configuration cannot hold goes
+    uint32_t queueCounter;   // counters[] slot holding the length of `queue`
+    uint32_t nextCounter;    // and of `nextQueue`
+    unsigned long long* counters;
+    uint32_t stackLimit;
+};
+
+struct TileWalkArgs {
+    const BVHNode* nodes;
+    const FrustumPre* tilePre;
+    const StackEntry* blockStacks;
+    const uint32_t* blockStackSize;
+    const uint32_t* queue;
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint32_t* triOut;
+    uint64_t capacity;
+    unsigned long long* counters;
+};
+
+struct IntersectArgs {
+    GBufferSubsample* gbuffer;
+    const DirectionalBeam* beams;
+    const uint32_t* tileCounts;
+    const uint32_t* tileOffsets;
+    const uint32_t* triIndices;
+    const SimpleRayFrustum* worldTiles;
+    const PrecomputedTriangleIntersect* tris;
+    unsigned long long* survivors;
+    uint64_t listCapacity;
+    matrix4x4 cameraToWorld;
+    vector2 offsets[32];
+    // depth of field: lens-position table [4][A][128], rank of each tile among the
+    // tiles with a non-empty list (its table row is rank % 4), thin lens
+    const vector2* lensTable;
+    const uint32_t* occRank;
+    float lensRadius, focalDistance;
+};
+
+struct ResolveArgs {
+    uint32_t* sampleResults;
+    const GBufferSubsample* gbuffer;
+    const DirectionalBeam* beams;
+    const uint32_t* tileCounts;
+    const PrecomputedTriangleIntersect* tris;
+    const PrecomputedTriangleShade* triShade;
+    const ShadingVertex* verts;
+    const SimpleMaterial* materials;
+    matrix4x4 cameraToWorld;
+    vector2 offsets[32];
+    LightingEnvironment env;
+    // depth of field (see IntersectArgs)
+    const vector2* lensTable;
+    const uint32_t* occRank;
+    float lensRadius, focalDistance;
+    // foveated path: per sample, the view-axis distance of subsample 0's hit (infinity on
+    // a miss; ResolveKernel's TMaxBuffer, resolve.cu:386-457); null = not written
+    float* tMax;
+};
+
+struct RemapArgs {
+    const uint32_t* src;
+    const int32_t* remap;
+    uint32_t* image;
+    uint32_t width, height, split;
+};
+
+// one view's arguments (host)
+FrustumSetupArgs frustumSetupArgs(const hvvr_gpu_view* v, const matrix4x4& m);
+BlockTraverseArgs blockTraverseArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s);
+// level 0..3: the standard, large, huge and giant tile-list configurations
+TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level);
+TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s);
+IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m);
+ResolveArgs resolveArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m);
+RemapArgs remapArgs(const hvvr_gpu_view* v);
+
+// work units of one view in each grid-indexed kernel
+inline uint32_t frustumSetupUnits(const hvvr_gpu_view* v) { return (v->tileCount + v->blockCount + 127) / 128; }
+inline uint32_t remapUnits(const hvvr_gpu_view* v) { return ((v->width + 15) / 16) * ((v->height + 7) / 8); }
+
+// Launch shape of a batch: persistent workgroups of the queued tile-list configurations
+// and the walk, and whether grid-indexed tile kernels map consecutive workgroups to one
+// XCD (workgroups are dealt to the eight XCDs in turn; with the swizzle, XCD x takes a
+// contiguous run of tiles, which share BVH nodes and triangles in its L2). None of these
+// changes a result.
+struct LaunchShape {
+    uint32_t largeGroups = 1280, hugeGroups = 512, giantGroups = 256, walkGroups = 1024;
+    uint32_t xcdSwizzle = 0;
+};
+
+// batch launchers; `all` = device array of per-view arguments
+void launchClearCountersBatch(uint64_t* const* counters, uint32_t viewCount, uint32_t slots, hipStream_t stream);
+void launchFrustumSetupBatch(const FrustumSetupArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                             hipStream_t stream);
+void launchBlockTraverseBatch(const BlockTraverseArgs* all, const uint32_t* start, uint32_t viewCount,
+                              uint32_t units, hipStream_t stream);
+// all[level * viewCount + view]: the four configurations' arguments
+void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                          const LaunchShape& shape, hipStream_t stream);
+void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
+                         hipStream_t stream);
+void launchIntersectBatch(int msaaRate, const IntersectArgs* all, const uint32_t* start, uint32_t viewCount,
+                          uint32_t units, const LaunchShape& shape, hipStream_t stream);
+void launchResolveBatch(int msaaRate, bool half, const ResolveArgs* all, const uint32_t* start, uint32_t viewCount,
+                        uint32_t units, const LaunchShape& shape, hipStream_t stream);
+void launchRemapBatch(const RemapArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                      hipStream_t stream);
+
+#ifdef __HIPCC__
+// the view whose work units [start[v], start[v+1]) hold `unit` (wave-uniform search)
+__device__ inline uint32_t batchView(const uint32_t* __restrict__ start, uint32_t viewCount, uint32_t unit) {
+    uint32_t lo = 0, hi = viewCount;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= unit)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// workgroup b of n, dealt round-robin to 8 XCDs, as the unit it takes: with the swizzle,
+// XCD x takes units [x * n / 8, (x + 1) * n / 8) in order
+__device__ inline uint32_t xcdUnit(uint32_t b, uint32_t n, uint32_t swizzle) {
+    if (!swizzle)
+        return b;
+    const uint32_t xcd = b & 7u, k = b >> 3;
+    const uint32_t q = n >> 3, r = n & 7u; // XCD x holds q + (x < r) units
+    const uint32_t first = xcd * q + (xcd < r ? xcd : r);
+    return first + k;
+}
+#endif
+
+} // namespace hvvr

@@ -19,29 +19,10 @@
 #include "device_common.h"
 #include "dof.h"
 #include "gpu_internal.h"
+#include "launch_args.h"
 
 namespace hvvr {
 
-struct ResolveArgs {
-    uint32_t* sampleResults;
-    const GBufferSubsample* gbuffer;
-    const DirectionalBeam* beams;
-    const uint32_t* tileCounts;
-    const PrecomputedTriangleIntersect* tris;
-    const PrecomputedTriangleShade* triShade;
-    const ShadingVertex* verts;
-    const SimpleMaterial* materials;
-    matrix4x4 cameraToWorld;
-    vector2 offsets[32];
-    LightingEnvironment env;
-    // depth of field (see IntersectArgs)
-    const vector2* lensTable;
-    const uint32_t* occRank;
-    float lensRadius, focalDistance;
-    // foveated path: per sample, the view-axis distance of subsample 0's hit (infinity on
-    // a miss; ResolveKernel's TMaxBuffer, resolve.cu:386-457); null = not written
-    float* tMax;
-};
 
 // IntersectTriangleThread::calcUVW (prim_tests.h:233-245): barycentrics of the ray at
 // subsample offset alpha; ndir/ndu/ndv are the negated world-space beam vectors
@@ -254,9 +235,8 @@ __device__ inline float acesFilm(float x) {
 }
 
 template <int A, bool DoF, bool Half>
-__global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
+__device__ inline void resolveBody(const ResolveArgs& args, uint32_t tile) {
     __shared__ DofTile dofTile;
-    const uint32_t tile = blockIdx.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t sample = tile * TILE_SIZE + tid;
     if (args.tileCounts[tile] == 0) {
@@ -365,10 +345,27 @@ __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
     }
 }
 
-__global__ void remapKernel(const uint32_t* __restrict__ src, const int32_t* __restrict__ remap,
-                            uint32_t* __restrict__ image, uint32_t width, uint32_t height, uint32_t split) {
-    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+template <int A, bool DoF, bool Half>
+__global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
+    resolveBody<A, DoF, Half>(args, blockIdx.x);
+}
+
+// every tile of the views whose units `start` lists (pinhole views of one MSAA rate and
+// shading precision)
+template <int A, bool Half>
+__global__ __launch_bounds__(128) void resolveBatchKernel(const ResolveArgs* __restrict__ all,
+                                                          const uint32_t* __restrict__ start, uint32_t n,
+                                                          uint32_t swizzle) {
+    const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
+    const uint32_t v = batchView(start, n, unit);
+    resolveBody<A, false, Half>(all[v], unit - start[v]);
+}
+
+__device__ inline void remapBody(const RemapArgs& a, uint32_t x, uint32_t y) {
+    const uint32_t* __restrict__ src = a.src;
+    const int32_t* __restrict__ remap = a.remap;
+    uint32_t* __restrict__ image = a.image;
+    const uint32_t width = a.width, height = a.height, split = a.split;
     if (x >= width || y >= height)
         return;
     size_t pix = size_t(y) * width + x;
@@ -390,7 +387,20 @@ __global__ void remapKernel(const uint32_t* __restrict__ src, const int32_t* __r
         image[pix] = (c[0] & 0xffu) | (c[1] & 0xff00u) | (c[2] & 0xff0000u) | 0xff000000u;
 }
 
-void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+__global__ void remapKernel(RemapArgs a) {
+    remapBody(a, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
+}
+
+// 16x8 pixel blocks of every view, row-major per view
+__global__ __launch_bounds__(128) void remapBatchKernel(const RemapArgs* __restrict__ all,
+                                                        const uint32_t* __restrict__ start, uint32_t n) {
+    const uint32_t v = batchView(start, n, blockIdx.x);
+    const RemapArgs& a = all[v];
+    const uint32_t local = blockIdx.x - start[v], bx = (a.width + 15) / 16;
+    remapBody(a, (local % bx) * 16 + (threadIdx.x & 15u), (local / bx) * 8 + (threadIdx.x >> 4));
+}
+
+ResolveArgs resolveArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m) {
     ResolveArgs a;
     a.sampleResults = v->sampleResults.ptr;
     a.gbuffer = v->gbuffer.ptr;
@@ -409,6 +419,22 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
     a.lensRadius = v->lens.radius;
     a.focalDistance = v->lens.focalDistance;
     a.tMax = v->foveated ? v->tMax.ptr : nullptr;
+    return a;
+}
+
+RemapArgs remapArgs(const hvvr_gpu_view* v) {
+    RemapArgs a;
+    a.src = v->sampleResults.ptr;
+    a.remap = v->remap.ptr;
+    a.image = v->image.ptr;
+    a.width = v->width;
+    a.height = v->height;
+    a.split = v->split;
+    return a;
+}
+
+void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream) {
+    const ResolveArgs a = resolveArgs(v, s, m);
     dim3 grid(v->tileCount), block(128);
     const bool dof = v->dof;
     const bool half = v->halfShading;
@@ -438,8 +464,33 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
 void launchRemap(hvvr_gpu_view* v, hipStream_t stream) {
     dim3 block(16, 8);
     dim3 grid((v->width + 15) / 16, (v->height + 7) / 8);
-    remapKernel<<<grid, block, 0, stream>>>(v->sampleResults.ptr, v->remap.ptr, v->image.ptr, v->width, v->height,
-                                            v->split);
+    remapKernel<<<grid, block, 0, stream>>>(remapArgs(v));
+}
+
+void launchResolveBatch(int msaaRate, bool half, const ResolveArgs* all, const uint32_t* start, uint32_t viewCount,
+                        uint32_t units, const LaunchShape& shape, hipStream_t stream) {
+    switch (msaaRate) {
+#define HVVR_RESOLVE_BATCH_CASE(A)                                                                            \
+    case A:                                                                                                   \
+        if (half)                                                                                             \
+            resolveBatchKernel<A, true><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle);  \
+        else                                                                                                  \
+            resolveBatchKernel<A, false><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle); \
+        break;
+        HVVR_RESOLVE_BATCH_CASE(1)
+        HVVR_RESOLVE_BATCH_CASE(2)
+        HVVR_RESOLVE_BATCH_CASE(4)
+        HVVR_RESOLVE_BATCH_CASE(8)
+        HVVR_RESOLVE_BATCH_CASE(16)
+        HVVR_RESOLVE_BATCH_CASE(32)
+#undef HVVR_RESOLVE_BATCH_CASE
+        default: break;
+    }
+}
+
+void launchRemapBatch(const RemapArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                      hipStream_t stream) {
+    remapBatchKernel<<<units, 128, 0, stream>>>(all, start, viewCount);
 }
 
 } // namespace hvvr

@@ -18,6 +18,7 @@
 //                             LDS (count pass, allocation, emit pass)
 #include "device_common.h"
 #include "gpu_internal.h"
+#include "launch_args.h"
 #include "reg_stack.h"
 #include "wave_traverse.h"
 
@@ -35,13 +36,14 @@ __device__ inline bool planeCullsFrustum(const vector4& p, const vector3 o[4], c
     return allOut;
 }
 
-__global__ void frustumSetupKernel(const SimpleRayFrustum* __restrict__ tileCam, uint32_t tileCount,
-                                   const SimpleRayFrustum* __restrict__ blockCam, uint32_t blockCount, matrix4x4 m,
-                                   CullPlanes cull, SimpleRayFrustum* __restrict__ tileWorld,
-                                   FrustumPre* __restrict__ tilePre, FrustumPre* __restrict__ blockPre) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= tileCount + blockCount)
+__device__ inline void frustumSetupBody(const FrustumSetupArgs& a, uint32_t i) {
+    const uint32_t tileCount = a.tileCount;
+    if (i >= tileCount + a.blockCount)
         return;
+    const matrix4x4& m = a.cameraToWorld;
+    const CullPlanes& cull = a.cull;
+    const SimpleRayFrustum* __restrict__ tileCam = a.tileCam;
+    const SimpleRayFrustum* __restrict__ blockCam = a.blockCam;
     bool isTile = i < tileCount;
     const SimpleRayFrustum& src = isTile ? tileCam[i] : blockCam[i - tileCount];
     vector3 o[4], d[4];
@@ -72,11 +74,21 @@ __global__ void frustumSetupKernel(const SimpleRayFrustum* __restrict__ tileCam,
             w.origins[v] = o[v];
             w.directions[v] = d[v];
         }
-        tileWorld[i] = w;
-        tilePre[i] = f;
+        a.tileWorld[i] = w;
+        a.tilePre[i] = f;
     } else {
-        blockPre[i - tileCount] = f;
+        a.blockPre[i - tileCount] = f;
     }
+}
+
+__global__ void frustumSetupKernel(FrustumSetupArgs a) {
+    frustumSetupBody(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(128) void frustumSetupBatchKernel(const FrustumSetupArgs* __restrict__ all,
+                                                               const uint32_t* __restrict__ start, uint32_t n) {
+    const uint32_t v = batchView(start, n, blockIdx.x);
+    frustumSetupBody(all[v], (blockIdx.x - start[v]) * 128u + threadIdx.x);
 }
 
 struct CullEntry {
@@ -95,12 +107,11 @@ __device__ inline float ftz(float x) {
 }
 
 // traverseBlocks + BlockFrame::sort, one wave64 per block
-__global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __restrict__ nodes,
-                                                              const FrustumPre* __restrict__ blockPre,
-                                                              uint32_t blockCount, StackEntry* __restrict__ stacks,
-                                                              uint32_t* __restrict__ stackSize) {
-    __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
-    const uint32_t b = blockIdx.x;
+__device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* cull, uint32_t b) {
+    const BVHNode* __restrict__ nodes = a.nodes;
+    const FrustumPre* __restrict__ blockPre = a.blockPre;
+    StackEntry* __restrict__ stacks = a.stacks;
+    uint32_t* __restrict__ stackSize = a.stackSize;
     const uint32_t lane = threadIdx.x, c = lane & 3u, role = lane >> 2;
     const uint32_t projAxis = laneProjAxis(role);
     WaveFrustum w;
@@ -182,6 +193,18 @@ __global__ __launch_bounds__(64) void blockTraverseWaveKernel(const BVHNode* __r
     }
     if (lane == 0)
         stackSize[b] = top;
+}
+
+__global__ __launch_bounds__(64) void blockTraverseWaveKernel(BlockTraverseArgs a) {
+    __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
+    blockTraverseBody(a, cull, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraverseArgs* __restrict__ all,
+                                                               const uint32_t* __restrict__ start, uint32_t n) {
+    __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
+    const uint32_t v = batchView(start, n, blockIdx.x);
+    blockTraverseBody(all[v], cull, blockIdx.x - start[v]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -323,22 +346,6 @@ __device__ inline uint32_t leafListLength(const BVHNode* __restrict__ nodes, uin
     return n;
 }
 
-struct TileListArgs {
-    const BVHNode* nodes;
-    const FrustumPre* tilePre;
-    const StackEntry* blockStacks;
-    const uint32_t* blockStackSize;
-    uint32_t* counts;
-    uint32_t* offsets;
-    uint32_t* triOut;
-    uint64_t capacity;
-    const uint32_t* queue;   // the tiles this launch takes (queued kernels)
-    uint32_t* nextQueue;     // where a tile this configuration cannot hold goes
-    uint32_t queueCounter;   // counters[] slot holding the length of `queue`
-    uint32_t nextCounter;    // and of `nextQueue`
-    unsigned long long* counters;
-    uint32_t stackLimit;
-};
 
 // Builds one tile's list; false when the tile does not fit this configuration (its count
 // and offset are then zero).
@@ -607,30 +614,63 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
 }
 
 // every tile, standard configuration; a tile it cannot hold is queued for the large one
-__global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
-    __shared__ TileListLds<TileListSmall> sh;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
+__device__ inline void tileListSmallOne(const TileListArgs& g, TileListLds<TileListSmall>& sh, uint32_t tile,
+                                        uint32_t lane) {
     if (!tileListBody<TileListSmall>(g, sh, tile, lane) && lane == 0) {
         const unsigned long long slot = atomicAdd(&g.counters[g.nextCounter], 1ull);
         g.nextQueue[slot] = tile;
     }
 }
 
-// the tiles a smaller configuration queued, persistent workgroups; a tile this one cannot
-// hold either is queued for the next
+__global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
+    __shared__ TileListLds<TileListSmall> sh;
+    tileListSmallOne(g, sh, blockIdx.x, threadIdx.x);
+}
+
+__global__ __launch_bounds__(64) void tileListBatchKernel(const TileListArgs* __restrict__ all,
+                                                          const uint32_t* __restrict__ start, uint32_t n,
+                                                          uint32_t swizzle) {
+    __shared__ TileListLds<TileListSmall> sh;
+    const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
+    const uint32_t v = batchView(start, n, unit);
+    tileListSmallOne(all[v], sh, unit - start[v], threadIdx.x);
+}
+
+// The tiles a smaller configuration queued, persistent workgroups: this workgroup takes
+// the entries `first`, `first` + stride, ... of the queue; a tile this configuration
+// cannot hold either is queued for the next.
 template <class Cfg>
-__global__ __launch_bounds__(64) void tileListQueueKernel(TileListArgs g) {
-    __shared__ TileListLds<Cfg> sh;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t queued = uniform(uint32_t(g.counters[g.queueCounter]));
-    for (uint32_t i = blockIdx.x; i < queued; i += gridDim.x) {
+__device__ inline void tileListQueueRun(const TileListArgs& g, TileListLds<Cfg>& sh, uint32_t first, uint32_t stride,
+                                        uint32_t queued, uint32_t lane) {
+    for (uint32_t i = first; i < queued; i += stride) {
         const uint32_t tile = uniform(g.queue[i]);
         waveLdsSync(); // the previous tile's LDS reads are done
         if (!tileListBody<Cfg>(g, sh, tile, lane) && lane == 0) {
             const unsigned long long slot = atomicAdd(&g.counters[g.nextCounter], 1ull);
             g.nextQueue[slot] = tile;
         }
+    }
+}
+
+template <class Cfg>
+__global__ __launch_bounds__(64) void tileListQueueKernel(TileListArgs g) {
+    __shared__ TileListLds<Cfg> sh;
+    const uint32_t queued = uniform(uint32_t(g.counters[g.queueCounter]));
+    tileListQueueRun<Cfg>(g, sh, blockIdx.x, gridDim.x, queued, threadIdx.x);
+}
+
+// every view's queue in turn, the entries of all of them dealt to the workgroups as one
+// sequence
+template <class Cfg>
+__global__ __launch_bounds__(64) void tileListQueueBatchKernel(const TileListArgs* __restrict__ all, uint32_t n) {
+    __shared__ TileListLds<Cfg> sh;
+    uint32_t base = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const TileListArgs& g = all[v];
+        const uint32_t queued = uniform(uint32_t(g.counters[g.queueCounter]));
+        const uint32_t first = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x;
+        tileListQueueRun<Cfg>(g, sh, first, gridDim.x, queued, threadIdx.x);
+        base += queued;
     }
 }
 
@@ -650,32 +690,27 @@ struct TileWalkLds {
     uint32_t m[TILE_STACK_SIZE];
 };
 
-__global__ __launch_bounds__(64) void tileWalkKernel(const BVHNode* __restrict__ nodes,
-                                                     const FrustumPre* __restrict__ tilePre,
-                                                     const StackEntry* __restrict__ blockStacks,
-                                                     const uint32_t* __restrict__ blockStackSize,
-                                                     const uint32_t* __restrict__ queue, uint32_t* __restrict__ counts,
-                                                     const uint32_t* __restrict__ offsets,
-                                                     uint32_t* __restrict__ triOut, uint64_t capacity,
-                                                     unsigned long long* __restrict__ counters, int emit) {
-    __shared__ TileWalkLds sh;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t queued = uniform(uint32_t(counters[8]));
-    for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
-        const uint32_t tile = uniform(queue[qi]);
+__device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint32_t tile, int emit,
+                                   uint32_t lane) {
+    const BVHNode* __restrict__ nodes = a.nodes;
+    const uint64_t capacity = a.capacity;
+    unsigned long long* __restrict__ counters = a.counters;
+    uint32_t* __restrict__ counts = a.counts;
+    uint32_t* __restrict__ triOut = a.triOut;
+    {
         const uint32_t block = tile / TILES_PER_BLOCK;
-        const uint32_t n0 = uniform(blockStackSize[block]);
-        const FrustumPre& f = tilePre[tile];
+        const uint32_t n0 = uniform(a.blockStackSize[block]);
+        const FrustumPre& f = a.tilePre[tile];
         if (n0 == 0 || f.degenerate) {
             if (!emit && lane == 0)
                 counts[tile] = 0;
-            continue;
+            return;
         }
         waveLdsSync(); // the previous tile's LDS reads are done
         LdsStack st;
         stackBindLds(st, sh.t, sh.m);
         {
-            const StackEntry* bs = blockStacks + size_t(block) * BLOCK_STACK_SIZE;
+            const StackEntry* bs = a.blockStacks + size_t(block) * BLOCK_STACK_SIZE;
             float t0 = 0.0f;
             uint32_t m0 = 0u;
             if (lane < n0) {
@@ -684,7 +719,7 @@ __global__ __launch_bounds__(64) void tileWalkKernel(const BVHNode* __restrict__
             }
             stackInit(st, t0, m0, n0, lane);
         }
-        const uint64_t outBase = emit ? offsets[tile] : 0;
+        const uint64_t outBase = emit ? a.offsets[tile] : 0;
         uint32_t count = 0, visits = 0, maxSize = n0;
         bool listOverflow = false, stackOverflow = false;
         const int axis = f.distanceAxis;
@@ -762,30 +797,113 @@ __global__ __launch_bounds__(64) void tileWalkKernel(const BVHNode* __restrict__
     }
 }
 
+__global__ __launch_bounds__(64) void tileWalkKernel(TileWalkArgs a, int emit) {
+    __shared__ TileWalkLds sh;
+    const uint32_t queued = uniform(uint32_t(a.counters[8]));
+    for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
+        waveLdsSync(); // the previous tile's LDS reads are done
+        tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __restrict__ all, uint32_t n,
+                                                          int emit) {
+    __shared__ TileWalkLds sh;
+    uint32_t base = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const TileWalkArgs& a = all[v];
+        const uint32_t queued = uniform(uint32_t(a.counters[8]));
+        for (uint32_t qi = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x; qi < queued; qi += gridDim.x) {
+            waveLdsSync();
+            tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
+        }
+        base += queued;
+    }
+}
+
 // List space for the tiles the walk kernel took, after its count pass.
-__global__ void walkAllocKernel(const uint32_t* __restrict__ queue, const uint32_t* __restrict__ counts,
-                                uint32_t* __restrict__ offsets, uint64_t capacity,
-                                unsigned long long* __restrict__ counters) {
+__device__ inline void walkAllocOne(const TileWalkArgs& a, uint32_t i) {
+    const uint32_t tile = a.queue[i];
+    const uint32_t count = a.counts[tile];
+    const unsigned long long start = atomicAdd(&a.counters[3], (unsigned long long)count);
+    a.offsets[tile] = uint32_t(start);
+    if (start + count > a.capacity)
+        atomicOr(&a.counters[1], 1ull);
+}
+
+__global__ void walkAllocKernel(TileWalkArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= uint32_t(counters[8]))
-        return;
-    const uint32_t tile = queue[i];
-    const unsigned long long start = atomicAdd(&counters[3], (unsigned long long)counts[tile]);
-    offsets[tile] = uint32_t(start);
-    if (start + counts[tile] > capacity)
-        atomicOr(&counters[1], 1ull);
+    if (i < uint32_t(a.counters[8]))
+        walkAllocOne(a, i);
+}
+
+__global__ void walkAllocBatchKernel(const TileWalkArgs* __restrict__ all, uint32_t n) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t base = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const TileWalkArgs& a = all[v];
+        const uint32_t queued = uint32_t(a.counters[8]);
+        for (uint32_t i = (id + stride - base % stride) % stride; i < queued; i += stride)
+            walkAllocOne(a, i);
+        base += queued;
+    }
+}
+
+FrustumSetupArgs frustumSetupArgs(const hvvr_gpu_view* v, const matrix4x4& m) {
+    FrustumSetupArgs a;
+    a.tileCam = v->tileFrustaCam.ptr;
+    a.blockCam = v->blockFrustaCam.ptr;
+    a.tileWorld = v->tileFrustaWorld.ptr;
+    a.tilePre = v->tilePre.ptr;
+    a.blockPre = v->blockPre.ptr;
+    a.tileCount = v->tileCount;
+    a.blockCount = v->blockCount;
+    a.cameraToWorld = m;
+    a.cull = v->cull;
+    return a;
+}
+
+BlockTraverseArgs blockTraverseArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
+    BlockTraverseArgs a;
+    a.nodes = s->nodes.ptr;
+    a.blockPre = v->blockPre.ptr;
+    a.stacks = v->blockStacks.ptr;
+    a.stackSize = v->blockStackSize.ptr;
+    return a;
+}
+
+TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
+    TileWalkArgs a;
+    a.nodes = s->nodes.ptr;
+    a.tilePre = v->tilePre.ptr;
+    a.blockStacks = v->blockStacks.ptr;
+    a.blockStackSize = v->blockStackSize.ptr;
+    a.queue = v->tileQueues.ptr + 3 * size_t(v->tileCount);
+    a.counts = v->tileCounts.ptr;
+    a.offsets = v->tileOffsets.ptr;
+    a.triOut = v->triIndices.ptr;
+    a.capacity = v->listCapacity;
+    a.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    return a;
 }
 
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream) {
-    uint32_t n = v->tileCount + v->blockCount;
-    frustumSetupKernel<<<(n + 127) / 128, 128, 0, stream>>>(v->tileFrustaCam.ptr, v->tileCount, v->blockFrustaCam.ptr,
-                                                            v->blockCount, m, v->cull, v->tileFrustaWorld.ptr,
-                                                            v->tilePre.ptr, v->blockPre.ptr);
+    frustumSetupKernel<<<frustumSetupUnits(v), 128, 0, stream>>>(frustumSetupArgs(v, m));
 }
 
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    blockTraverseWaveKernel<<<v->blockCount, 64, 0, stream>>>(s->nodes.ptr, v->blockPre.ptr, v->blockCount,
-                                                              v->blockStacks.ptr, v->blockStackSize.ptr);
+    blockTraverseWaveKernel<<<v->blockCount, 64, 0, stream>>>(blockTraverseArgs(v, s));
+}
+
+void launchFrustumSetupBatch(const FrustumSetupArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                             hipStream_t stream) {
+    frustumSetupBatchKernel<<<units, 128, 0, stream>>>(all, start, viewCount);
+}
+
+void launchBlockTraverseBatch(const BlockTraverseArgs* all, const uint32_t* start, uint32_t viewCount,
+                              uint32_t units, hipStream_t stream) {
+    blockTraverseBatchKernel<<<units, 64, 0, stream>>>(all, start, viewCount);
 }
 
 // Rank of every tile among the tiles with a non-empty list (the reference's compacted
@@ -819,9 +937,9 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
     occupiedRankKernel<<<1, kRankThreads, 0, stream>>>(v->tileCounts.ptr, v->tileCount, v->occRank.ptr);
 }
 
-void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
+TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level) {
     uint32_t* queues = v->tileQueues.ptr; // [4][tileCount]: large, huge, giant, walk
-    const uint32_t T = v->tileCount;
+    const size_t T = v->tileCount;
     TileListArgs g;
     g.nodes = s->nodes.ptr;
     g.tilePre = v->tilePre.ptr;
@@ -831,55 +949,62 @@ void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stre
     g.offsets = v->tileOffsets.ptr;
     g.triOut = v->triIndices.ptr;
     g.capacity = v->listCapacity;
-    g.queue = nullptr;
-    g.nextQueue = queues;
-    g.queueCounter = 0;
-    g.nextCounter = 5;
     g.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
-    g.stackLimit = v->waveStackLimit;
-    tileListKernel<<<T, 64, 0, stream>>>(g);
-    // the larger configurations' stack limit is the view's limit scaled past the standard
-    // one (a test limit below 64 keeps every configuration short)
+    // level L takes queue L - 1 (counters[4 + L]) and fills queue L (counters[5 + L]); the
+    // larger configurations' stack limit is the view's limit scaled past the standard one
+    // (a test limit below 64 keeps every configuration short)
+    g.queue = level > 0 ? queues + (level - 1) * T : nullptr;
+    g.queueCounter = level > 0 ? uint32_t(4 + level) : 0u;
+    g.nextQueue = queues + level * T;
+    g.nextCounter = uint32_t(5 + level);
     const bool full = v->waveStackLimit >= REG_STACK_CAP;
-    TileListArgs large = g;
-    large.stackLimit = full ? REG_STACK4_CAP : v->waveStackLimit;
-    large.queue = queues;
-    large.queueCounter = 5;
-    large.nextQueue = queues + T;
-    large.nextCounter = 6;
-    // 28 KB of LDS: five workgroups per CU
-    tileListQueueKernel<TileListLarge><<<T < 1280u ? T : 1280u, 64, 0, stream>>>(large);
-    // 63 KB of LDS: two workgroups per CU
-    TileListArgs huge = large;
-    huge.stackLimit = full ? REG_STACK8_CAP : v->waveStackLimit;
-    huge.queue = queues + T;
-    huge.queueCounter = 6;
-    huge.nextQueue = queues + 2 * T;
-    huge.nextCounter = 7;
-    tileListQueueKernel<TileListHuge><<<T < 512u ? T : 512u, 64, 0, stream>>>(huge);
-    // 140 KB of LDS: one workgroup per CU
-    TileListArgs giant = huge;
-    giant.stackLimit = full ? REG_STACK16_CAP : v->waveStackLimit;
-    giant.queue = queues + 2 * T;
-    giant.queueCounter = 7;
-    giant.nextQueue = queues + 3 * T;
-    giant.nextCounter = 8;
-    tileListQueueKernel<TileListGiant><<<T < 256u ? T : 256u, 64, 0, stream>>>(giant);
+    const uint32_t caps[4] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP};
+    g.stackLimit = level == 0 || !full ? v->waveStackLimit : caps[level];
+    return g;
+}
+
+static uint32_t clampGroups(uint32_t want, uint32_t tiles) {
+    return want < tiles ? want : (tiles ? tiles : 1u);
+}
+
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
+    const LaunchShape shape;
+    const uint32_t T = v->tileCount;
+    tileListKernel<<<T, 64, 0, stream>>>(tileListArgs(v, s, 0));
+    // 28 KB of LDS: five workgroups per CU; 63 KB: two; 140 KB: one
+    tileListQueueKernel<TileListLarge><<<clampGroups(shape.largeGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 1));
+    tileListQueueKernel<TileListHuge><<<clampGroups(shape.hugeGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 2));
+    tileListQueueKernel<TileListGiant><<<clampGroups(shape.giantGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 3));
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    unsigned long long* counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    const LaunchShape shape;
     const uint32_t T = v->tileCount;
-    const uint32_t* walkQueue = v->tileQueues.ptr + 3 * size_t(T);
-    const uint32_t groups = T < 1024u ? T : 1024u; // 32 KB of LDS each
+    const TileWalkArgs a = tileWalkArgs(v, s);
     for (int emit = 0; emit < 2; emit++) {
         if (emit)
-            walkAllocKernel<<<(T + 63) / 64, 64, 0, stream>>>(walkQueue, v->tileCounts.ptr, v->tileOffsets.ptr,
-                                                              v->listCapacity, counters);
-        tileWalkKernel<<<groups, 64, 0, stream>>>(s->nodes.ptr, v->tilePre.ptr, v->blockStacks.ptr,
-                                                  v->blockStackSize.ptr, walkQueue, v->tileCounts.ptr,
-                                                  v->tileOffsets.ptr, v->triIndices.ptr, v->listCapacity, counters,
-                                                  emit);
+            walkAllocKernel<<<(T + 63) / 64, 64, 0, stream>>>(a);
+        tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a, emit); // 32 KB of LDS each
+    }
+}
+
+void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
+                          const LaunchShape& shape, hipStream_t stream) {
+    tileListBatchKernel<<<units, 64, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle);
+    tileListQueueBatchKernel<TileListLarge><<<clampGroups(shape.largeGroups, units), 64, 0, stream>>>(
+        all + viewCount, viewCount);
+    tileListQueueBatchKernel<TileListHuge><<<clampGroups(shape.hugeGroups, units), 64, 0, stream>>>(
+        all + 2 * viewCount, viewCount);
+    tileListQueueBatchKernel<TileListGiant><<<clampGroups(shape.giantGroups, units), 64, 0, stream>>>(
+        all + 3 * viewCount, viewCount);
+}
+
+void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
+                         hipStream_t stream) {
+    for (int emit = 0; emit < 2; emit++) {
+        if (emit)
+            walkAllocBatchKernel<<<(maxTiles + 63) / 64, 64, 0, stream>>>(all, viewCount);
+        tileWalkBatchKernel<<<clampGroups(shape.walkGroups, maxTiles), 64, 0, stream>>>(all, viewCount, emit);
     }
 }
 

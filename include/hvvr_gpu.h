@@ -114,12 +114,49 @@ int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float
 /* Several views of one scene in one submission (batched multi-view). */
 int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
                           const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
+/* Batches: many views of one scene with each kernel of the frame launched once for all
+ * of them (the reference renders its cameras one after another, render.cpp:204-219). The
+ * views must be pinhole grid views (no depth of field, not foveated) of the batch's
+ * context and outlive the batch; any sizes, MSAA rates and shading precisions mix. After
+ * a batch render every view reads back as after its own render. */
+typedef struct hvvr_gpu_batch hvvr_gpu_batch;
+typedef struct {
+    uint32_t largeGroups, hugeGroups, giantGroups; /* persistent workgroups of the queued tile-list configurations */
+    uint32_t walkGroups;                           /* ... and of the LDS-stack walk */
+    uint32_t xcdSwizzle;                           /* 1: each XCD takes a contiguous run of tiles */
+} hvvr_gpu_launch_shape;
+int hvvr_gpu_batch_create(hvvr_gpu_context* ctx, hvvr_gpu_view* const* views, uint32_t count, hvvr_gpu_batch** out);
+int hvvr_gpu_batch_destroy(hvvr_gpu_batch* batch);
+/* 1: capture the frame (argument upload, optional refit, the kernel chain) as a HIP graph
+ * on the next render and replay it on later ones (captured again when the views' MSAA
+ * rates, shading precisions or sizes, the scene, the refit setting or the launch shape
+ * change). */
+int hvvr_gpu_batch_set_graph(hvvr_gpu_batch* batch, int enable);
+/* 1: every frame refits the scene's BVH first, inside the batch's chain (animated
+ * vertices or transforms set through hvvr_gpu_scene_set_*). 0 (default): the scene is
+ * updated as for single-view renders, when dirty. */
+int hvvr_gpu_batch_set_refit(hvvr_gpu_batch* batch, int everyFrame);
+/* The launch shape changes how work is scheduled, never a result. */
+int hvvr_gpu_batch_set_launch_shape(hvvr_gpu_batch* batch, const hvvr_gpu_launch_shape* shape);
+int hvvr_gpu_batch_get_launch_shape(hvvr_gpu_batch* batch, hvvr_gpu_launch_shape* shape);
+/* One frame of every view; cameraToWorld: [count][16]. Asynchronous; a render waits for
+ * the batch's previous frame. */
+int hvvr_gpu_batch_render(hvvr_gpu_batch* batch, hvvr_gpu_scene* scene, const float* cameraToWorld);
+/* Waits for the last frame; *gpuMs (may be NULL) = its device time. */
+int hvvr_gpu_batch_synchronize(hvvr_gpu_batch* batch, float* gpuMs);
+/* Tries launch shapes one field at a time (coordinate descent), framesPerTrial frames
+ * each (median device time), keeps the fastest and leaves it set; a candidate whose
+ * per-view work counters differ from the first fails the call. */
+int hvvr_gpu_batch_autotune(hvvr_gpu_batch* batch, hvvr_gpu_scene* scene, const float* cameraToWorld,
+                            uint32_t framesPerTrial, hvvr_gpu_launch_shape* best, float* bestMs);
+
 /* Tile traversal keeps its distance-sorted stack in registers, up to `entries` (1..64,
  * default 64; it holds only nodes that still have work). Tiles that need more (or more
- * than 768 visited nodes, or that meet a NaN distance) are re-traversed with a full-depth
- * (4096-entry, the reference's TileFrame::stackSize, traverse_avx.h:281) global-memory
- * stack. The result is the same either way; the limit only moves work between the two
- * paths. */
+ * than 768 visited nodes, or that meet a NaN distance) go to larger configurations (2048,
+ * 4608 and 10240 nodes; 256, 512 and 1024 stack entries, scaled down with a limit below
+ * 64) and what none holds to a walk with the reference's full-depth stack (4096 entries,
+ * TileFrame::stackSize, traverse_avx.h:281) in LDS. The result is the same either way;
+ * the limit only moves work between the paths. */
 int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entries);
 /* 32 (default): FP32 shading, the oracle's arithmetic. 16: the GGX colour terms of each
  * light in packed FP16 (the scalar D / visibility / Fresnel weights and the radiance sum
