@@ -365,9 +365,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--pipeline", choices=["plain", "batched"], default="plain",
+    ap.add_argument("--pipeline", choices=["plain", "batched", "graph"], default="plain",
                     help="plain: one view after the other on one stream; batched: both eyes in one "
-                         "submission (hvvr_gpu_render_views), overlapping on separate streams")
+                         "submission (hvvr_gpu_render_views), overlapping on separate streams; graph: both "
+                         "eyes as one batch (each kernel launched once for both), captured as a HIP graph "
+                         "and replayed, launch shape autotuned")
+    ap.add_argument("--shading", type=int, choices=[32, 16], default=32,
+                    help="shading precision: 32 = FP32 (the oracle's arithmetic), 16 = packed-FP16 colour terms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the reference "
                          "workload for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -404,40 +408,69 @@ def main():
     ctx = H.GpuContext(0)
     gs = H.GpuScene(ctx, scene)
     gvs = [H.GpuView(ctx, v, A) for v in views]
+    for gv in gvs:
+        gv.set_shading_precision(args.shading)
+    batch, shape = None, None
+    if args.pipeline == "graph":
+        batch = H.GpuBatch(ctx, gvs)
+        shape, _ = batch.autotune(gs, frames_per_trial=5)
+        batch.set_graph(True)
+        log(f"autotuned launch shape {shape.as_dict()}")
 
     def frame():
-        if args.pipeline == "batched":
+        if args.pipeline == "graph":
+            batch.render(gs)
+        elif args.pipeline == "batched":
             H.render_views(gs, gvs)
         else:
             for gv in gvs:
                 gv.render(gs)
 
+    def sync():
+        if batch is not None:
+            batch.synchronize()
+        ctx.synchronize()
+
     for _ in range(args.warmup):
         frame()
-    ctx.synchronize()
+    sync()
 
-    # timed region: K stereo frames, each synchronised so its latency is measured too; the
-    # views record per-stage HIP events on the library's stream (no extra synchronisation),
-    # read after each frame's synchronise
-    ctx.set_profiling(True)
+    # timed region: K stereo frames, each synchronised so its latency is measured too; in
+    # the plain and batched pipelines the views record per-stage HIP events on the
+    # library's stream (no extra synchronisation), read after each frame's synchronise
+    ctx.set_profiling(batch is None)
     lat = []
     stage_ms = np.zeros(len(STAGES))
-    ctx.synchronize()
+    sync()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         f0 = time.perf_counter()
         frame()
-        ctx.synchronize()
+        sync()
         lat.append(time.perf_counter() - f0)
-        for gv in gvs:
-            stage_ms += gv.stage_times()
-    ctx.synchronize()
+        if batch is None:
+            for gv in gvs:
+                stage_ms += gv.stage_times()
+    sync()
     elapsed = time.perf_counter() - t_start
     ctx.set_profiling(False)
     samples_per_frame = EYES * W * HGT * A
     value = samples_per_frame * args.steps / elapsed / 1e6
     lat_ms = np.array(lat) * 1e3
-    stage_ms /= args.steps * EYES  # average per launch (one launch per eye)
+    if batch is not None:
+        # per-stage times of the same views rendered one at a time (a graph replay records
+        # no per-stage events)
+        ctx.set_profiling(True)
+        for _ in range(5):
+            for gv in gvs:
+                gv.render(gs)
+            ctx.synchronize()
+            for gv in gvs:
+                stage_ms += gv.stage_times()
+        ctx.set_profiling(False)
+        stage_ms /= 5 * EYES
+    else:
+        stage_ms /= args.steps * EYES  # average per launch (one launch per eye)
     stats = [gv.stats() for gv in gvs]
     survivors = sum(s.survivorPairs for s in stats) / EYES
     entries = sum(s.listEntries for s in stats) / EYES
@@ -461,10 +494,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.shading == 32 else "f32 intersect, f16 shade",
         "data": "synthetic: seeded room scene (1,024 meshes x 256 triangles), grid samples with barrel distortion",
         "config": {"workload": f"stereo 2x1080x1200, MSAA 4, 262,144 triangles, {args.pipeline} pipeline",
-                   "subsamples_per_frame": samples_per_frame, "pipeline": args.pipeline},
+                   "subsamples_per_frame": samples_per_frame, "pipeline": args.pipeline,
+                   "shading_bits": args.shading, "launch_shape": shape.as_dict() if shape else None},
         "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage_ms)},
         "list_entries_per_eye": int(entries),
         "survivor_pairs_per_eye": int(survivors),

@@ -436,7 +436,7 @@ def render_views(scene, views, cameras=None, jitters=None):
 
 class LaunchShape(ctypes.Structure):
     _fields_ = [("largeGroups", c_uint32), ("hugeGroups", c_uint32), ("giantGroups", c_uint32),
-                ("walkGroups", c_uint32), ("xcdSwizzle", c_uint32)]
+                ("walkGroups", c_uint32), ("xcdSwizzle", c_uint32), ("streams", c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -9,14 +9,17 @@
 // page-locked block with one copy, so the frame can be captured once as a HIP graph
 // (copy, optional BVH refit, counter clear, the chain) and replayed with new cameras.
 // The counters are cleared by a kernel, not a memset, so the captured graph holds none.
-// The launch shape (persistent workgroup counts, XCD tile swizzle) is the autotuner's
-// to pick; none of it changes a result.
+// The views can be dealt to up to four chains, each on its own stream (forked from the
+// batch's stream and joined back, inside the graph too), so one chain's intersect
+// overlaps another's traversal. The launch shape (streams, persistent workgroup counts,
+// XCD tile swizzle) is the autotuner's to pick; none of it changes a result.
 
 #include "gpu_internal.h"
 #include "launch_args.h"
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <string.h>
 
 using namespace hvvr;
@@ -52,22 +55,35 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 } // namespace
 
-struct hvvr_gpu_batch {
-    hvvr_gpu_context* ctx = nullptr;
+namespace {
+
+// The views of one stream's chain: their frame structure and argument block.
+struct Chain {
     std::vector<hvvr_gpu_view*> views;
-    hipStream_t stream = nullptr;
-    hipEvent_t start = nullptr, stop = nullptr, sceneFence = nullptr;
-    LaunchShape shape;
-    bool graphEnabled = false, refitEveryFrame = false, inFlight = false;
-    // the frame's structure: groups, units and the argument-block layout
+    hipStream_t stream = nullptr; // null: the batch's own stream
     std::vector<Group> groups;
     std::vector<uint32_t> remapMembers;
     uint32_t uFrustum = 0, uBlock = 0, uTile = 0, uRemap = 0, maxTiles = 0;
     size_t offCounters = 0, offFrustum = 0, offBlock = 0, offTile = 0, offWalk = 0, offRemap = 0;
     size_t sFrustum = 0, sBlock = 0, sTile = 0, sRemap = 0, argBytes = 0;
-    std::vector<uint64_t> signature; // what the layout and a captured graph were built for
-    uint8_t* hostArgs = nullptr;     // page-locked
+    std::vector<uint32_t> index; // position of each view in the batch (its camera)
+    uint8_t* hostArgs = nullptr; // page-locked
     DeviceBuffer<uint8_t> devArgs;
+    hipEvent_t joined = nullptr;
+};
+
+} // namespace
+
+struct hvvr_gpu_batch {
+    hvvr_gpu_context* ctx = nullptr;
+    std::vector<hvvr_gpu_view*> views;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr, sceneFence = nullptr, fork = nullptr;
+    LaunchShape shape;
+    bool graphEnabled = false, refitEveryFrame = false, inFlight = false;
+    std::vector<std::unique_ptr<Chain>> chains;
+    std::vector<hipStream_t> extraStreams; // chain streams beyond the batch's own
+    std::vector<uint64_t> signature;      // what the chains and a captured graph were built for
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> graphSignature;
@@ -79,14 +95,17 @@ struct hvvr_gpu_batch {
             if (v->lastStream == stream)
                 v->lastStream = nullptr;
         dropGraph();
-        if (hostArgs)
-            (void)hipHostFree(hostArgs);
+        dropChains();
+        for (hipStream_t st : extraStreams)
+            (void)hipStreamDestroy(st);
         if (start)
             (void)hipEventDestroy(start);
         if (stop)
             (void)hipEventDestroy(stop);
         if (sceneFence)
             (void)hipEventDestroy(sceneFence);
+        if (fork)
+            (void)hipEventDestroy(fork);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -98,6 +117,15 @@ struct hvvr_gpu_batch {
         exec = nullptr;
         graph = nullptr;
         graphSignature.clear();
+    }
+    void dropChains() {
+        for (auto& c : chains) {
+            if (c->hostArgs)
+                (void)hipHostFree(c->hostArgs);
+            if (c->joined)
+                (void)hipEventDestroy(c->joined);
+        }
+        chains.clear();
     }
 };
 
@@ -119,7 +147,7 @@ int batchFail(const std::string& msg, int code = HVVR_GPU_ERR_INVALID) {
 std::vector<uint64_t> frameSignature(const hvvr_gpu_batch* b, const hvvr_gpu_scene* s) {
     std::vector<uint64_t> sig = {uint64_t(uintptr_t(s)), uint64_t(b->refitEveryFrame), b->shape.largeGroups,
                                  b->shape.hugeGroups,    b->shape.giantGroups,          b->shape.walkGroups,
-                                 b->shape.xcdSwizzle};
+                                 b->shape.xcdSwizzle,    b->shape.streams};
     for (const hvvr_gpu_view* v : b->views) {
         sig.push_back(v->msaaRate);
         sig.push_back(v->halfShading ? 1 : 0);
@@ -129,85 +157,107 @@ std::vector<uint64_t> frameSignature(const hvvr_gpu_batch* b, const hvvr_gpu_sce
     return sig;
 }
 
-// groups, unit counts and the argument-block layout
-int planLayout(hvvr_gpu_batch* b) {
-    const uint32_t n = uint32_t(b->views.size());
+// one chain's groups, unit counts and argument-block layout
+int planChain(Chain& c) {
+    const uint32_t n = uint32_t(c.views.size());
     std::map<std::pair<uint32_t, bool>, Group> byKind;
-    b->remapMembers.clear();
-    b->uFrustum = b->uBlock = b->uTile = b->uRemap = b->maxTiles = 0;
+    c.remapMembers.clear();
+    c.uFrustum = c.uBlock = c.uTile = c.uRemap = c.maxTiles = 0;
     for (uint32_t i = 0; i < n; i++) {
-        const hvvr_gpu_view* v = b->views[i];
+        const hvvr_gpu_view* v = c.views[i];
         Group& g = byKind[{v->msaaRate, v->halfShading}];
         g.msaaRate = v->msaaRate;
         g.half = v->halfShading;
         g.members.push_back(i);
         g.units += v->tileCount;
-        b->uFrustum += frustumSetupUnits(v);
-        b->uBlock += v->blockCount;
-        b->uTile += v->tileCount;
-        b->maxTiles = std::max(b->maxTiles, v->tileCount);
+        c.uFrustum += frustumSetupUnits(v);
+        c.uBlock += v->blockCount;
+        c.uTile += v->tileCount;
+        c.maxTiles = std::max(c.maxTiles, v->tileCount);
         if (v->remap.ptr) {
-            b->remapMembers.push_back(i);
-            b->uRemap += remapUnits(v);
+            c.remapMembers.push_back(i);
+            c.uRemap += remapUnits(v);
         }
     }
-    b->groups.clear();
+    c.groups.clear();
     for (auto& kv : byKind)
-        b->groups.push_back(kv.second);
+        c.groups.push_back(kv.second);
     size_t at = 0;
     auto take = [&](size_t bytes) {
         size_t off = at;
         at = align256(at + bytes);
         return off;
     };
-    b->offCounters = take(n * sizeof(uint64_t*));
-    b->offFrustum = take(n * sizeof(FrustumSetupArgs));
-    b->offBlock = take(n * sizeof(BlockTraverseArgs));
-    b->offTile = take(4 * n * sizeof(TileListArgs));
-    b->offWalk = take(n * sizeof(TileWalkArgs));
-    for (Group& g : b->groups) {
+    c.offCounters = take(n * sizeof(uint64_t*));
+    c.offFrustum = take(n * sizeof(FrustumSetupArgs));
+    c.offBlock = take(n * sizeof(BlockTraverseArgs));
+    c.offTile = take(4 * n * sizeof(TileListArgs));
+    c.offWalk = take(n * sizeof(TileWalkArgs));
+    for (Group& g : c.groups) {
         g.intersectOff = take(g.members.size() * sizeof(IntersectArgs));
         g.resolveOff = take(g.members.size() * sizeof(ResolveArgs));
     }
-    b->offRemap = take(b->remapMembers.size() * sizeof(RemapArgs) + 1);
-    b->sFrustum = take((n + 1) * 4);
-    b->sBlock = take((n + 1) * 4);
-    b->sTile = take((n + 1) * 4);
-    for (Group& g : b->groups)
+    c.offRemap = take(c.remapMembers.size() * sizeof(RemapArgs) + 1);
+    c.sFrustum = take((n + 1) * 4);
+    c.sBlock = take((n + 1) * 4);
+    c.sTile = take((n + 1) * 4);
+    for (Group& g : c.groups)
         g.startOff = take((g.members.size() + 1) * 4);
-    b->sRemap = take((b->remapMembers.size() + 1) * 4);
-    if (at > b->argBytes || !b->hostArgs) {
-        if (b->hostArgs)
-            (void)hipHostFree(b->hostArgs);
-        b->hostArgs = nullptr;
-        BATCH_HIP(hipHostMalloc((void**)&b->hostArgs, at, hipHostMallocDefault));
-        hipError_t e = b->devArgs.resize(at);
-        if (e != hipSuccess)
-            return batchFail(std::string("batch argument block: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
-        b->argBytes = at;
-        b->dropGraph(); // its copy node names the old buffers
+    c.sRemap = take((c.remapMembers.size() + 1) * 4);
+    BATCH_HIP(hipHostMalloc((void**)&c.hostArgs, at, hipHostMallocDefault));
+    hipError_t e = c.devArgs.resize(at);
+    if (e != hipSuccess)
+        return batchFail(std::string("batch argument block: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+    c.argBytes = at;
+    BATCH_HIP(hipEventCreateWithFlags(&c.joined, hipEventDisableTiming));
+    return HVVR_GPU_OK;
+}
+
+// the views dealt to streamCount chains (round-robin, so large and small views spread),
+// each chain planned; chain 0 runs on the batch's stream
+int planChains(hvvr_gpu_batch* b) {
+    b->dropGraph(); // its nodes name the old chains' buffers
+    b->dropChains();
+    const uint32_t k = std::min<uint32_t>(b->shape.streams, uint32_t(b->views.size()));
+    while (b->extraStreams.size() + 1 < k) {
+        hipStream_t st = nullptr;
+        BATCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->extraStreams.push_back(st);
+    }
+    for (uint32_t c = 0; c < k; c++) {
+        auto chain = std::make_unique<Chain>();
+        chain->stream = c == 0 ? b->stream : b->extraStreams[c - 1];
+        for (uint32_t i = c; i < b->views.size(); i += k) {
+            chain->views.push_back(b->views[i]);
+            chain->index.push_back(i);
+        }
+        int rc = planChain(*chain);
+        b->chains.push_back(std::move(chain));
+        if (rc)
+            return rc;
     }
     return HVVR_GPU_OK;
 }
 
-// this frame's arguments into the page-locked block
-void fillArgs(hvvr_gpu_batch* b, const hvvr_gpu_scene* s, const matrix4x4* m) {
-    const uint32_t n = uint32_t(b->views.size());
-    uint8_t* h = b->hostArgs;
-    auto counters = reinterpret_cast<uint64_t**>(h + b->offCounters);
-    auto frustum = reinterpret_cast<FrustumSetupArgs*>(h + b->offFrustum);
-    auto block = reinterpret_cast<BlockTraverseArgs*>(h + b->offBlock);
-    auto tile = reinterpret_cast<TileListArgs*>(h + b->offTile);
-    auto walk = reinterpret_cast<TileWalkArgs*>(h + b->offWalk);
-    auto sF = reinterpret_cast<uint32_t*>(h + b->sFrustum);
-    auto sB = reinterpret_cast<uint32_t*>(h + b->sBlock);
-    auto sT = reinterpret_cast<uint32_t*>(h + b->sTile);
+// this frame's arguments of one chain into its page-locked block
+void fillArgs(Chain& c, const hvvr_gpu_scene* s, const matrix4x4* m) {
+    const uint32_t n = uint32_t(c.views.size());
+    uint8_t* h = c.hostArgs;
+    auto counters = reinterpret_cast<uint64_t**>(h + c.offCounters);
+    auto frustum = reinterpret_cast<FrustumSetupArgs*>(h + c.offFrustum);
+    auto block = reinterpret_cast<BlockTraverseArgs*>(h + c.offBlock);
+    auto tile = reinterpret_cast<TileListArgs*>(h + c.offTile);
+    auto walk = reinterpret_cast<TileWalkArgs*>(h + c.offWalk);
+    auto sF = reinterpret_cast<uint32_t*>(h + c.sFrustum);
+    auto sB = reinterpret_cast<uint32_t*>(h + c.sBlock);
+    auto sT = reinterpret_cast<uint32_t*>(h + c.sTile);
     sF[0] = sB[0] = sT[0] = 0;
     for (uint32_t i = 0; i < n; i++) {
-        hvvr_gpu_view* v = b->views[i];
+        hvvr_gpu_view* v = c.views[i];
+        const matrix4x4& mi = m[c.index[i]];
         v->cull.enabled = 0;
         counters[i] = v->counters.ptr;
-        frustum[i] = frustumSetupArgs(v, m[i]);
+        frustum[i] = frustumSetupArgs(v, mi);
         block[i] = blockTraverseArgs(v, s);
         for (int level = 0; level < 4; level++)
             tile[level * n + i] = tileListArgs(v, s, level);
@@ -216,55 +266,71 @@ void fillArgs(hvvr_gpu_batch* b, const hvvr_gpu_scene* s, const matrix4x4* m) {
         sB[i + 1] = sB[i] + v->blockCount;
         sT[i + 1] = sT[i] + v->tileCount;
     }
-    for (const Group& g : b->groups) {
+    for (const Group& g : c.groups) {
         auto ia = reinterpret_cast<IntersectArgs*>(h + g.intersectOff);
         auto ra = reinterpret_cast<ResolveArgs*>(h + g.resolveOff);
         auto st = reinterpret_cast<uint32_t*>(h + g.startOff);
         st[0] = 0;
         for (size_t k = 0; k < g.members.size(); k++) {
-            const hvvr_gpu_view* v = b->views[g.members[k]];
-            ia[k] = intersectArgs(v, s, m[g.members[k]]);
-            ra[k] = resolveArgs(v, s, m[g.members[k]]);
+            const hvvr_gpu_view* v = c.views[g.members[k]];
+            const matrix4x4& mi = m[c.index[g.members[k]]];
+            ia[k] = intersectArgs(v, s, mi);
+            ra[k] = resolveArgs(v, s, mi);
             st[k + 1] = st[k] + v->tileCount;
         }
     }
-    auto rm = reinterpret_cast<RemapArgs*>(h + b->offRemap);
-    auto sR = reinterpret_cast<uint32_t*>(h + b->sRemap);
+    auto rm = reinterpret_cast<RemapArgs*>(h + c.offRemap);
+    auto sR = reinterpret_cast<uint32_t*>(h + c.sRemap);
     sR[0] = 0;
-    for (size_t k = 0; k < b->remapMembers.size(); k++) {
-        const hvvr_gpu_view* v = b->views[b->remapMembers[k]];
+    for (size_t k = 0; k < c.remapMembers.size(); k++) {
+        const hvvr_gpu_view* v = c.views[c.remapMembers[k]];
         rm[k] = remapArgs(v);
         sR[k + 1] = sR[k] + remapUnits(v);
     }
 }
 
-// the frame's work on the batch stream (eagerly, or into a capture)
-void enqueueFrame(hvvr_gpu_batch* b, hvvr_gpu_scene* s) {
-    const uint32_t n = uint32_t(b->views.size());
-    hipStream_t st = b->stream;
-    uint8_t* d = b->devArgs.ptr;
-    (void)hipMemcpyAsync(d, b->hostArgs, b->argBytes, hipMemcpyHostToDevice, st);
-    if (b->refitEveryFrame)
-        launchSceneRefit(s, st);
-    launchClearCountersBatch(reinterpret_cast<uint64_t* const*>(d + b->offCounters), n, kCounterSlots, st);
+// one chain's work on its stream
+void enqueueChain(const hvvr_gpu_batch* b, Chain& c) {
+    const uint32_t n = uint32_t(c.views.size());
+    hipStream_t st = c.stream;
+    uint8_t* d = c.devArgs.ptr;
+    (void)hipMemcpyAsync(d, c.hostArgs, c.argBytes, hipMemcpyHostToDevice, st);
+    launchClearCountersBatch(reinterpret_cast<uint64_t* const*>(d + c.offCounters), n, kCounterSlots, st);
     auto u32 = [&](size_t off) { return reinterpret_cast<const uint32_t*>(d + off); };
-    launchFrustumSetupBatch(reinterpret_cast<const FrustumSetupArgs*>(d + b->offFrustum), u32(b->sFrustum), n,
-                            b->uFrustum, st);
-    launchBlockTraverseBatch(reinterpret_cast<const BlockTraverseArgs*>(d + b->offBlock), u32(b->sBlock), n,
-                             b->uBlock, st);
-    launchTileListsBatch(reinterpret_cast<const TileListArgs*>(d + b->offTile), u32(b->sTile), n, b->uTile, b->shape,
+    launchFrustumSetupBatch(reinterpret_cast<const FrustumSetupArgs*>(d + c.offFrustum), u32(c.sFrustum), n,
+                            c.uFrustum, st);
+    launchBlockTraverseBatch(reinterpret_cast<const BlockTraverseArgs*>(d + c.offBlock), u32(c.sBlock), n, c.uBlock,
+                             st);
+    launchTileListsBatch(reinterpret_cast<const TileListArgs*>(d + c.offTile), u32(c.sTile), n, c.uTile, b->shape,
                          st);
-    launchTileWalkBatch(reinterpret_cast<const TileWalkArgs*>(d + b->offWalk), n, b->maxTiles, b->shape, st);
-    for (const Group& g : b->groups) {
+    launchTileWalkBatch(reinterpret_cast<const TileWalkArgs*>(d + c.offWalk), n, c.maxTiles, b->shape, st);
+    for (const Group& g : c.groups) {
         const uint32_t gn = uint32_t(g.members.size());
         launchIntersectBatch(int(g.msaaRate), reinterpret_cast<const IntersectArgs*>(d + g.intersectOff),
                              u32(g.startOff), gn, g.units, b->shape, st);
         launchResolveBatch(int(g.msaaRate), g.half, reinterpret_cast<const ResolveArgs*>(d + g.resolveOff),
                            u32(g.startOff), gn, g.units, b->shape, st);
     }
-    if (!b->remapMembers.empty())
-        launchRemapBatch(reinterpret_cast<const RemapArgs*>(d + b->offRemap), u32(b->sRemap),
-                         uint32_t(b->remapMembers.size()), b->uRemap, st);
+    if (!c.remapMembers.empty())
+        launchRemapBatch(reinterpret_cast<const RemapArgs*>(d + c.offRemap), u32(c.sRemap),
+                         uint32_t(c.remapMembers.size()), c.uRemap, st);
+}
+
+// the frame on the batch's stream (eagerly, or into a capture): the optional refit, then
+// the chains, forked onto their streams and joined back
+void enqueueFrame(hvvr_gpu_batch* b, hvvr_gpu_scene* s) {
+    if (b->refitEveryFrame)
+        launchSceneRefit(s, b->stream);
+    if (b->chains.size() > 1)
+        (void)hipEventRecord(b->fork, b->stream);
+    for (size_t k = 1; k < b->chains.size(); k++)
+        (void)hipStreamWaitEvent(b->chains[k]->stream, b->fork, 0);
+    for (auto& c : b->chains)
+        enqueueChain(b, *c);
+    for (size_t k = 1; k < b->chains.size(); k++) {
+        (void)hipEventRecord(b->chains[k]->joined, b->chains[k]->stream);
+        (void)hipStreamWaitEvent(b->stream, b->chains[k]->joined, 0);
+    }
 }
 
 int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
@@ -276,7 +342,7 @@ int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
         if (v->dof || v->foveated)
             return batchFail("batched views must be pinhole grid views (no depth of field, not foveated)");
     BATCH_HIP(hipSetDevice(b->ctx->device));
-    // the previous frame has read the argument block before it is rewritten
+    // the previous frame has read the argument blocks before they are rewritten
     if (b->inFlight)
         BATCH_HIP(hipStreamSynchronize(b->stream));
     b->inFlight = false;
@@ -292,14 +358,15 @@ int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
     }
     std::vector<uint64_t> sig = frameSignature(b, s);
     if (sig != b->signature) {
-        int rc = planLayout(b);
+        int rc = planChains(b);
         if (rc)
             return rc;
         b->signature = sig;
     }
     std::vector<matrix4x4> m(b->views.size());
     memcpy(m.data(), m16, m.size() * sizeof(matrix4x4));
-    fillArgs(b, s, m.data());
+    for (auto& c : b->chains)
+        fillArgs(*c, s, m.data());
     BATCH_HIP(hipEventRecord(b->start, b->stream));
     if (b->graphEnabled) {
         if (!b->exec || b->graphSignature != sig) {
@@ -366,7 +433,8 @@ int hvvr_gpu_batch_create(hvvr_gpu_context* ctx, hvvr_gpu_view* const* views, ui
     bool ok = hipSetDevice(ctx->device) == hipSuccess &&
               hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&b->start) == hipSuccess && hipEventCreate(&b->stop) == hipSuccess &&
-              hipEventCreateWithFlags(&b->sceneFence, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&b->sceneFence, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         delete b;
         return batchFail("batch stream or event creation failed", HVVR_GPU_ERR_HIP);
@@ -404,13 +472,14 @@ int hvvr_gpu_batch_set_launch_shape(hvvr_gpu_batch* b, const hvvr_gpu_launch_sha
         return batchFail("null argument");
     auto inRange = [](uint32_t x) { return x >= 1 && x <= 65536; };
     if (!inRange(shape->largeGroups) || !inRange(shape->hugeGroups) || !inRange(shape->giantGroups) ||
-        !inRange(shape->walkGroups) || shape->xcdSwizzle > 1)
-        return batchFail("launch shape: workgroup counts in [1, 65536], xcdSwizzle 0 or 1");
+        !inRange(shape->walkGroups) || shape->xcdSwizzle > 1 || shape->streams < 1 || shape->streams > 4)
+        return batchFail("launch shape: workgroup counts in [1, 65536], xcdSwizzle 0 or 1, streams 1..4");
     b->shape.largeGroups = shape->largeGroups;
     b->shape.hugeGroups = shape->hugeGroups;
     b->shape.giantGroups = shape->giantGroups;
     b->shape.walkGroups = shape->walkGroups;
     b->shape.xcdSwizzle = shape->xcdSwizzle;
+    b->shape.streams = shape->streams;
     return HVVR_GPU_OK;
 }
 
@@ -422,6 +491,7 @@ int hvvr_gpu_batch_get_launch_shape(hvvr_gpu_batch* b, hvvr_gpu_launch_shape* sh
     shape->giantGroups = b->shape.giantGroups;
     shape->walkGroups = b->shape.walkGroups;
     shape->xcdSwizzle = b->shape.xcdSwizzle;
+    shape->streams = b->shape.streams;
     return HVVR_GPU_OK;
 }
 
@@ -474,12 +544,13 @@ int hvvr_gpu_batch_autotune(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* c
         return rc;
     // coordinate descent over the shape's fields
     const std::vector<uint32_t> largeC = {320, 640, 1280, 2560}, hugeC = {256, 512, 1024}, giantC = {128, 256, 512},
-                                walkC = {256, 1024, 2048}, swizzleC = {0, 1};
+                                walkC = {256, 1024, 2048}, swizzleC = {0, 1}, streamsC = {1, 2, 4};
     struct Field {
         uint32_t LaunchShape::*member;
         const std::vector<uint32_t>* values;
     };
-    const Field fields[] = {{&LaunchShape::xcdSwizzle, &swizzleC}, {&LaunchShape::largeGroups, &largeC},
+    const Field fields[] = {{&LaunchShape::streams, &streamsC},        {&LaunchShape::xcdSwizzle, &swizzleC},
+                            {&LaunchShape::largeGroups, &largeC},
                             {&LaunchShape::hugeGroups, &hugeC},    {&LaunchShape::giantGroups, &giantC},
                             {&LaunchShape::walkGroups, &walkC}};
     for (const Field& f : fields) {

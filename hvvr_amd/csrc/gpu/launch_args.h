@@ -122,14 +122,15 @@ RemapArgs remapArgs(const hvvr_gpu_view* v);
 inline uint32_t frustumSetupUnits(const hvvr_gpu_view* v) { return (v->tileCount + v->blockCount + 127) / 128; }
 inline uint32_t remapUnits(const hvvr_gpu_view* v) { return ((v->width + 15) / 16) * ((v->height + 7) / 8); }
 
-// Launch shape of a batch: persistent workgroups of the queued tile-list configurations
-// and the walk, and whether grid-indexed tile kernels map consecutive workgroups to one
-// XCD (workgroups are dealt to the eight XCDs in turn; with the swizzle, XCD x takes a
-// contiguous run of tiles, which share BVH nodes and triangles in its L2). None of these
-// changes a result.
+// Launch shape of a batch: streams, persistent workgroups of the queued tile-list
+// configurations and the walk, and whether grid-indexed tile kernels map consecutive
+// workgroups to one XCD (workgroups are dealt to the eight XCDs in turn; with the swizzle,
+// XCD x takes a contiguous run of tiles, which share BVH nodes and triangles in its L2).
+// None of these changes a result.
 struct LaunchShape {
     uint32_t largeGroups = 1280, hugeGroups = 512, giantGroups = 256, walkGroups = 1024;
     uint32_t xcdSwizzle = 0;
+    uint32_t streams = 1; // batches: chains the views are dealt to, each on its own stream
 };
 
 // batch launchers; `all` = device array of per-view arguments

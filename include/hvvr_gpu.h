@@ -124,6 +124,8 @@ typedef struct {
     uint32_t largeGroups, hugeGroups, giantGroups; /* persistent workgroups of the queued tile-list configurations */
     uint32_t walkGroups;                           /* ... and of the LDS-stack walk */
     uint32_t xcdSwizzle;                           /* 1: each XCD takes a contiguous run of tiles */
+    uint32_t streams;                              /* 1..4 chains the views are dealt to, each on its own
+                                                      stream (forked and joined inside the frame) */
 } hvvr_gpu_launch_shape;
 int hvvr_gpu_batch_create(hvvr_gpu_context* ctx, hvvr_gpu_view* const* views, uint32_t count, hvvr_gpu_batch** out);
 int hvvr_gpu_batch_destroy(hvvr_gpu_batch* batch);

@@ -50,7 +50,8 @@ def _same(a, b, what):
         assert ok, f"{what}: {k} differs"
 
 
-def test_batch_matches_solo_renders(H, ctx, scene2):
+@pytest.mark.parametrize("streams", [1, 3])
+def test_batch_matches_solo_renders(H, ctx, scene2, streams):
     gs = H.GpuScene(ctx, scene2)
     views = _views(H, ctx)
     cams = [_moved(v.view.camera_to_world, i) for i, v in enumerate(views)]
@@ -59,6 +60,9 @@ def test_batch_matches_solo_renders(H, ctx, scene2):
         v.render(gs, m)
         solo.append(_results(v))
     batch = H.GpuBatch(ctx, views)
+    shape = batch.launch_shape()
+    shape.streams = streams
+    batch.set_launch_shape(shape)
     batch.render(gs, cams)
     ms = batch.synchronize()
     assert ms > 0
@@ -71,6 +75,9 @@ def test_batch_graph_replays_match_eager(H, ctx, scene2):
     gs = H.GpuScene(ctx, scene2)
     va, vb = _views(H, ctx), _views(H, ctx)
     graph, eager = H.GpuBatch(ctx, va), H.GpuBatch(ctx, vb)
+    shape = graph.launch_shape()
+    shape.streams = 2  # the captured graph forks onto a second stream and joins back
+    graph.set_launch_shape(shape)
     graph.set_graph(True)
     for k in range(5):
         cams = [_moved(v.view.camera_to_world, k + i) for i, v in enumerate(va)]
