@@ -3,7 +3,7 @@
 /*
  * CPU oracle for the HVVR raycasting hot path. TEST INFRASTRUCTURE ONLY: nothing in
  * the product (hvvr_amd/, include/) links, loads or calls this code; only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the checker.
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, to check results.
  *
  * It restates, in plain C99 and scalar form, the reference algorithms the GPU
  * kernels implement (citations are relative to the reference repository root,
@@ -26,7 +26,8 @@
  * (_mm256_fmadd_ps and the "n FMA" annotations of prim_tests.h); correctly rounded
  * division and sqrt; IEEE denormals. The reference's own binaries run with fast-math
  * and FTZ/DAZ on NVIDIA hardware and cannot be run here, so agreement with them is
- * not claimed bit for bit; each function's pinning is listed in DESIGN.md.
+ * not claimed bit for bit; each function is pinned instead by closed forms, brute
+ * force and invariants (tests/test_oracle_pins.py and the tests named in each file).
  *
  * All layouts below are byte-identical to the product's (include/hvvr/types.h),
  * which a CPU test checks field by field.

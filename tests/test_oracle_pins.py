@@ -1,6 +1,6 @@
 """Pins the CPU oracle against mathematics that fixes its results independently:
 closed forms, brute force on small inputs and invariants. (The reference ships no
-golden vectors or tests; see DESIGN.md for what each check pins.)"""
+golden vectors or tests; each test's docstring says what it pins.)"""
 
 import itertools
 
