@@ -507,7 +507,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                     const bool isChild = lane < nChildren;
                     const uint32_t childKey = isChild ? sh.recKey[first + lane] : 0u;
                     const uint32_t childMeta = isChild ? sh.recMeta[first + lane] : 0u;
-                    const uint32_t liveBits = uint32_t(ballot(isChild && metaMask(childMeta) != 0u));
+                    const bool live = isChild && metaMask(childMeta) != 0u;
+                    const uint32_t liveBits = uint32_t(ballot(live));
                     if (liveBits != 0u) {
                         const uint32_t nLive = uint32_t(__builtin_popcount(liveBits));
                         if (st.size + nLive - 1u >= limit) {
@@ -517,22 +518,23 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                         // The walk pops the top right after these inserts. When the closest
                         // child would land on top (ties go to the newest insert), visit it
                         // directly and insert only its siblings: the stack ends the same.
-                        // (the newest of the closest live children; the tile has no NaN keys)
-                        const uint32_t last = 31u - uint32_t(__builtin_clz(liveBits));
-                        uint32_t best = last;
-                        float bestKey = bitsf(readLane(childKey, last));
-                        for (uint32_t j = last; j-- > 0u;) {
-                            const float kj = bitsf(readLane(childKey, j));
-                            if (((liveBits >> j) & 1u) && kj < bestKey) {
-                                bestKey = kj;
-                                best = j;
-                            }
-                        }
+                        // The newest of the closest live children (the tile has no NaN
+                        // keys): the least key over the four child lanes (two quad DPP
+                        // steps), then the highest live lane holding it.
+                        const float ck = live ? bitsf(childKey) : INFINITY;
+                        float least = fminf(ck, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
+                                                    0, int(fbits(ck)), 0xb1, 0xf, 0xf, false)))); // lane ^ 1
+                        least = fminf(least, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
+                                                 0, int(fbits(least)), 0x4e, 0xf, 0xf, false)))); // lane ^ 2
+                        const uint32_t atLeast = uint32_t(ballot(live && ck == least));
+                        const uint32_t best = 31u - uint32_t(__builtin_clz(atLeast));
+                        const float bestKey = bitsf(readLane(fbits(least), 0));
                         const bool take = st.size == 0u || bestKey <= stackTopKey(st);
                         const uint32_t entry = stackEntry<Cfg>(first + lane, childMeta);
-                        for (uint32_t j = 0; j < nChildren; j++)
-                            if (((liveBits >> j) & 1u) && !(take && j == best))
-                                stackInsert(st, bitsf(readLane(childKey, j)), readLane(entry, j), lane);
+                        for (uint32_t sib = liveBits & ~(take ? 1u << best : 0u); sib != 0u; sib &= sib - 1u) {
+                            const uint32_t j = uint32_t(__builtin_ctz(sib));
+                            stackInsert(st, bitsf(readLane(childKey, j)), readLane(entry, j), lane);
+                        }
                         const uint32_t depth = st.size + (take ? 1u : 0u);
                         deepest = depth > deepest ? depth : deepest;
                         if (take) {
