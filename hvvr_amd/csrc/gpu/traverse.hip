@@ -333,19 +333,6 @@ __device__ inline ChildBox loadChildBox(const BVHNode* __restrict__ nodes, uint3
     return x;
 }
 
-// number of triangles in the leaves `leaf` of `node`
-__device__ inline uint32_t leafListLength(const BVHNode* __restrict__ nodes, uint32_t node, uint32_t leaf) {
-    const uint32_t* d = nodes[node].data;
-    uint32_t n = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint32_t t0 = d[k], t1 = d[k + 1];
-        if ((leaf & (1u << k)) && t1 > t0)
-            n += t1 - t0;
-    }
-    return n;
-}
-
 
 // Builds one tile's list; false when the tile does not fit this configuration (its count
 // and offset are then zero).
@@ -564,17 +551,43 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
     }
 
     // ---- phase C: emit ----
+    // The first 64 emitted records keep their node's leaf ranges and prefix in registers
+    // from the length pass to the write pass; later ones (dense tiles) are read again.
     uint32_t total = 0;
+    uint32_t d0[5] = {0u, 0u, 0u, 0u, 0u};
+    uint32_t leaf0 = 0, len0 = 0, incl0 = 0;
     for (uint32_t base = 0; base < nEmit; base += 64) {
         const uint32_t i = base + lane;
         uint32_t len = 0;
         if (i < nEmit) {
             const uint32_t rec = sh.order[i];
             const uint32_t meta = sh.recMeta[rec];
-            len = leafListLength(nodes, sh.recNode[rec], metaMask(meta) & metaLeaf(meta));
-            sh.recKey[i] = len;
+            const uint32_t leaf = metaMask(meta) & metaLeaf(meta);
+            const uint32_t* d = nodes[sh.recNode[rec]].data;
+            uint32_t dk[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+                dk[k] = d[k];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((leaf & (1u << k)) && dk[k + 1] > dk[k])
+                    len += dk[k + 1] - dk[k];
+            if (base == 0) {
+#pragma unroll
+                for (int k = 0; k < 5; k++)
+                    d0[k] = dk[k];
+                leaf0 = leaf;
+            } else {
+                sh.recKey[i] = len;
+            }
         }
-        total += waveSum(len);
+        if (base == 0) {
+            len0 = len;
+            incl0 = waveInclusiveScan(len, lane);
+            total += __shfl(incl0, 63);
+        } else {
+            total += waveSum(len);
+        }
     }
     unsigned long long start = 0;
     if (lane == 0)
@@ -591,8 +604,18 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
     }
     if (!fits)
         return true;
-    uint32_t run = uint32_t(start);
-    for (uint32_t base = 0; base < nEmit; base += 64) {
+    {
+        uint32_t at = uint32_t(start) + incl0 - len0;
+        if (len0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (leaf0 & (1u << k))
+                    for (uint32_t t = d0[k]; t < d0[k + 1]; t++)
+                        g.triOut[at++] = t;
+        }
+    }
+    uint32_t run = uint32_t(start) + (nEmit ? uint32_t(__shfl(incl0, 63)) : 0u);
+    for (uint32_t base = 64; base < nEmit; base += 64) {
         const uint32_t i = base + lane;
         const uint32_t len = i < nEmit ? sh.recKey[i] : 0u;
         const uint32_t incl = waveInclusiveScan(len, lane);
