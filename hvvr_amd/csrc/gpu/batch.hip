@@ -259,8 +259,12 @@ void fillArgs(Chain& c, const hvvr_gpu_scene* s, const matrix4x4* m) {
         counters[i] = v->counters.ptr;
         frustum[i] = frustumSetupArgs(v, mi);
         block[i] = blockTraverseArgs(v, s);
-        for (int level = 0; level < 4; level++)
+        for (int level = 0; level < 4; level++) {
             tile[level * n + i] = tileListArgs(v, s, level);
+            // the chain's persistent workgroups take every view's queue in turn: one
+            // scratch (its first view's) serves them
+            tile[level * n + i].recScratch = c.views[0]->recScratch.ptr;
+        }
         walk[i] = tileWalkArgs(v, s);
         sF[i + 1] = sF[i] + frustumSetupUnits(v);
         sB[i + 1] = sB[i] + v->blockCount;
@@ -543,7 +547,7 @@ int hvvr_gpu_batch_autotune(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* c
     if (rc)
         return rc;
     // coordinate descent over the shape's fields
-    const std::vector<uint32_t> largeC = {320, 640, 1280, 2560}, hugeC = {256, 512, 1024}, giantC = {128, 256, 512},
+    const std::vector<uint32_t> largeC = {640, 1280, 2560}, hugeC = {256, 512, 1024}, giantC = {128, 256, 512},
                                 walkC = {256, 1024, 2048}, swizzleC = {0, 1}, streamsC = {1, 2, 4};
     struct Field {
         uint32_t LaunchShape::*member;

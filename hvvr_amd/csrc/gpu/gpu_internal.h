@@ -135,6 +135,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<hvvr::FrustumPre> tilePre, blockPre;
     hvvr::DeviceBuffer<hvvr::StackEntry> blockStacks;   // [blockCount][64]
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
+    hvvr::DeviceBuffer<uint32_t> recScratch;            // record node indices and emission order of the
+                                                        // queued tile-list configurations, per workgroup
     hvvr::DeviceBuffer<uint32_t> tileQueues;            // [4][tileCount] tiles queued for the large, huge
                                                         // and giant LDS configurations and the walk
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length

@@ -46,6 +46,7 @@ struct TileListArgs {
     uint32_t nextCounter;    // and of `nextQueue`
     unsigned long long* counters;
     uint32_t stackLimit;
+    uint32_t* recScratch; // per-workgroup record scratch of the queued configurations
 };
 
 struct TileWalkArgs {
@@ -128,10 +129,15 @@ inline uint32_t remapUnits(const hvvr_gpu_view* v) { return ((v->width + 15) / 1
 // XCD x takes a contiguous run of tiles, which share BVH nodes and triangles in its L2).
 // None of these changes a result.
 struct LaunchShape {
-    uint32_t largeGroups = 1280, hugeGroups = 512, giantGroups = 256, walkGroups = 1024;
+    uint32_t largeGroups = 2560, hugeGroups = 1024, giantGroups = 512, walkGroups = 1024;
     uint32_t xcdSwizzle = 0;
     uint32_t streams = 1; // batches: chains the views are dealt to, each on its own stream
 };
+// the most persistent workgroups each queued configuration launches (its record scratch
+// is sized for them): as many as the chip holds at once
+enum : uint32_t { kMaxLargeGroups = 2560, kMaxHugeGroups = 1024, kMaxGiantGroups = 512 };
+// 32-bit words of record scratch one view (or one batch chain) needs
+size_t tileListScratchWords();
 
 // batch launchers; `all` = device array of per-view arguments
 void launchClearCountersBatch(uint64_t* const* counters, uint32_t viewCount, uint32_t slots, hipStream_t stream);

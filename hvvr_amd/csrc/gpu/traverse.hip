@@ -16,6 +16,8 @@
 //   tileTraverseKernel        one thread per tile with the full 4096-entry stack in global
 //                             memory; only runs the tiles tileListKernel could not hold in
 //                             LDS (count pass, allocation, emit pass)
+#include <algorithm>
+
 #include "device_common.h"
 #include "gpu_internal.h"
 #include "launch_args.h"
@@ -239,9 +241,9 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 //
 // Four configurations share the code. The standard one (768 records, a one-register
 // 64-entry stack, 10.5 KB of LDS) runs every tile; a tile it cannot hold is queued for
-// the large one (2048 records, a four-register 256-entry stack, 28 KB of LDS), what that
-// cannot hold for the huge one (4608 records, eight registers, 512 entries, 63 KB), and
-// so on to the giant one (10240 records, sixteen registers, 1024 entries, 140 KB); each
+// the large one (2048 records, a four-register 256-entry stack, 16 KB of LDS), what that
+// cannot hold for the huge one (4608 records, eight registers, 512 entries, 36 KB), and
+// so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB); each
 // takes its queue with persistent workgroups, as many as fit the chip at once. What none
 // holds goes to the wave-per-tile walk with its stack in LDS.
 // ---------------------------------------------------------------------------------
@@ -256,31 +258,48 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 
 // The standard configuration keeps the record index in bits 10..19 of a stack entry's
 // meta (records < 1024), so a pop needs no LDS read; the others stack record indices.
+// The standard configuration keeps every record array in LDS. The others keep only the
+// keys and metas the stack walk reads there; each record's node index and the emission
+// order go to a per-workgroup region of global scratch (their persistent workgroups
+// reuse it tile after tile), which halves their LDS and doubles the workgroups a CU holds
+// (the walk is latency-bound: throughput follows the resident waves).
 struct TileListSmall {
-    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1 };
+    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1 };
     using Stack = RegStack;
 };
 struct TileListLarge {
-    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0 };
+    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0, LdsRecords = 0 };
     using Stack = RegStack4;
 };
 struct TileListHuge {
-    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0 };
+    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0, LdsRecords = 0 };
     using Stack = RegStack8;
 };
 struct TileListGiant {
-    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0 };
+    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0 };
     using Stack = RegStack16;
 };
 
+// 32-bit words of one workgroup's scratch region: node indices, then 16-bit order entries
 template <class Cfg>
-struct TileListLds {
+constexpr uint32_t recScratchWords() {
+    return Cfg::LdsRecords ? 0u : ((Cfg::RecCap + Cfg::RecCap / 2 + 63u) & ~63u);
+}
+
+template <class Cfg, bool InLds = bool(Cfg::LdsRecords)>
+struct TileListLdsRecords {
     uint32_t recNode[Cfg::RecCap];
+    uint16_t order[Cfg::RecCap]; // records with emitted leaves, in emission order
+};
+template <class Cfg>
+struct TileListLdsRecords<Cfg, false> {};
+
+template <class Cfg>
+struct TileListLds : TileListLdsRecords<Cfg> {
     uint32_t recKey[Cfg::RecCap];  // float bits of the record's distance key; in phase C, the
                                    // list length of emitted record i
     uint32_t recMeta[Cfg::RecCap]; // packMeta(first child record, child mask, leaf mask,
                                    // stack-entry leaf mask)
-    uint16_t order[Cfg::RecCap];   // records with emitted leaves, in emission order
 };
 
 template <class Cfg>
@@ -350,6 +369,24 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         }
         return true;
     }
+    uint32_t* gNode = nullptr; // the record node indices and emission order outside LDS
+    uint16_t* gOrder = nullptr;
+    if constexpr (!Cfg::LdsRecords) {
+        gNode = g.recScratch + size_t(blockIdx.x) * recScratchWords<Cfg>();
+        gOrder = reinterpret_cast<uint16_t*>(gNode + Cfg::RecCap);
+    }
+    auto recNode = [&](uint32_t i) -> uint32_t& {
+        if constexpr (Cfg::LdsRecords)
+            return sh.recNode[i];
+        else
+            return gNode[i];
+    };
+    auto order = [&](uint32_t i) -> uint16_t& {
+        if constexpr (Cfg::LdsRecords)
+            return sh.order[i];
+        else
+            return gOrder[i];
+    };
     const uint32_t limit = g.stackLimit < uint32_t(Cfg::StackCap) ? g.stackLimit : uint32_t(Cfg::StackCap);
     bool overflow = n0 > limit;
     uint32_t deepest = n0; // deepest stack size seen
@@ -362,7 +399,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         bool nanKey = false;
         if (lane < n0) {
             StackEntry e = bs[lane];
-            sh.recNode[lane] = e.nodeMask & STACK_NODE_MASK;
+            recNode(lane) = e.nodeMask & STACK_NODE_MASK;
             sh.recKey[lane] = fbits(e.tMin);
             sh.recMeta[lane] = packMeta(0u, 0u, 0u, e.nodeMask >> 28);
             nanKey = e.tMin != e.tMin;
@@ -380,11 +417,11 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             const bool validA = rA < end, validB = rB < end;
             uint32_t nodeA = 0, entryA = 0, nodeB = 0, entryB = 0;
             if (validA) {
-                nodeA = sh.recNode[rA];
+                nodeA = recNode(rA);
                 entryA = metaEntry(sh.recMeta[rA]);
             }
             if (validB) {
-                nodeB = sh.recNode[rB];
+                nodeB = recNode(rB);
                 entryB = metaEntry(sh.recMeta[rB]);
             }
             const ChildBox A = loadChildBox(nodes, nodeA, c, axis);
@@ -409,12 +446,12 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             const uint32_t slotA = nrec + lanesBelow(biA);
             const uint32_t slotB = nrec + cntA + lanesBelow(biB);
             if (innerA && slotA < Cfg::RecCap) {
-                sh.recNode[slotA] = nodeA + A.childOffset;
+                recNode(slotA) = nodeA + A.childOffset;
                 sh.recKey[slotA] = fbits(A.key);
                 sh.recMeta[slotA] = 0u;
             }
             if (innerB && slotB < Cfg::RecCap) {
-                sh.recNode[slotB] = nodeB + B.childOffset;
+                recNode(slotB) = nodeB + B.childOffset;
                 sh.recKey[slotB] = fbits(B.key);
                 sh.recMeta[slotB] = 0u;
             }
@@ -474,7 +511,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                 emitBuf = writeLane(emitBuf, curRec, nEmit & 63u, lane);
                 nEmit++;
                 if ((nEmit & 63u) == 0u)
-                    sh.order[nEmit - 64u + lane] = uint16_t(emitBuf);
+                    order(nEmit - 64u + lane) = uint16_t(emitBuf);
             }
             const uint32_t inner = mask & ~leafMask;
             if (inner != 0u && metaEntry(curMeta) == 0u) {
@@ -539,7 +576,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
         }
         if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
-            sh.order[(nEmit & ~63u) + lane] = uint16_t(emitBuf);
+            order((nEmit & ~63u) + lane) = uint16_t(emitBuf);
         waveLdsSync();
     }
     if (overflow) {
@@ -560,10 +597,10 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         const uint32_t i = base + lane;
         uint32_t len = 0;
         if (i < nEmit) {
-            const uint32_t rec = sh.order[i];
+            const uint32_t rec = order(i);
             const uint32_t meta = sh.recMeta[rec];
             const uint32_t leaf = metaMask(meta) & metaLeaf(meta);
-            const uint32_t* d = nodes[sh.recNode[rec]].data;
+            const uint32_t* d = nodes[recNode(rec)].data;
             uint32_t dk[5];
 #pragma unroll
             for (int k = 0; k < 5; k++)
@@ -622,10 +659,10 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         uint32_t at = run + incl - len;
         run += __shfl(incl, 63);
         if (len) {
-            const uint32_t rec = sh.order[i];
+            const uint32_t rec = order(i);
             const uint32_t meta = sh.recMeta[rec];
             const uint32_t leaf = metaMask(meta) & metaLeaf(meta);
-            const uint32_t* d = nodes[sh.recNode[rec]].data;
+            const uint32_t* d = nodes[recNode(rec)].data;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint32_t t0 = d[k], t1 = d[k + 1];
@@ -985,6 +1022,7 @@ TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int l
     const bool full = v->waveStackLimit >= REG_STACK_CAP;
     const uint32_t caps[4] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP};
     g.stackLimit = level == 0 || !full ? v->waveStackLimit : caps[level];
+    g.recScratch = v->recScratch.ptr;
     return g;
 }
 
@@ -992,14 +1030,24 @@ static uint32_t clampGroups(uint32_t want, uint32_t tiles) {
     return want < tiles ? want : (tiles ? tiles : 1u);
 }
 
+size_t tileListScratchWords() {
+    const size_t large = size_t(kMaxLargeGroups) * recScratchWords<TileListLarge>();
+    const size_t huge = size_t(kMaxHugeGroups) * recScratchWords<TileListHuge>();
+    const size_t giant = size_t(kMaxGiantGroups) * recScratchWords<TileListGiant>();
+    return std::max(large, std::max(huge, giant));
+}
+
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
     const LaunchShape shape;
     const uint32_t T = v->tileCount;
     tileListKernel<<<T, 64, 0, stream>>>(tileListArgs(v, s, 0));
     // 28 KB of LDS: five workgroups per CU; 63 KB: two; 140 KB: one
-    tileListQueueKernel<TileListLarge><<<clampGroups(shape.largeGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 1));
-    tileListQueueKernel<TileListHuge><<<clampGroups(shape.hugeGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 2));
-    tileListQueueKernel<TileListGiant><<<clampGroups(shape.giantGroups, T), 64, 0, stream>>>(tileListArgs(v, s, 3));
+    tileListQueueKernel<TileListLarge><<<clampGroups(std::min(shape.largeGroups, uint32_t(kMaxLargeGroups)), T), 64,
+                                         0, stream>>>(tileListArgs(v, s, 1));
+    tileListQueueKernel<TileListHuge><<<clampGroups(std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups)), T), 64, 0,
+                                        stream>>>(tileListArgs(v, s, 2));
+    tileListQueueKernel<TileListGiant><<<clampGroups(std::min(shape.giantGroups, uint32_t(kMaxGiantGroups)), T), 64,
+                                         0, stream>>>(tileListArgs(v, s, 3));
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
@@ -1016,12 +1064,15 @@ void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream
 void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
                           const LaunchShape& shape, hipStream_t stream) {
     tileListBatchKernel<<<units, 64, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle);
-    tileListQueueBatchKernel<TileListLarge><<<clampGroups(shape.largeGroups, units), 64, 0, stream>>>(
-        all + viewCount, viewCount);
-    tileListQueueBatchKernel<TileListHuge><<<clampGroups(shape.hugeGroups, units), 64, 0, stream>>>(
-        all + 2 * viewCount, viewCount);
-    tileListQueueBatchKernel<TileListGiant><<<clampGroups(shape.giantGroups, units), 64, 0, stream>>>(
-        all + 3 * viewCount, viewCount);
+    tileListQueueBatchKernel<TileListLarge>
+        <<<clampGroups(std::min(shape.largeGroups, uint32_t(kMaxLargeGroups)), units), 64, 0, stream>>>(
+            all + viewCount, viewCount);
+    tileListQueueBatchKernel<TileListHuge>
+        <<<clampGroups(std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups)), units), 64, 0, stream>>>(
+            all + 2 * viewCount, viewCount);
+    tileListQueueBatchKernel<TileListGiant>
+        <<<clampGroups(std::min(shape.giantGroups, uint32_t(kMaxGiantGroups)), units), 64, 0, stream>>>(
+            all + 3 * viewCount, viewCount);
 }
 
 void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
