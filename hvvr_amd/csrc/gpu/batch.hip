@@ -404,6 +404,7 @@ int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
         v->rendered = true;
         v->stagesRecorded = false;
         v->lastStream = b->stream;
+        v->hintScene = s;
     }
     b->inFlight = true;
     return HVVR_GPU_OK;

@@ -403,6 +403,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->tileCounts.resize(v->tileCount));
     step(v->tileQueues.resize(size_t(4) * v->tileCount));
     step(v->recScratch.resize(tileListScratchWords()));
+    step(v->tileLevel.resize(v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
@@ -514,6 +515,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     HVVR_HIP_CHECK(hipGetLastError());
     v->stagesRecorded = prof;
     v->rendered = true;
+    v->hintScene = s; // this frame writes every tile's configuration hint
     v->lastStream = st;
     return HVVR_GPU_OK;
 }
@@ -863,7 +865,7 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + 6 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
+                 B * 4 + 7 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
                  S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 16 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
@@ -883,7 +885,8 @@ int hvvr_gpu_view_device_bytes(hvvr_gpu_view* v, uint64_t* bytes) {
         return fail("null argument");
     uint64_t b = v->beams.bytes() + v->tileFrustaCam.bytes() + v->blockFrustaCam.bytes() +
                  v->tileFrustaWorld.bytes() + v->tilePre.bytes() + v->blockPre.bytes() + v->blockStacks.bytes() +
-                 v->blockStackSize.bytes() + v->tileQueues.bytes() + v->recScratch.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
+                 v->blockStackSize.bytes() + v->tileQueues.bytes() + v->recScratch.bytes() + v->tileLevel.bytes() +
+                 v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
                  v->gbuffer.bytes() + v->sampleResults.bytes() + v->remap.bytes() + v->image.bytes() +
                  v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes();
     const FoveatedState& f = v->fov;

@@ -137,6 +137,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> blockStackSize;        // [blockCount]
     hvvr::DeviceBuffer<uint32_t> recScratch;            // record node indices and emission order of the
                                                         // queued tile-list configurations, per workgroup
+    hvvr::DeviceBuffer<uint32_t> tileLevel;             // [tileCount] tile-list configuration hint
+    const hvvr_gpu_scene* hintScene = nullptr;          // the scene the hints were last written for
     hvvr::DeviceBuffer<uint32_t> tileQueues;            // [4][tileCount] tiles queued for the large, huge
                                                         // and giant LDS configurations and the walk
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length

@@ -47,7 +47,16 @@ struct TileListArgs {
     unsigned long long* counters;
     uint32_t stackLimit;
     uint32_t* recScratch; // per-workgroup record scratch of the queued configurations
+    // per tile, the queued configuration (1..3, 4 = the walk) whose capacity the tile's
+    // last list on this scene needed; the queued kernels send a tile with a higher hint
+    // straight to that queue (every configuration builds the same list, so a hint only
+    // saves work)
+    uint32_t* tileLevel;
+    uint32_t useHints; // 0 on a view's first frame of a scene
+    uint32_t tileCount;
 };
+
+struct TileWalkArgs;
 
 struct TileWalkArgs {
     const BVHNode* nodes;
@@ -60,6 +69,7 @@ struct TileWalkArgs {
     uint32_t* triOut;
     uint64_t capacity;
     unsigned long long* counters;
+    uint32_t* tileLevel;
 };
 
 struct IntersectArgs {

@@ -264,19 +264,19 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 // reuse it tile after tile), which halves their LDS and doubles the workgroups a CU holds
 // (the walk is latency-bound: throughput follows the resident waves).
 struct TileListSmall {
-    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1 };
+    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
     using Stack = RegStack;
 };
 struct TileListLarge {
-    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0, LdsRecords = 0 };
+    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0, LdsRecords = 0, Level = 1 };
     using Stack = RegStack4;
 };
 struct TileListHuge {
-    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0, LdsRecords = 0 };
+    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0, LdsRecords = 0, Level = 2 };
     using Stack = RegStack8;
 };
 struct TileListGiant {
-    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0 };
+    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 3 };
     using Stack = RegStack16;
 };
 
@@ -368,6 +368,20 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             g.offsets[tile] = 0u;
         }
         return true;
+    }
+    if constexpr (Cfg::Level > 0) {
+        // A tile whose last list on this scene needed a still bigger configuration goes
+        // straight to that one's queue (queue h - 1, its length in counters[4 + h]). The
+        // standard configuration takes every tile first and keeps no hints: its code stays
+        // as short as it is.
+        const uint32_t hint = g.useHints ? g.tileLevel[tile] : 0u;
+        if (hint > uint32_t(Cfg::Level)) {
+            if (lane == 0) {
+                const unsigned long long slot = atomicAdd(&g.counters[4 + hint], 1ull);
+                g.nextQueue[size_t(hint - 1u - Cfg::Level) * g.tileCount + slot] = tile;
+            }
+            return true;
+        }
     }
     uint32_t* gNode = nullptr; // the record node indices and emission order outside LDS
     uint16_t* gOrder = nullptr;
@@ -638,6 +652,14 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             atomicOr(&g.counters[1], 1ull);
         atomicMax(&g.counters[2], (unsigned long long)deepest);
         atomicAdd(&g.counters[4], (unsigned long long)nrec); // node visits
+        if constexpr (Cfg::Level > 0) {
+            // the smallest queued configuration this tile's records and stack fit, for its
+            // next frame
+            const uint32_t level = nrec <= TileListLarge::RecCap && deepest <= REG_STACK4_CAP  ? 1u
+                                   : nrec <= TileListHuge::RecCap && deepest <= REG_STACK8_CAP ? 2u
+                                                                                               : 3u;
+            g.tileLevel[tile] = level;
+        }
     }
     if (!fits)
         return true;
@@ -848,8 +870,10 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
         }
         const bool overflow = stackOverflow || ballot(listOverflow) != 0ull;
         if (lane == 0) {
-            if (!emit)
+            if (!emit) {
                 counts[tile] = count;
+                a.tileLevel[tile] = 4u;
+            }
             if (overflow)
                 atomicOr(&counters[1], 1ull);
             atomicMax(&counters[2], (unsigned long long)maxSize);
@@ -947,6 +971,7 @@ TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
     a.triOut = v->triIndices.ptr;
     a.capacity = v->listCapacity;
     a.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    a.tileLevel = v->tileLevel.ptr;
     return a;
 }
 
@@ -1023,6 +1048,9 @@ TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int l
     const uint32_t caps[4] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP};
     g.stackLimit = level == 0 || !full ? v->waveStackLimit : caps[level];
     g.recScratch = v->recScratch.ptr;
+    g.tileLevel = v->tileLevel.ptr;
+    g.useHints = v->hintScene == s ? 1u : 0u;
+    g.tileCount = v->tileCount;
     return g;
 }
 

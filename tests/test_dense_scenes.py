@@ -8,7 +8,9 @@ of 1M entries. The reference's tile walk (traverse_avx.h:412-527) fixes the list
 tiles must get; the oracle restates it. GPU: the first frame reports the overflow and the
 entries it needed, the view grows its storage to that, and the frame rendered again
 matches the oracle bit for bit (lists, triangle ids, masks), with every tile taken by the
-huge LDS configuration or the LDS-stack walk as the stack limit decides."""
+huge LDS configuration or the LDS-stack walk as the stack limit decides; on the second
+frame the queued configurations send a tile straight to the one the first frame needed
+(per-tile hints)."""
 
 import numpy as np
 import pytest
@@ -38,6 +40,7 @@ def test_dense_scene_lists_grow_and_match(H, O, ctx, dense, dense_view, dense_re
     if limit:
         gv.set_traversal_stack_limit(limit)
     gv.render(gs)
+    first = gv.stats()  # no configuration hints yet: every tile starts at the standard one
     need, cap, overflowed = gv.list_demand()
     total = int(dense_ref["counts"].sum())
     print(f"limit {limit}: demand {need}, capacity {cap}, oracle total {total}", flush=True)
@@ -63,9 +66,13 @@ def test_dense_scene_lists_grow_and_match(H, O, ctx, dense, dense_view, dense_re
     assert st.overflow == 0 and st.survivorPairs == dense_ref["survivors"]
     if limit is None:
         # tiles past the large configuration exist and the huge one holds most of them
-        assert st.hugeTiles > 0 and st.walkTiles < st.hugeTiles
+        assert first.hugeTiles > 0 and first.walkTiles < first.hugeTiles
+        assert st.hugeTiles > 0
     else:
-        assert st.walkTiles == st.largeTiles == st.hugeTiles == st.giantTiles > 0
+        # every tile passes every queue to the walk; on the second frame the large
+        # configuration's kernel sends it there directly (the hint from the first)
+        assert first.walkTiles == first.largeTiles == first.hugeTiles == first.giantTiles > 0
+        assert st.largeTiles == st.walkTiles == first.walkTiles and st.hugeTiles == st.giantTiles == 0
     gv.close()
     gs.close()
 
