@@ -440,15 +440,21 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             }
             const ChildBox A = loadChildBox(nodes, nodeA, c, axis);
             const ChildBox B = loadChildBox(nodes, nodeB, c, axis);
-            bool passA = validA && childCoarsePass(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
-            bool passB = validB && childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+            // (`&` rather than `&&` throughout: no branch per lane condition)
+            bool passA = validA & childCoarsePass(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
+            bool passB = validB & childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
             uint32_t maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
             uint32_t maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
-            // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
-            if (passA && (maskA & A.leafMask) != 0u)
-                passA = !childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
-            if (passB && (maskB & B.leafMask) != 0u)
-                passB = !childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+            // refine only nodes with a passing leaf child (traverse_avx.h:447-452); the wave
+            // refines both boxes when any lane needs it
+            const bool refineA = passA & ((maskA & A.leafMask) != 0u);
+            const bool refineB = passB & ((maskB & B.leafMask) != 0u);
+            if (ballot(refineA | refineB) != 0ull) {
+                const bool cullA = childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
+                const bool cullB = childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+                passA &= !(refineA & cullA);
+                passB &= !(refineB & cullB);
+            }
             maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
             maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
             // internal children are pushed only from entries without a leaf mask
@@ -634,8 +640,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         }
         if (base == 0) {
             len0 = len;
-            incl0 = waveInclusiveScan(len, lane);
-            total += __shfl(incl0, 63);
+            incl0 = waveInclusiveScan(len);
+            total += readLane(incl0, 63);
         } else {
             total += waveSum(len);
         }
@@ -643,7 +649,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
     unsigned long long start = 0;
     if (lane == 0)
         start = atomicAdd(&g.counters[3], (unsigned long long)total);
-    start = __shfl(start, 0);
+    start = (uint64_t(readLane(uint32_t(start >> 32), 0)) << 32) | readLane(uint32_t(start), 0);
     const bool fits = start + total <= g.capacity;
     if (lane == 0) {
         g.counts[tile] = total;
@@ -673,13 +679,13 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                         g.triOut[at++] = t;
         }
     }
-    uint32_t run = uint32_t(start) + (nEmit ? uint32_t(__shfl(incl0, 63)) : 0u);
+    uint32_t run = uint32_t(start) + (nEmit ? readLane(incl0, 63) : 0u);
     for (uint32_t base = 64; base < nEmit; base += 64) {
         const uint32_t i = base + lane;
         const uint32_t len = i < nEmit ? sh.recKey[i] : 0u;
-        const uint32_t incl = waveInclusiveScan(len, lane);
+        const uint32_t incl = waveInclusiveScan(len);
         uint32_t at = run + incl - len;
-        run += __shfl(incl, 63);
+        run += readLane(incl, 63);
         if (len) {
             const uint32_t rec = order(i);
             const uint32_t meta = sh.recMeta[rec];

@@ -134,7 +134,10 @@ __device__ inline void waveLdsSync() {
 // ---------------------------------------------------------------------------------
 // Coarse test of one child box (testBVHNodeChildren, traverse_avx.h:104-164, for one
 // child lane): the four frustum planes against the box's n-vertex, then the three
-// world-axis projections. Same operations and rounding as testChildren().
+// world-axis projections. Same operations and rounding as testChildren(). Every test is
+// evaluated and the verdicts are combined with `&`: a short-circuit chain compiles to
+// one branch per test, each reloading its frustum coefficients with a scalar load and a
+// wait, where this way the coefficients load in batches.
 __device__ inline bool childCoarsePass(const FrustumPre* __restrict__ f, float b0, float b1, float b2, float b3,
                                        float b4, float b5) {
     bool pass = true;
@@ -142,17 +145,18 @@ __device__ inline bool childCoarsePass(const FrustumPre* __restrict__ f, float b
     for (int p = 0; p < 4; p++) {
         float dmax = fmaf(f->planeNeg[p][0], b0, fmaf(f->planeNeg[p][1], b2, f->planeNeg[p][2] * b4));
         float dneg = fmaf(f->planePos[p][0], b1, fmaf(f->planePos[p][1], b3, f->planePos[p][2] * b5));
-        pass = pass && ((dmax + dneg) <= f->planeNeg[p][3]);
+        pass &= (dmax + dneg) <= f->planeNeg[p][3];
     }
-    pass = pass && (b0 >= f->projLo[0]) && (b1 >= f->projHi[0]);
-    pass = pass && (b2 >= f->projLo[1]) && (b3 >= f->projHi[1]);
-    pass = pass && (b4 >= f->projLo[2]) && (b5 >= f->projHi[2]);
+    pass &= (b0 >= f->projLo[0]) & (b1 >= f->projHi[0]);
+    pass &= (b2 >= f->projLo[1]) & (b3 >= f->projHi[1]);
+    pass &= (b4 >= f->projLo[2]) & (b5 >= f->projHi[2]);
     return pass;
 }
 
 // Refinement of one child box (testBVHNodeChildrenRefine, traverse_avx.h:166-217, for
 // one child lane): true if one of the 12 frustum-edge x world-axis separating axes
-// separates the box from the frustum. Same operations as testChildrenRefine().
+// separates the box from the frustum. Same operations as testChildrenRefine(), every
+// axis evaluated (see childCoarsePass).
 __device__ inline bool childRefineCull(const FrustumPre* __restrict__ f, float b0, float b1, float b2, float b3,
                                        float b4, float b5) {
     bool cull = false;
@@ -167,7 +171,7 @@ __device__ inline bool childRefineCull(const FrustumPre* __restrict__ f, float b
         float nz = ez > 0.0f ? -b5 : b4;
         float maxBox = fmaf(ex, px, fmaf(ey, py, ez * pz));
         float minBox = fmaf(ex, nx, fmaf(ey, ny, ez * nz));
-        cull = cull || (maxBox < f->edgeMin[e]) || (f->edgeMax[e] < minBox);
+        cull |= (maxBox < f->edgeMin[e]) | (f->edgeMax[e] < minBox);
     }
     return cull;
 }
@@ -180,21 +184,24 @@ __device__ inline uint32_t lanesBelow(uint64_t b) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(b >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(b), 0u));
 }
 
-__device__ inline uint32_t waveSum(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o);
+// Inclusive prefix sum over the wave; every lane must be active. Within each row of 16
+// lanes, four DPP row shifts (by 1, 2, 4 and 8 lanes; a lane with no source lane adds the
+// 0 it keeps); then row_bcast:15 adds lane 15's sum (row 0) to row 1 and lane 47's (row 2)
+// to row 3, and row_bcast:31 adds lane 31's (rows 0 and 1) to rows 2 and 3. No LDS
+// round trip, unlike a shuffle (ds_bpermute) per step.
+__device__ inline uint32_t waveInclusiveScan(uint32_t v) {
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, false)); // row_shr:1
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, false)); // row_shr:2
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, false)); // row_shr:4
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, false)); // row_shr:8
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false)); // row_bcast:15
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false)); // row_bcast:31
     return v;
 }
 
-__device__ inline uint32_t waveInclusiveScan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o);
-        if (lane >= uint32_t(o))
-            v += t;
-    }
-    return v;
+// sum over the wave, in every lane; every lane must be active
+__device__ inline uint32_t waveSum(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_readlane(int(waveInclusiveScan(v)), 63));
 }
 
 } // namespace hvvr

@@ -1,6 +1,8 @@
-// GPU unit probe for the VGPR-resident traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h):
-// one wave runs a host-given sequence of inserts and pops, lane 0 records every popped
-// record, and the final register contents are written out. Test infrastructure only.
+// GPU unit probes of wave-level building blocks. The VGPR-resident traversal stacks
+// (hvvr_amd/csrc/gpu/reg_stack.h): one wave runs a host-given sequence of inserts and
+// pops, lane 0 records every popped record, and the final register contents are written
+// out. The wave prefix sum and sum (wave_traverse.h): one wave per 64 values. Test
+// infrastructure only.
 
 #include <hip/hip_runtime.h>
 
@@ -134,7 +136,43 @@ int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t
     return rc;
 }
 
+// one wave per 64 values: inclusive prefix sums within each group of 64, and the group's
+// sum in every lane
+__global__ __launch_bounds__(64) void waveScanProbe(const uint32_t* in, uint32_t* scan, uint32_t* sum) {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t v = in[i];
+    scan[i] = waveInclusiveScan(v);
+    sum[i] = waveSum(v);
+}
+
+int scanRun(const uint32_t* in, uint32_t n, uint32_t* scan, uint32_t* sum) {
+    if (n == 0 || n % 64u != 0u)
+        return -5;
+    uint32_t *dIn = nullptr, *dScan = nullptr, *dSum = nullptr;
+    int rc = 0;
+    if (hipMalloc(&dIn, n * 4) || hipMalloc(&dScan, n * 4) || hipMalloc(&dSum, n * 4))
+        rc = -1;
+    if (!rc && hipMemcpy(dIn, in, n * 4, hipMemcpyHostToDevice))
+        rc = -2;
+    if (!rc) {
+        waveScanProbe<<<n / 64u, 64>>>(dIn, dScan, dSum);
+        if (hipDeviceSynchronize() != hipSuccess)
+            rc = -3;
+    }
+    if (!rc && (hipMemcpy(scan, dScan, n * 4, hipMemcpyDeviceToHost) || hipMemcpy(sum, dSum, n * 4, hipMemcpyDeviceToHost)))
+        rc = -4;
+    (void)hipFree(dIn);
+    (void)hipFree(dScan);
+    (void)hipFree(dSum);
+    return rc;
+}
+
 } // namespace
+
+// n a multiple of 64
+extern "C" int wave_scan_probe_run(const uint32_t* in, uint32_t n, uint32_t* scan, uint32_t* sum) {
+    return scanRun(in, n, scan, sum);
+}
 
 extern "C" int reg_stack_probe_run(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n,
                                    uint32_t* popped, float* finalT, uint32_t* finalS, uint32_t* finalSize) {

@@ -5,7 +5,10 @@ and eight registers (256, 512) and LDS (2048 here): random sequences of sorted i
 them shallow, past each register boundary and at capacity; every popped record and the
 final contents must match a list model of the reference's insertion rule
 (traverse_avx.h:499-520: a new entry goes below the run of top entries whose key is
-smaller)."""
+smaller).
+
+The wave prefix sum and sum (hvvr_amd/csrc/gpu/wave_traverse.h, DPP row shifts and row
+broadcasts): against numpy's cumulative sum per 64 values, modulo 2^32."""
 
 import ctypes
 import os
@@ -89,3 +92,22 @@ def _depths(ops, cap):
         d = max(d - 1, 0) if o == 0 else min(d + 1, cap)
         out.append(d)
     return out
+
+
+@pytest.mark.gpu
+def test_wave_scan_matches_cumsum():
+    lib = ctypes.CDLL(LIB)
+    rng = np.random.default_rng(11)
+    groups = [rng.integers(0, 1000, 64), np.zeros(64, np.int64), np.ones(64, np.int64),
+              rng.integers(0, 2**32, 64),  # sums wrap modulo 2^32
+              np.where(np.arange(64) % 17 == 0, 5, 0), np.eye(64, dtype=np.int64)[63] * 7,
+              np.eye(64, dtype=np.int64)[0] * 9] + [rng.integers(0, 64, 64) for _ in range(57)]
+    x = np.concatenate(groups).astype(np.uint32)
+    scan = np.zeros_like(x)
+    total = np.zeros_like(x)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.wave_scan_probe_run(p(x), ctypes.c_uint32(len(x)), p(scan), p(total)) == 0
+    g = x.reshape(-1, 64).astype(np.uint64)
+    want = (np.cumsum(g, axis=1) % 2**32).astype(np.uint32)
+    assert np.array_equal(scan.reshape(-1, 64), want)
+    assert np.array_equal(total.reshape(-1, 64), np.repeat(want[:, 63:], 64, axis=1))
