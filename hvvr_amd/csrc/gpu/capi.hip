@@ -404,6 +404,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     step(v->tileQueues.resize(size_t(4) * v->tileCount));
     step(v->recScratch.resize(tileListScratchWords()));
     step(v->tileLevel.resize(v->tileCount));
+    step(v->tileWork.resize(size_t(3) * v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
@@ -421,6 +422,11 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         step(hipEventCreate(&ev));
     if (e == hipSuccess)
         e = hipMemsetAsync(v->image.ptr, 0, v->image.bytes(), st);
+    // no hints yet (0), and zero work: neither may hold what a freed allocation left
+    if (e == hipSuccess)
+        e = hipMemsetAsync(v->tileLevel.ptr, 0, v->tileLevel.bytes(), st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(v->tileWork.ptr, 0, v->tileWork.bytes(), st);
     if (e == hipSuccess)
         e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -865,7 +871,7 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + 7 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
+                 B * 4 + 10 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
                  S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 16 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
@@ -886,7 +892,7 @@ int hvvr_gpu_view_device_bytes(hvvr_gpu_view* v, uint64_t* bytes) {
     uint64_t b = v->beams.bytes() + v->tileFrustaCam.bytes() + v->blockFrustaCam.bytes() +
                  v->tileFrustaWorld.bytes() + v->tilePre.bytes() + v->blockPre.bytes() + v->blockStacks.bytes() +
                  v->blockStackSize.bytes() + v->tileQueues.bytes() + v->recScratch.bytes() + v->tileLevel.bytes() +
-                 v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
+                 v->tileWork.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
                  v->gbuffer.bytes() + v->sampleResults.bytes() + v->remap.bytes() + v->image.bytes() +
                  v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes();
     const FoveatedState& f = v->fov;
@@ -1015,7 +1021,11 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     if (rc && rc != HVVR_GPU_ERR_OVERFLOW)
         return rc;
     uint64_t counters[16];
-    HVVR_HIP_CHECK(hipMemcpy(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+    hipStream_t stream = v->lastStream ? v->lastStream : v->ctx->stream;
+    launchTileWorkReduce(v, stream);
+    HVVR_HIP_CHECK(hipGetLastError());
+    HVVR_HIP_CHECK(hipMemcpyAsync(counters, v->counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, stream));
+    HVVR_HIP_CHECK(hipStreamSynchronize(stream));
     std::vector<uint32_t> counts(v->tileCount);
     HVVR_HIP_CHECK(hipMemcpy(counts.data(), v->tileCounts.ptr, counts.size() * 4, hipMemcpyDeviceToHost));
     uint64_t total = 0;

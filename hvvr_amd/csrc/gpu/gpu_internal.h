@@ -138,6 +138,9 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> recScratch;            // record node indices and emission order of the
                                                         // queued tile-list configurations, per workgroup
     hvvr::DeviceBuffer<uint32_t> tileLevel;             // [tileCount] tile-list configuration hint
+    hvvr::DeviceBuffer<uint32_t> tileWork;              // [3][tileCount] per tile, last frame: BVH nodes
+                                                        // tested by the tile traversal, its deepest stack,
+                                                        // triangles surviving intersect's tile cull
     const hvvr_gpu_scene* hintScene = nullptr;          // the scene the hints were last written for
     hvvr::DeviceBuffer<uint32_t> tileQueues;            // [4][tileCount] tiles queued for the large, huge
                                                         // and giant LDS configurations and the walk
@@ -153,7 +156,8 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint64_t> counters;              // [0] survivors, [1] overflow flag, [2] max tile stack,
                                                         // [3] list entries allocated, [4] node visits,
                                                         // [5..8] tiles queued for the large, huge and
-                                                        // giant configurations and for the walk
+                                                        // giant configurations and for the walk; [0], [2]
+                                                        // and [4] are reduced from tileWork on request
     uint64_t listCapacity = 0;
     uint32_t waveStackLimit = 64; // register stack entries per tile traversal (<= 64)
     // foveated path (hvvr_gpu_view_set_foveated): cull planes and the t-max buffer apply to
@@ -190,6 +194,8 @@ void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t 
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
 void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream);
+// counters[0], [2] and [4] from the view's per-tile work of its last frame
+void launchTileWorkReduce(hvvr_gpu_view* v, hipStream_t stream);
 void launchIntersect(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m, hipStream_t stream);
 void launchRemap(hvvr_gpu_view* v, hipStream_t stream);

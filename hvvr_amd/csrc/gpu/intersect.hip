@@ -285,8 +285,8 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
         }
         __syncthreads(); // before the next chunk overwrites the cache
     }
-    if (args.survivors && tid == 0)
-        atomicAdd(args.survivors, (unsigned long long)survivorsSum * 128ull);
+    if (tid == 0)
+        args.tileSurvivors[tile] = survivorsSum;
 
     GBufferSubsample* waveBase = args.gbuffer + size_t(sample >> 6) * 64 * A;
     emitGBuffer<A>(hit, waveBase, lane);
@@ -316,7 +316,7 @@ IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, con
     a.triIndices = v->triIndices.ptr;
     a.worldTiles = v->tileFrustaWorld.ptr;
     a.tris = s->triIntersect.ptr;
-    a.survivors = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    a.tileSurvivors = v->tileWork.ptr + size_t(2) * v->tileCount;
     a.listCapacity = v->listCapacity;
     a.cameraToWorld = m;
     for (int i = 0; i < 32; i++)

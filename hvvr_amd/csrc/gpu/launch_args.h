@@ -54,6 +54,10 @@ struct TileListArgs {
     uint32_t* tileLevel;
     uint32_t useHints; // 0 on a view's first frame of a scene
     uint32_t tileCount;
+    // per tile: [tile] BVH nodes tested, [tileCount + tile] deepest stack (plain stores:
+    // one atomic per tile on a shared counter serialises every tile of the launch at one
+    // memory channel)
+    uint32_t* tileWork;
 };
 
 struct TileWalkArgs;
@@ -70,6 +74,8 @@ struct TileWalkArgs {
     uint64_t capacity;
     unsigned long long* counters;
     uint32_t* tileLevel;
+    uint32_t* tileWork; // as in TileListArgs
+    uint32_t tileCount;
 };
 
 struct IntersectArgs {
@@ -80,7 +86,7 @@ struct IntersectArgs {
     const uint32_t* triIndices;
     const SimpleRayFrustum* worldTiles;
     const PrecomputedTriangleIntersect* tris;
-    unsigned long long* survivors;
+    uint32_t* tileSurvivors; // per tile: list entries surviving the tile cull
     uint64_t listCapacity;
     matrix4x4 cameraToWorld;
     vector2 offsets[32];

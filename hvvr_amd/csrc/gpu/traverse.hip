@@ -366,6 +366,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         if (lane == 0) {
             g.counts[tile] = 0u;
             g.offsets[tile] = 0u;
+            g.tileWork[tile] = 0u;
+            g.tileWork[g.tileCount + tile] = 0u;
         }
         return true;
     }
@@ -373,9 +375,9 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         // A tile whose last list on this scene needed a still bigger configuration goes
         // straight to that one's queue (queue h - 1, its length in counters[4 + h]). The
         // standard configuration takes every tile first and keeps no hints: its code stays
-        // as short as it is.
+        // as short as it is. Hints are 0 (none) or 1..4; anything else is ignored.
         const uint32_t hint = g.useHints ? g.tileLevel[tile] : 0u;
-        if (hint > uint32_t(Cfg::Level)) {
+        if (hint > uint32_t(Cfg::Level) && hint <= 4u) {
             if (lane == 0) {
                 const unsigned long long slot = atomicAdd(&g.counters[4 + hint], 1ull);
                 g.nextQueue[size_t(hint - 1u - Cfg::Level) * g.tileCount + slot] = tile;
@@ -656,8 +658,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         g.offsets[tile] = uint32_t(start);
         if (!fits)
             atomicOr(&g.counters[1], 1ull);
-        atomicMax(&g.counters[2], (unsigned long long)deepest);
-        atomicAdd(&g.counters[4], (unsigned long long)nrec); // node visits
+        g.tileWork[tile] = nrec; // node visits
+        g.tileWork[g.tileCount + tile] = deepest;
         if constexpr (Cfg::Level > 0) {
             // the smallest queued configuration this tile's records and stack fit, for its
             // next frame
@@ -792,8 +794,11 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
         const uint32_t n0 = uniform(a.blockStackSize[block]);
         const FrustumPre& f = a.tilePre[tile];
         if (n0 == 0 || f.degenerate) {
-            if (!emit && lane == 0)
+            if (!emit && lane == 0) {
                 counts[tile] = 0;
+                a.tileWork[tile] = 0u;
+                a.tileWork[a.tileCount + tile] = 0u;
+            }
             return;
         }
         waveLdsSync(); // the previous tile's LDS reads are done
@@ -879,12 +884,11 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
             if (!emit) {
                 counts[tile] = count;
                 a.tileLevel[tile] = 4u;
+                a.tileWork[tile] = visits; // node visits
+                a.tileWork[a.tileCount + tile] = maxSize;
             }
             if (overflow)
                 atomicOr(&counters[1], 1ull);
-            atomicMax(&counters[2], (unsigned long long)maxSize);
-            if (!emit)
-                atomicAdd(&counters[4], (unsigned long long)visits); // node visits
         }
     }
 }
@@ -978,6 +982,8 @@ TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
     a.capacity = v->listCapacity;
     a.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
     a.tileLevel = v->tileLevel.ptr;
+    a.tileWork = v->tileWork.ptr;
+    a.tileCount = v->tileCount;
     return a;
 }
 
@@ -1030,6 +1036,46 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
     occupiedRankKernel<<<1, kRankThreads, 0, stream>>>(v->tileCounts.ptr, v->tileCount, v->occRank.ptr);
 }
 
+// The frame's work statistics from the per-tile values (one workgroup): node visits and
+// surviving list entries summed, the deepest stack as the maximum; a survivor is tested
+// against the tile's 128 samples.
+enum : uint32_t { kReduceThreads = 256 };
+__global__ __launch_bounds__(kReduceThreads) void tileWorkReduceKernel(const uint32_t* __restrict__ work, uint32_t n,
+                                                                       unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long sums[2][kReduceThreads];
+    __shared__ uint32_t deepest[kReduceThreads];
+    const uint32_t tid = threadIdx.x;
+    unsigned long long visits = 0, survivors = 0;
+    uint32_t deep = 0;
+    for (uint32_t i = tid; i < n; i += kReduceThreads) {
+        visits += work[i];
+        deep = max(deep, work[n + i]);
+        survivors += work[2 * size_t(n) + i];
+    }
+    sums[0][tid] = visits;
+    sums[1][tid] = survivors;
+    deepest[tid] = deep;
+    __syncthreads();
+    for (uint32_t half = kReduceThreads / 2; half > 0; half >>= 1) {
+        if (tid < half) {
+            sums[0][tid] += sums[0][tid + half];
+            sums[1][tid] += sums[1][tid + half];
+            deepest[tid] = max(deepest[tid], deepest[tid + half]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        counters[0] = sums[1][0] * TILE_SIZE;
+        counters[2] = deepest[0];
+        counters[4] = sums[0][0];
+    }
+}
+
+void launchTileWorkReduce(hvvr_gpu_view* v, hipStream_t stream) {
+    tileWorkReduceKernel<<<1, kReduceThreads, 0, stream>>>(v->tileWork.ptr, v->tileCount,
+                                                          reinterpret_cast<unsigned long long*>(v->counters.ptr));
+}
+
 TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level) {
     uint32_t* queues = v->tileQueues.ptr; // [4][tileCount]: large, huge, giant, walk
     const size_t T = v->tileCount;
@@ -1057,6 +1103,7 @@ TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int l
     g.tileLevel = v->tileLevel.ptr;
     g.useHints = v->hintScene == s ? 1u : 0u;
     g.tileCount = v->tileCount;
+    g.tileWork = v->tileWork.ptr;
     return g;
 }
 
