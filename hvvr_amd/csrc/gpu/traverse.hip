@@ -239,8 +239,8 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // of anything), so a tile with one, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
-// Four configurations share the code. The standard one (768 records, a one-register
-// 64-entry stack, 10.5 KB of LDS) runs every tile; a tile it cannot hold is queued for
+// Four configurations share the code. The standard one (704 records, a one-register
+// 64-entry stack, 9.6 KB of LDS) runs every tile; a tile it cannot hold is queued for
 // the large one (2048 records, a four-register 256-entry stack, 16 KB of LDS), what that
 // cannot hold for the huge one (4608 records, eight registers, 512 entries, 36 KB), and
 // so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB); each
@@ -264,7 +264,7 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 // reuse it tile after tile), which halves their LDS and doubles the workgroups a CU holds
 // (the walk is latency-bound: throughput follows the resident waves).
 struct TileListSmall {
-    enum : uint32_t { RecCap = 768, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
+    enum : uint32_t { RecCap = 704, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
     using Stack = RegStack;
 };
 struct TileListLarge {
@@ -535,67 +535,78 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                 if ((nEmit & 63u) == 0u)
                     order(nEmit - 64u + lane) = uint16_t(emitBuf);
             }
-            const uint32_t inner = mask & ~leafMask;
-            if (inner != 0u && metaEntry(curMeta) == 0u) {
+            // internal children are followed only from entries without a leaf mask
+            const uint32_t inner = metaEntry(curMeta) == 0u ? mask & ~leafMask : 0u;
+            // Every path ends at the one pop below (or skips it), so the loop keeps a single
+            // latch: no state variable or register copies between the ways a step can end.
+            bool pop = true;
+            if (inner != 0u) {
+                // the hit children's records, contiguous from `first` in child order; lanes
+                // past them read an in-range record and are masked off
                 const uint32_t first = metaFirst<Cfg>(curMeta);
-                if ((inner & (inner - 1u)) == 0u) {
+                const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
+                const uint32_t r = first + lane < uint32_t(Cfg::RecCap) ? first + lane : uint32_t(Cfg::RecCap) - 1u;
+                const uint32_t childKey = sh.recKey[r];
+                const uint32_t childMeta = sh.recMeta[r];
+                const uint32_t liveBits =
+                    uint32_t(ballot(metaMask(childMeta) != 0u)) & ((1u << nChildren) - 1u);
+                if (nChildren == 1u) {
                     // single hit child: test it next without touching the stack
-                    const uint32_t child = uniform(sh.recMeta[first]);
-                    if (metaMask(child) != 0u) {
+                    if (liveBits != 0u) {
                         curRec = first;
-                        curMeta = child;
-                        continue;
+                        curMeta = readLane(childMeta, 0);
+                        pop = false;
                     }
-                } else {
+                } else if (liveBits != 0u) {
                     // children in child order, each inserted below the run of closer entries
                     // (traverse_avx.h:499-520), dead ones left out
-                    const uint32_t nChildren = uint32_t(__builtin_popcount(inner));
-                    const bool isChild = lane < nChildren;
-                    const uint32_t childKey = isChild ? sh.recKey[first + lane] : 0u;
-                    const uint32_t childMeta = isChild ? sh.recMeta[first + lane] : 0u;
-                    const bool live = isChild && metaMask(childMeta) != 0u;
-                    const uint32_t liveBits = uint32_t(ballot(live));
-                    if (liveBits != 0u) {
-                        const uint32_t nLive = uint32_t(__builtin_popcount(liveBits));
-                        if (st.size + nLive - 1u >= limit) {
-                            overflow = true;
-                            break;
-                        }
-                        // The walk pops the top right after these inserts. When the closest
-                        // child would land on top (ties go to the newest insert), visit it
-                        // directly and insert only its siblings: the stack ends the same.
-                        // The newest of the closest live children (the tile has no NaN
-                        // keys): the least key over the four child lanes (two quad DPP
-                        // steps), then the highest live lane holding it.
-                        const float ck = live ? bitsf(childKey) : INFINITY;
-                        float least = fminf(ck, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
-                                                    0, int(fbits(ck)), 0xb1, 0xf, 0xf, false)))); // lane ^ 1
-                        least = fminf(least, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
-                                                 0, int(fbits(least)), 0x4e, 0xf, 0xf, false)))); // lane ^ 2
-                        const uint32_t atLeast = uint32_t(ballot(live && ck == least));
-                        const uint32_t best = 31u - uint32_t(__builtin_clz(atLeast));
-                        const float bestKey = bitsf(readLane(fbits(least), 0));
-                        const bool take = st.size == 0u || bestKey <= stackTopKey(st);
-                        const uint32_t entry = stackEntry<Cfg>(first + lane, childMeta);
-                        for (uint32_t sib = liveBits & ~(take ? 1u << best : 0u); sib != 0u; sib &= sib - 1u) {
+                    const uint32_t nLive = uint32_t(__builtin_popcount(liveBits));
+                    if (st.size + nLive - 1u >= limit) {
+                        overflow = true;
+                        break;
+                    }
+                    // The walk pops the top right after these inserts. When the closest child
+                    // would land on top (ties go to the newest insert), visit it directly and
+                    // insert only its siblings: the stack ends the same. The newest of the
+                    // closest live children (the tile has no NaN keys): the least key over the
+                    // four child lanes (two quad DPP steps), then the highest live lane
+                    // holding it.
+                    const bool live = ((uint64_t(liveBits) >> lane) & 1ull) != 0ull;
+                    const float ck = live ? bitsf(childKey) : INFINITY;
+                    float least = fminf(ck, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
+                                                0, int(fbits(ck)), 0xb1, 0xf, 0xf, false)))); // lane ^ 1
+                    least = fminf(least, bitsf(uint32_t(__builtin_amdgcn_update_dpp(
+                                             0, int(fbits(least)), 0x4e, 0xf, 0xf, false)))); // lane ^ 2
+                    const uint32_t atLeast = uint32_t(ballot(ck == least)) & liveBits;
+                    const uint32_t best = 31u - uint32_t(__builtin_clz(atLeast));
+                    const float bestKey = bitsf(readLane(fbits(least), 0));
+                    const bool take = st.size == 0u || bestKey <= stackTopKey(st);
+                    const uint32_t entry = stackEntry<Cfg>(first + lane, childMeta);
+                    // at most four siblings: straight-line code (a loop here keeps the
+                    // compiler from holding the stack in the same registers on every path)
+                    uint32_t sib = liveBits & ~(take ? 1u << best : 0u);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (sib != 0u) {
                             const uint32_t j = uint32_t(__builtin_ctz(sib));
                             stackInsert(st, bitsf(readLane(childKey, j)), readLane(entry, j), lane);
+                            sib &= sib - 1u;
                         }
-                        const uint32_t depth = st.size + (take ? 1u : 0u);
-                        deepest = depth > deepest ? depth : deepest;
-                        if (take) {
-                            curRec = first + best;
-                            curMeta = readLane(childMeta, best);
-                        } else {
-                            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
-                        }
-                        continue;
+                    }
+                    const uint32_t depth = st.size + (take ? 1u : 0u);
+                    deepest = depth > deepest ? depth : deepest;
+                    if (take) {
+                        curRec = first + best;
+                        curMeta = readLane(childMeta, best);
+                        pop = false;
                     }
                 }
             }
-            if (st.size == 0u)
-                break;
-            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
+            if (pop) {
+                if (st.size == 0u)
+                    break;
+                fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
+            }
         }
         if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
             order((nEmit & ~63u) + lane) = uint16_t(emitBuf);
@@ -714,6 +725,9 @@ __device__ inline void tileListSmallOne(const TileListArgs& g, TileListLds<TileL
     }
 }
 
+// The LDS records (9856 bytes) let a CU hold 16 of these waves. A record capacity below
+// what a scene's densest tiles need costs more than the waves it adds: the tiles it sends
+// to the large configuration run after this kernel, each a serial walk of one wave.
 __global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
     __shared__ TileListLds<TileListSmall> sh;
     tileListSmallOne(g, sh, blockIdx.x, threadIdx.x);
