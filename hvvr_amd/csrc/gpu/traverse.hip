@@ -124,25 +124,25 @@ __device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* 
         return;
     }
     const int axis = w.axis, sizeAxis = w.axis & ~1;
+    // The stack in registers, bottom-up: entry q in lane q of (sBits, sT, sNode). The walk
+    // stops at 60 entries and a step inserts at most 4 (a leaf entry and 3 children, or 4
+    // children), so it never holds more than 62.
     uint32_t top = 0;
+    uint32_t sBits = 0u, sNode = 0u;
+    float sT = 0.0f;
     float tMin = 0.0f;
     uint32_t node = 0;
-    // insert e so that it lands below the run of top entries whose (DAZ) key exceeds `key`
-    auto insert = [&](float key, const CullEntry& e) {
-        int j = int(top) - 1 - int(lane);
-        bool larger = j >= 0 && daz(cull[j >= 0 ? j : 0].bits) > key;
-        uint64_t run = ~ballot(larger);
-        uint32_t n = run ? uint32_t(__builtin_ctzll(run)) : 64u;
-        CullEntry moved;
-        if (lane < n)
-            moved = cull[j];
-        waveLdsSync();
-        if (lane < n)
-            cull[j + 1] = moved;
-        waveLdsSync();
-        if (lane == 0)
-            cull[top - n] = e;
-        waveLdsSync();
+    // Inserts (bits, t, node) below the run of top entries whose (DAZ) key exceeds `key`:
+    // one above the highest entry whose key does not, position 0 when there is none. One
+    // ballot finds the position, two DPP shifts per register move the entries above it.
+    auto insert = [&](float key, uint32_t eBits, float eT, uint32_t eNode) {
+        const uint64_t notLarger = ballot(lane < top && !(daz(sBits) > key));
+        const uint32_t p = notLarger ? 64u - uint32_t(__builtin_clzll(notLarger)) : 0u;
+        const uint32_t ub = laneShiftUp(sBits, 0u), un = laneShiftUp(sNode, 0u);
+        const float ut = bitsf(laneShiftUp(fbits(sT), 0u));
+        sBits = lane < p ? sBits : (lane == p ? eBits : ub);
+        sNode = lane < p ? sNode : (lane == p ? eNode : un);
+        sT = lane < p ? sT : (lane == p ? eT : ut);
         top++;
     };
     for (;;) {
@@ -155,29 +155,27 @@ __device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* 
             uint32_t leaf = mask & leafMask;
             if (leaf) {
                 mask &= ~leaf;
-                CullEntry e;
-                e.bits = leaf;
-                e.tMinTemp = tMin;
-                e.node = node;
-                insert(0.0f, e);
+                insert(0.0f, leaf, tMin, node);
             }
             for (int k = 0; k < 4; k++) {
                 if (!(mask & (1u << k)))
                     continue;
-                float size = ftz(nodes[node].bounds[sizeAxis][k] + nodes[node].bounds[sizeAxis + 1][k]);
-                CullEntry e;
-                e.bits = fbits(size);
-                e.tMinTemp = -nodes[node].bounds[axis][k];
-                e.node = node + nodes[node].data[1 + k];
-                insert(daz(e.bits), e);
+                const float size = ftz(nodes[node].bounds[sizeAxis][k] + nodes[node].bounds[sizeAxis + 1][k]);
+                insert(daz(fbits(size)), fbits(size), -nodes[node].bounds[axis][k], node + nodes[node].data[1 + k]);
             }
         }
-        if (top >= BLOCK_STACK_SIZE - 4 || top == 0 || cull[top - 1].bits <= 0xfu)
+        if (top >= BLOCK_STACK_SIZE - 4 || top == 0 || readLane(sBits, top - 1u) <= 0xfu)
             break;
         top--;
-        node = uniform(cull[top].node);
-        tMin = cull[top].tMinTemp;
+        node = readLane(sNode, top);
+        tMin = bitsf(readLane(fbits(sT), top));
     }
+    if (lane < top) {
+        cull[lane].bits = sBits;
+        cull[lane].tMinTemp = sT;
+        cull[lane].node = sNode;
+    }
+    waveLdsSync();
     // BlockFrame::sort as ranks: entry t ends above every entry with a larger distance
     // and above earlier entries with an equal one
     if (lane < top) {
