@@ -445,15 +445,37 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             bool passB = validB & childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
             uint32_t maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
             uint32_t maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
-            // refine only nodes with a passing leaf child (traverse_avx.h:447-452); the wave
-            // refines both boxes when any lane needs it
+            // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
             const bool refineA = passA & ((maskA & A.leafMask) != 0u);
             const bool refineB = passB & ((maskB & B.leafMask) != 0u);
-            if (ballot(refineA | refineB) != 0ull) {
-                const bool cullA = childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
-                const bool cullB = childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
-                passA &= !(refineA & cullA);
-                passB &= !(refineB & cullB);
+            const uint64_t needA = ballot(refineA), needB = ballot(refineB);
+            if ((needA | needB) != 0ull) {
+                const uint32_t nA = uint32_t(__builtin_popcountll(needA));
+                const uint32_t nB = uint32_t(__builtin_popcountll(needB));
+                if (nA + nB < 64u) {
+                    // Few boxes need the 12-axis test (typically a sixth of the lanes): move
+                    // them to lanes 0 .. nA + nB - 1 with forward permutes (every other lane
+                    // sends to lane 63, which then holds no needed box), test them in one
+                    // pass, and read each verdict back from the ballot.
+                    const uint32_t dA = refineA ? lanesBelow(needA) : 63u;
+                    const uint32_t dB = refineB ? nA + lanesBelow(needB) : 63u;
+                    auto gather = [&](float a, float b) {
+                        const uint32_t xa = uint32_t(__builtin_amdgcn_ds_permute(int(dA << 2), int(fbits(a))));
+                        const uint32_t xb = uint32_t(__builtin_amdgcn_ds_permute(int(dB << 2), int(fbits(b))));
+                        return bitsf(lane < nA ? xa : xb);
+                    };
+                    const float g0 = gather(A.b0, B.b0), g1 = gather(A.b1, B.b1), g2 = gather(A.b2, B.b2);
+                    const float g3 = gather(A.b3, B.b3), g4 = gather(A.b4, B.b4), g5 = gather(A.b5, B.b5);
+                    const bool cull = childRefineCull(f, g0, g1, g2, g3, g4, g5);
+                    const uint64_t culled = ballot(cull & (lane < nA + nB));
+                    passA &= !(refineA & (((culled >> dA) & 1ull) != 0ull));
+                    passB &= !(refineB & (((culled >> dB) & 1ull) != 0ull));
+                } else {
+                    const bool cullA = childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
+                    const bool cullB = childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
+                    passA &= !(refineA & cullA);
+                    passB &= !(refineB & cullB);
+                }
             }
             maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
             maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
