@@ -152,8 +152,14 @@ __device__ inline vector3 ggxShade(const SimpleMaterial& mat, vector3 pos, vecto
 // lightColor * F * (1 - base NdotL) * s. The scalar terms (D, vis, Fresnel weight) keep
 // FP32: D * vis spans many decades (up to ~1e7 for a glossy highlight seen edge-on). The
 // two colour vectors, bounded by the light colour, are formed with packed FP16 operations
-// (v_pk_mul_f16 / v_pk_add_f16); the radiance sum stays FP32.
+// (v_pk_mul_f16 / v_pk_add_f16); the radiance sum stays FP32. The scalar terms use the
+// hardware reciprocal and reciprocal square root (about 1 ulp in FP32, far below the FP16
+// colour terms' resolution) in place of correctly rounded divisions and square roots:
+// three divisions and a normalisation per light are most of the shading's instructions.
 typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+
+__device__ inline float rcpFast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline vector3 normalizeFast(const vector3& v) { return v * __builtin_amdgcn_rsqf(dot(v, v)); }
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 __device__ inline half4v toHalf4(const vector3& v) {
@@ -163,11 +169,11 @@ __device__ inline half4v toHalf4(const vector3& v) {
 __device__ inline void pbrBrdfHalf(const vector3& L, const vector3& N, const vector3& V, half4v base,
                                    float roughness, half4v F0, half4v lightColor, float4v& rad) {
     float alpha = roughness * roughness;
-    vector3 H = normalize(L + V);
+    vector3 H = normalizeFast(L + V);
     float alpha2 = alpha * alpha;
     float NdotH = saturate(dot(N, H));
     float q = NdotH * NdotH * (alpha2 - 1.0f) + 1.0f;
-    float D = alpha2 / (3.141592653589f * (q * q));
+    float D = alpha2 * rcpFast(3.141592653589f * (q * q));
     float LdotH = saturate(dot(L, H));
     float x = 1.0f - LdotH;
     float x2 = x * x;
@@ -175,7 +181,7 @@ __device__ inline void pbrBrdfHalf(const vector3& L, const vector3& N, const vec
     float NdotL = saturate(dot(N, L));
     float NdotV = saturate(dot(N, V));
     float k = alpha * 0.5f;
-    float vis = (1.0f / (NdotL * (1.0f - k) + k)) * (1.0f / (NdotV * (1.0f - k) + k));
+    float vis = rcpFast(NdotL * (1.0f - k) + k) * rcpFast(NdotV * (1.0f - k) + k);
     float sc = (D * vis) * 0.25f;
     const half4v one = (_Float16)1.0f;
     const half4v F = F0 + (one - F0) * (_Float16)p5;
@@ -187,12 +193,12 @@ __device__ inline void pbrBrdfHalf(const vector3& L, const vector3& N, const vec
 
 __device__ inline vector3 ggxShadeHalf(const SimpleMaterial& mat, vector3 pos, vector3 nrm, vector3 camPos,
                                        const LightingEnvironment& env) {
-    vector3 normal = normalize(nrm);
+    vector3 normal = normalizeFast(nrm);
     uint32_t model = uint32_t(mat.textureIDsAndShadingModel & 0xffffu);
     if (model == ShadingModelEmissive)
         return mat.emissive.xyz();
     const half4v base = toHalf4(mat.diffuse.xyz());
-    vector3 V = normalize(camPos - pos);
+    vector3 V = normalizeFast(camPos - pos);
     const half4v F0 = toHalf4(mat.specular.xyz());
     float roughness = 1.0f - mat.glossiness;
     vector3 amb = mat.emissive.xyz() + mat.diffuse.xyz() * vector3(0.15f, 0.15f, 0.165f);
@@ -208,8 +214,9 @@ __device__ inline vector3 ggxShadeHalf(const SimpleMaterial& mat, vector3 pos, v
             break;
         const LightPoint& l = env.pointLights[i];
         vector3 delta = l.Position - pos;
-        float r = length(delta);
-        vector3 L = normalize(delta);
+        const float d2 = dot(delta, delta), inv = __builtin_amdgcn_rsqf(d2);
+        float r = d2 * inv;
+        vector3 L = delta * inv;
         float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
         pbrBrdfHalf(L, normal, V, base, roughness, F0, toHalf4(l.Color * fdist), rad);
     }
@@ -218,8 +225,9 @@ __device__ inline vector3 ggxShadeHalf(const SimpleMaterial& mat, vector3 pos, v
             break;
         const LightSpot& l = env.spotLights[i];
         vector3 delta = l.Position - pos;
-        float r = length(delta);
-        vector3 L = normalize(delta);
+        const float d2 = dot(delta, delta), inv = __builtin_amdgcn_rsqf(d2);
+        float r = d2 * inv;
+        vector3 L = delta * inv;
         float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
         float cosA = dot(l.Direction, -L);
         float fang = saturate((cosA - l.CosOuterAngle) * l.CosAngleScale);
