@@ -231,9 +231,13 @@ def serving_bench(args):
         assert all(d for d, _ in ls)
         lat.append(max(l for _, l in ls))
     lat = np.array(lat)
+    def cls_of(r):
+        sc, c, a, fv = r
+        return f"{['100k', '262k', '1M', '2M'][sc]}{' foveated' if fv else ''}"
+
     by_class = {}
-    for (sc, c, a, fv), l in zip(reqs, lat):
-        by_class.setdefault(f"{['100k', '262k', '1M', '2M'][sc]}{' foveated' if fv else ''}", []).append(l)
+    for r, l in zip(reqs, lat):
+        by_class.setdefault(cls_of(r), []).append(l)
     by_class = {k: {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2),
                     "p99_ms": round(float(np.percentile(v, 99)), 2)} for k, v in sorted(by_class.items())}
     subsamples = 0
@@ -248,6 +252,9 @@ def serving_bench(args):
         "msamples_per_s": round(subsamples / elapsed / 1e6, 1),
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "mean_ms": round(float(lat.mean()), 3), "latency_by_scene": by_class,
+        "slowest": [{"request": int(k), "arrival_s": round(float(arrival[k]), 3), "class": cls_of(reqs[k]),
+                     "size_class": int(reqs[k][1]), "msaa": int(reqs[k][2]), "ms": round(float(lat[k]), 2)}
+                    for k in np.argsort(lat)[::-1][:5]],
         "peak_in_flight": st.peakInFlight, "views_created": st.viewsCreated, "views_evicted": st.viewsEvicted,
         "list_growths": st.listGrowths, "failed": st.failed,
         "peak_device_gib": round(st.peakDeviceBytes / (1 << 30), 2), "dtype": "f32 intersect, f16 shade",

@@ -62,6 +62,7 @@ struct PooledView {
     uint64_t bytes = 0;
     double lastUsed = 0.0;
     bool busy = false;
+    const hvvr_gpu_scene* lastScene = nullptr; // the scene of its last frame
 };
 
 struct Request {
@@ -179,14 +180,20 @@ struct hvvr_server {
     }
 
     // an idle view of the shape, or a new one if the budget allows after evicting idle
-    // views of other shapes (least recently used first); null when none fits yet
-    int acquire(uint32_t shapeId, PooledView*& out) {
+    // views of other shapes (least recently used first); null when none fits yet. An idle
+    // view whose last frame was of the same scene comes first: a view keeps per-tile
+    // hints of the tile-list configuration each tile needed on its last scene, and a
+    // dense scene's frame without them walks its deep tiles through every configuration.
+    int acquire(uint32_t shapeId, const hvvr_gpu_scene* scene, PooledView*& out) {
         out = nullptr;
         for (PooledView* pv : pool)
-            if (!pv->busy && pv->shape == shapeId) {
+            if (!pv->busy && pv->shape == shapeId && (!out || pv->lastScene == scene)) {
                 out = pv;
-                return HVVR_GPU_OK;
+                if (pv->lastScene == scene)
+                    break;
             }
+        if (out)
+            return HVVR_GPU_OK;
         const uint64_t need = shapes[shapeId].bytes;
         while (used + need > budget) {
             PooledView* victim = nullptr;
@@ -294,13 +301,14 @@ struct hvvr_server {
         while (!queue.empty() && inFlight.size() < maxInFlight) {
             Request* r = queue.front();
             PooledView* pv = nullptr;
-            int rc = acquire(r->shape, pv);
+            int rc = acquire(r->shape, r->scene, pv);
             if (rc)
                 return rc;
             if (!pv)
                 break; // memory frees up as requests finish
             pv->busy = true;
             pv->lastUsed = nowMs();
+            pv->lastScene = r->scene;
             r->pv = pv;
             queue.pop_front();
             rc = launch(r);

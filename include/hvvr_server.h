@@ -48,7 +48,9 @@ int hvvr_server_submit(hvvr_server* server, hvvr_gpu_scene* scene, uint32_t shap
                        const float* eyeDir, uint32_t* image, uint64_t* requestId);
 /* Completes the finished requests (their images are written), then admits queued ones
  * while a slot is free and their views fit the budget (idle views of other shapes are
- * evicted, oldest first). Returns the number completed, or < 0 on error. */
+ * evicted, oldest first). A request takes an idle view of its shape whose last frame was
+ * of the same scene when there is one (it keeps that scene's per-tile traversal hints).
+ * Returns the number completed, or < 0 on error. */
 int hvvr_server_poll(hvvr_server* server);
 /* Polls until nothing is queued or in flight. */
 int hvvr_server_drain(hvvr_server* server);
