@@ -492,8 +492,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
         if (prof)
             (void)hipEventRecord(v->stageEvents[i], st);
     };
-    HVVR_HIP_CHECK(hipMemsetAsync(v->counters.ptr, 0, v->counters.bytes(), st));
-    mark(0);
+    mark(0); // the frame's counters are zeroed by the frustum setup kernel
     launchFrustumSetup(v, m, st);
     mark(1);
     launchBlockTraverse(v, s, st);

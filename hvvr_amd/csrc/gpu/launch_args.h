@@ -22,6 +22,10 @@ struct FrustumSetupArgs {
     uint32_t tileCount, blockCount;
     matrix4x4 cameraToWorld;
     CullPlanes cull;
+    // a single view's frame counters, zeroed by the first workgroup (the later kernels of
+    // the frame run after this one on the same stream); batches clear theirs separately
+    unsigned long long* counters;
+    uint32_t counterCount;
 };
 
 struct BlockTraverseArgs {

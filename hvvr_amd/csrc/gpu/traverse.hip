@@ -83,7 +83,9 @@ __device__ inline void frustumSetupBody(const FrustumSetupArgs& a, uint32_t i) {
     }
 }
 
-__global__ void frustumSetupKernel(FrustumSetupArgs a) {
+__global__ __launch_bounds__(128) void frustumSetupKernel(FrustumSetupArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x < a.counterCount)
+        a.counters[threadIdx.x] = 0ull;
     frustumSetupBody(a, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -830,6 +832,7 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
         if (n0 == 0 || f.degenerate) {
             if (!emit && lane == 0) {
                 counts[tile] = 0;
+                a.offsets[tile] = 0u;
                 a.tileWork[tile] = 0u;
                 a.tileWork[a.tileCount + tile] = 0u;
             }
@@ -916,7 +919,12 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
         const bool overflow = stackOverflow || ballot(listOverflow) != 0ull;
         if (lane == 0) {
             if (!emit) {
+                // the count pass claims the tile's list space; the emit pass fills it
                 counts[tile] = count;
+                const unsigned long long start = atomicAdd(&counters[3], (unsigned long long)count);
+                a.offsets[tile] = uint32_t(start);
+                if (start + count > capacity)
+                    atomicOr(&counters[1], 1ull);
                 a.tileLevel[tile] = 4u;
                 a.tileWork[tile] = visits; // node visits
                 a.tileWork[a.tileCount + tile] = maxSize;
@@ -951,35 +959,6 @@ __global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __
     }
 }
 
-// List space for the tiles the walk kernel took, after its count pass.
-__device__ inline void walkAllocOne(const TileWalkArgs& a, uint32_t i) {
-    const uint32_t tile = a.queue[i];
-    const uint32_t count = a.counts[tile];
-    const unsigned long long start = atomicAdd(&a.counters[3], (unsigned long long)count);
-    a.offsets[tile] = uint32_t(start);
-    if (start + count > a.capacity)
-        atomicOr(&a.counters[1], 1ull);
-}
-
-__global__ void walkAllocKernel(TileWalkArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < uint32_t(a.counters[8]))
-        walkAllocOne(a, i);
-}
-
-__global__ void walkAllocBatchKernel(const TileWalkArgs* __restrict__ all, uint32_t n) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t base = 0;
-    for (uint32_t v = 0; v < n; v++) {
-        const TileWalkArgs& a = all[v];
-        const uint32_t queued = uint32_t(a.counters[8]);
-        for (uint32_t i = (id + stride - base % stride) % stride; i < queued; i += stride)
-            walkAllocOne(a, i);
-        base += queued;
-    }
-}
-
 FrustumSetupArgs frustumSetupArgs(const hvvr_gpu_view* v, const matrix4x4& m) {
     FrustumSetupArgs a;
     a.tileCam = v->tileFrustaCam.ptr;
@@ -991,6 +970,8 @@ FrustumSetupArgs frustumSetupArgs(const hvvr_gpu_view* v, const matrix4x4& m) {
     a.blockCount = v->blockCount;
     a.cameraToWorld = m;
     a.cull = v->cull;
+    a.counters = reinterpret_cast<unsigned long long*>(v->counters.ptr);
+    a.counterCount = uint32_t(v->counters.count);
     return a;
 }
 
@@ -1169,11 +1150,8 @@ void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream
     const LaunchShape shape;
     const uint32_t T = v->tileCount;
     const TileWalkArgs a = tileWalkArgs(v, s);
-    for (int emit = 0; emit < 2; emit++) {
-        if (emit)
-            walkAllocKernel<<<(T + 63) / 64, 64, 0, stream>>>(a);
+    for (int emit = 0; emit < 2; emit++)
         tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a, emit); // 32 KB of LDS each
-    }
 }
 
 void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
@@ -1192,11 +1170,8 @@ void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32
 
 void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
                          hipStream_t stream) {
-    for (int emit = 0; emit < 2; emit++) {
-        if (emit)
-            walkAllocBatchKernel<<<(maxTiles + 63) / 64, 64, 0, stream>>>(all, viewCount);
+    for (int emit = 0; emit < 2; emit++)
         tileWalkBatchKernel<<<clampGroups(shape.walkGroups, maxTiles), 64, 0, stream>>>(all, viewCount, emit);
-    }
 }
 
 } // namespace hvvr
