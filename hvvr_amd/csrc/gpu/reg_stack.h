@@ -28,6 +28,12 @@ __device__ inline uint32_t writeLane(uint32_t v, uint32_t x, uint32_t l, uint32_
     return lane == l ? x : v;
 }
 
+// lanes 0 .. n-1 (n <= 64), built in scalar registers: a ballot of `lane < n && c` compiles
+// to a compare, a select and a second compare; ballot(c) & lanesBelowMask(n) does not
+__device__ inline uint64_t lanesBelowMask(uint32_t n) {
+    return n >= 64u ? ~0ull : (1ull << n) - 1ull;
+}
+
 // Whole-wave lane shift (DPP wave_shr:1): lane i takes lane i-1, lane 0 keeps `fill`. A DPP
 // read from an inactive lane yields the old value, so the shift must run with all 64
 // lanes active. Written as volatile asm, it stays where it is written (always
@@ -54,7 +60,7 @@ __device__ inline float stackTopKey(const RegStack& st) {
 // entry lands below the run of closer entries at the top (traverse_avx.h:505-515; the
 // sentinel below the bottom is never smaller). The caller keeps size < 64.
 __device__ inline void stackInsert(RegStack& st, float key, uint32_t meta, uint32_t lane) {
-    const uint64_t b = ballot(lane < st.size && !(st.t < key));
+    const uint64_t b = ballot(!(st.t < key)) & lanesBelowMask(st.size);
     const uint32_t p = b ? 64u - uint32_t(__builtin_clzll(b)) : 0u;
     const float ut = bitsf(laneShiftUp(fbits(st.t), 0u));
     const uint32_t um = laneShiftUp(st.m, 0u);
@@ -141,7 +147,7 @@ __device__ inline void stackInsert(RegStackN<N>& st, float key, uint32_t meta, u
 #pragma unroll
     for (int reg = N - 1; reg >= 0; reg--) {
         if (p == 0u && 64u * uint32_t(reg) < st.size) {
-            const uint64_t b = ballot(64u * uint32_t(reg) + lane < st.size && !(st.t[reg] < key));
+            const uint64_t b = ballot(!(st.t[reg] < key)) & lanesBelowMask(st.size - 64u * uint32_t(reg));
             if (b)
                 p = 64u * uint32_t(reg) + (64u - uint32_t(__builtin_clzll(b)));
         }

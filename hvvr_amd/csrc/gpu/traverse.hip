@@ -138,7 +138,7 @@ __device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* 
     // one above the highest entry whose key does not, position 0 when there is none. One
     // ballot finds the position, two DPP shifts per register move the entries above it.
     auto insert = [&](float key, uint32_t eBits, float eT, uint32_t eNode) {
-        const uint64_t notLarger = ballot(lane < top && !(daz(sBits) > key));
+        const uint64_t notLarger = ballot(!(daz(sBits) > key)) & lanesBelowMask(top);
         const uint32_t p = notLarger ? 64u - uint32_t(__builtin_clzll(notLarger)) : 0u;
         const uint32_t ub = laneShiftUp(sBits, 0u), un = laneShiftUp(sNode, 0u);
         const float ut = bitsf(laneShiftUp(fbits(sT), 0u));
