@@ -517,6 +517,43 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         }
     }
 
+    // ---- records whose subtree emits nothing leave the walk ----
+    // Only a record with a passing leaf child emits, and a visit pushes only the record's
+    // own children, so a record none of whose subtree emits changes no emission when it is
+    // visited: its entries are popped later without effect. Dropping it moves no other pop,
+    // as for a record with no passing child (the stack stays sorted, and the order of the
+    // remaining entries, ties included, is the same), and the walk already drops a record
+    // whose mask is clear, so that is how such a record is marked. Children follow their
+    // parent in record order: one pass from the last 64 records to the first, a chunk
+    // repeated while a record in it loses its mask (parents and children in one chunk).
+    if (!overflow) {
+        for (int32_t b = int32_t((nrec - 1u) & ~63u); b >= 0; b -= 64) {
+            const uint32_t r = uint32_t(b) + lane;
+            const bool valid = r < nrec;
+            const uint32_t meta = valid ? sh.recMeta[r] : 0u;
+            const uint32_t mask = metaMask(meta), leaf = metaLeaf(meta);
+            const uint32_t inner = metaEntry(meta) == 0u ? mask & ~leaf : 0u;
+            const uint32_t first = metaFirst<Cfg>(meta);
+            const uint32_t nInner = uint32_t(__builtin_popcount(inner));
+            // emitting records keep their mask; the others keep it while a child is live
+            bool open = valid && mask != 0u && (mask & leaf) == 0u;
+            for (uint32_t round = 0; round < 64u; round++) {
+                bool live = false;
+                for (uint32_t j = 0; j < nInner; j++)
+                    live = live || metaMask(sh.recMeta[first + j]) != 0u;
+                const bool drop = open && !live;
+                waveLdsSync(); // every lane has read its children before any mask changes
+                if (drop)
+                    sh.recMeta[r] = meta & ~(0xfu << 20);
+                waveLdsSync();
+                open = open && !drop;
+                if (ballot(drop) == 0ull)
+                    break;
+            }
+        }
+        waveLdsSync();
+    }
+
     // ---- phase B: the reference's stack walk over the live records, stack in VGPRs ----
     if (!overflow) {
         Stack st;
