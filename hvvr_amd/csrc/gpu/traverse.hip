@@ -235,7 +235,8 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // and does nothing else. Its stack stays sorted by key, ties newest on top (the block
 // stack arrives sorted and every insert lands at its sorted position), so a pop always
 // takes the least (key, newest) entry; an entry that changes nothing when popped can be
-// dropped without moving any other pop. A NaN key breaks that order (it inserts on top
+// dropped without moving any other pop. The same holds for a record none of whose
+// subtree emits (a pass between phases A and B marks those dead too). A NaN key breaks that order (it inserts on top
 // of anything), so a tile with one, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
