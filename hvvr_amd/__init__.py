@@ -551,6 +551,15 @@ class GpuView:
     def reserve_lists(self, entries):
         _check_gpu(self.lib.hvvr_gpu_view_reserve_lists(self.handle, c_uint64(entries)))
 
+    def set_hit_output(self, enable):
+        _check_gpu(self.lib.hvvr_gpu_view_set_hit_output(self.handle, c_int(1 if enable else 0)))
+
+    def read_hits(self):
+        """[sample_count, msaa_rate, 4] float32: (t, u, v, w) of each subsample's nearest hit."""
+        out = np.zeros((self.view.sample_count, self.msaa_rate, 4), np.float32)
+        _check_gpu(self.lib.hvvr_gpu_view_read_hits(self.handle, _ptr(out)))
+        return out
+
     def set_jitter_spin(self, enable):
         _check_gpu(self.lib.hvvr_gpu_view_set_jitter_spin(self.handle, c_int(1 if enable else 0)))
 

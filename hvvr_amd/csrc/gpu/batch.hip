@@ -402,6 +402,7 @@ int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
         hvvr_gpu_view* v = b->views[i];
         v->lastCameraToWorld = m[i];
         v->rendered = true;
+        v->hitsValid = v->hitOutput;
         v->stagesRecorded = false;
         v->lastStream = b->stream;
         v->hintScene = s;

@@ -463,6 +463,25 @@ int hvvr_gpu_view_set_shading_precision(hvvr_gpu_view* v, int bits) {
     return HVVR_GPU_OK;
 }
 
+static int viewWait(hvvr_gpu_view* v);
+
+int hvvr_gpu_view_set_hit_output(hvvr_gpu_view* v, int enable) {
+    if (!v)
+        return fail("null view");
+    if (enable && !v->hits.ptr) {
+        if (v->rendered) {
+            int rc = viewWait(v);
+            if (rc)
+                return rc;
+        }
+        HVVR_HIP_CHECK(hipSetDevice(v->ctx->device));
+        HVVR_HIP_CHECK(v->hits.resize(size_t(v->sampleCount) * v->msaaRate));
+    }
+    v->hitOutput = enable != 0;
+    v->hitsValid = false;
+    return HVVR_GPU_OK;
+}
+
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
     if (!v)
         return fail("null view");
@@ -520,6 +539,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     HVVR_HIP_CHECK(hipGetLastError());
     v->stagesRecorded = prof;
     v->rendered = true;
+    v->hitsValid = v->hitOutput;
     v->hintScene = s; // this frame writes every tile's configuration hint
     v->lastStream = st;
     return HVVR_GPU_OK;
@@ -893,7 +913,8 @@ int hvvr_gpu_view_device_bytes(hvvr_gpu_view* v, uint64_t* bytes) {
                  v->blockStackSize.bytes() + v->tileQueues.bytes() + v->recScratch.bytes() + v->tileLevel.bytes() +
                  v->tileWork.bytes() + v->tileCounts.bytes() + v->tileOffsets.bytes() + v->triIndices.bytes() +
                  v->gbuffer.bytes() + v->sampleResults.bytes() + v->remap.bytes() + v->image.bytes() +
-                 v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes();
+                 v->lensTable.bytes() + v->occRank.bytes() + v->counters.bytes() + v->tMax.bytes() +
+                 v->hits.bytes();
     const FoveatedState& f = v->fov;
     b += f.polarRemap.bytes() + f.polarColor.bytes() + f.polarDepth.bytes() + f.moment1.bytes() +
          f.moment2.bytes() + f.history[0].bytes() + f.history[1].bytes() + f.filtered[0].bytes() +
@@ -975,6 +996,19 @@ int hvvr_gpu_view_read_gbuffer(hvvr_gpu_view* v, uint32_t* triIndex, uint32_t* s
                 break;
         }
     }
+    return HVVR_GPU_OK;
+}
+
+int hvvr_gpu_view_read_hits(hvvr_gpu_view* v, float* hits) {
+    if (!hits)
+        return fail("null argument");
+    int rc = viewSync(v);
+    if (rc)
+        return rc;
+    if (!v->hitsValid)
+        return fail("the last frame was rendered without the hit output (hvvr_gpu_view_set_hit_output)",
+                    HVVR_GPU_ERR_STATE);
+    HVVR_HIP_CHECK(hipMemcpy(hits, v->hits.ptr, v->hits.bytes(), hipMemcpyDeviceToHost));
     return HVVR_GPU_OK;
 }
 

@@ -165,6 +165,11 @@ struct hvvr_gpu_view {
     bool foveated = false;
     hvvr::CullPlanes cull = {};
     hvvr::DeviceBuffer<float> tMax; // [sampleCount]
+    // per-subsample hit output (hvvr_gpu_view_set_hit_output): [sampleCount * A]
+    // (t, u, v, w), written by intersect while enabled
+    bool hitOutput = false;
+    bool hitsValid = false; // the last frame wrote `hits`
+    hvvr::DeviceBuffer<hvvr::vector4> hits;
     hvvr::FoveatedState fov;
     hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
     bool stagesRecorded = false;

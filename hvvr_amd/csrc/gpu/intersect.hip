@@ -21,6 +21,7 @@
 #include "dof.h"
 #include "gpu_internal.h"
 #include "launch_args.h"
+#include "uvw.h"
 
 #include <type_traits>
 
@@ -288,6 +289,33 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
     if (tid == 0)
         args.tileSurvivors[tile] = survivorsSum;
 
+    if (args.hits) {
+        // per-subsample hit output: the nearest hit's ray parameter as the test left it,
+        // and its barycentrics on that subsample's ray (calcUVW, prim_tests.h:233-245, or
+        // its thin-lens twin :366-392), recomputed here from the winning triangle. The
+        // first pass parks (depth, id) in the output so the second need not be unrolled.
+        vector4* out = args.hits + size_t(sample) * A;
+#pragma unroll
+        for (int i = 0; i < A; i++)
+            out[i] = vector4(depth[i], __uint_as_float(hit[i]), 0.0f, 0.0f);
+        for (int i = 0; i < A; i++) {
+            const vector4 h = out[i];
+            const uint32_t id = __float_as_uint(h.y);
+            float bx = 0.0f, by = 0.0f, bz = 0.0f;
+            if (id != badTriIndex) {
+                const PrecomputedTriangleIntersect T = args.tris[id];
+                if constexpr (DoF) {
+                    const vector2 lensUV = args.lensTable[lensBase + i * TILE_SIZE];
+                    const vector2 dirUV = dofDirUV(args.offsets[i], dofTile.focalToLensScale, lensUV);
+                    dofUVW(T, rayOrigin, dofTile, lcfc, lensUV.x, lensUV.y, dirUV.x, dirUV.y, bx, by, bz);
+                } else {
+                    pinholeUVW(T, rayOrigin, ndir, ndu, ndv, args.offsets[i].x, args.offsets[i].y, bx, by, bz);
+                }
+            }
+            out[i] = vector4(h.x, bx, by, bz);
+        }
+    }
+
     GBufferSubsample* waveBase = args.gbuffer + size_t(sample >> 6) * 64 * A;
     emitGBuffer<A>(hit, waveBase, lane);
 }
@@ -317,6 +345,7 @@ IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, con
     a.worldTiles = v->tileFrustaWorld.ptr;
     a.tris = s->triIntersect.ptr;
     a.tileSurvivors = v->tileWork.ptr + size_t(2) * v->tileCount;
+    a.hits = v->hitOutput ? v->hits.ptr : nullptr;
     a.listCapacity = v->listCapacity;
     a.cameraToWorld = m;
     for (int i = 0; i < 32; i++)

@@ -91,6 +91,9 @@ struct IntersectArgs {
     const SimpleRayFrustum* worldTiles;
     const PrecomputedTriangleIntersect* tris;
     uint32_t* tileSurvivors; // per tile: list entries surviving the tile cull
+    // optional per-subsample output [sampleCount * A]: (ray parameter of the nearest hit,
+    // FLT_MAX on a miss; barycentrics u, v, w of that hit, 0 on a miss); null = not written
+    vector4* hits;
     uint64_t listCapacity;
     matrix4x4 cameraToWorld;
     vector2 offsets[32];

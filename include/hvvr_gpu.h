@@ -164,6 +164,13 @@ int hvvr_gpu_view_set_traversal_stack_limit(hvvr_gpu_view* view, uint32_t entrie
  * light in packed FP16 (the scalar D / visibility / Fresnel weights and the radiance sum
  * stay FP32); error stated against the FP64 oracle. */
 int hvvr_gpu_view_set_shading_precision(hvvr_gpu_view* view, int bits);
+/* Per-subsample depth and barycentrics. The reference keeps each subsample's nearest hit
+ * distance in IntersectSamples' sampleTMax (intersect.cu:117-168) and computes barycentrics
+ * with calcUVW (prim_tests.h:233-245, thin lens :366-392) only at shading time; neither is
+ * read back. 1 = the following frames also write, for every subsample, the ray parameter
+ * t of its nearest hit and the barycentrics of that hit on the same ray (allocates
+ * sampleCount * msaaRate * 16 bytes on first use). */
+int hvvr_gpu_view_set_hit_output(hvvr_gpu_view* view, int enable);
 /* JITTER_SAMPLES (kernel_constants.h:15): 1 = rotate the subsample spiral by jitter.x*6.28 */
 int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* view, int enable);
 
@@ -252,6 +259,11 @@ int hvvr_gpu_view_read_sample_colors(hvvr_gpu_view* view, uint32_t* colors); /* 
 /* GBuffer in logical order: for sample s, slots s*A .. s*A+A-1 as (triIndex, sampleMask),
  * ascending triangle id, misses (0xffffffff) last, unused slots (0xffffffff, 0). */
 int hvvr_gpu_view_read_gbuffer(hvvr_gpu_view* view, uint32_t* triIndex, uint32_t* sampleMask);
+/* hits[(sample * A + i) * 4 + k], k = 0: t of subsample i's nearest hit (the ray is
+ * origin + t * (centre + du * alpha.x + dv * alpha.y), unnormalised; FLT_MAX on a miss);
+ * k = 1..3: barycentrics (u, v, w) of that hit, weights of the triangle's vertices 0, 1, 2
+ * (0 on a miss). The last frame must have been rendered with the hit output on. */
+int hvvr_gpu_view_read_hits(hvvr_gpu_view* view, float* hits);
 /* Per-tile triangle lists of the last frame in tile order: counts[tileCount]; lists
  * concatenated into triIndices (capacity `capacity` entries); *total = list entries. */
 int hvvr_gpu_view_read_tile_lists(hvvr_gpu_view* view, uint32_t* counts, uint32_t* triIndices, uint64_t capacity,

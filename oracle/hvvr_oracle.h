@@ -152,6 +152,13 @@ void or_remap(const uint32_t* results, const int32_t* remap, uint32_t width, uin
  * barycentrics (calcUVW at subsample 0), +inf on a miss or in a tile with an empty list. */
 int or_tmax_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
                  const uint32_t* triIds, const o_tri* tris, const o_dof* dof, const uint32_t* counts, float* tMaxOut);
+/* Per subsample (sample*A + i, 3 floats): barycentrics (u, v, w) of the subsample's hit
+ * triangle on that subsample's own ray (IntersectTriangleThread::calcUVW, prim_tests.h:233-245;
+ * with depth of field IntersectTriangleThreadDoF::calcUVW, :366-392, FMAs as in its test),
+ * zero on a miss or in a tile with an empty list. The ray parameter of the hit is the
+ * depth or_intersect_view returns. */
+int or_hits_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                 const uint32_t* triIds, const o_tri* tris, const o_dof* dof, const uint32_t* counts, float* uvwOut);
 /* sRGB byte -> linear (util_graphics.h:16-18), and byte -> linear -> byte (truncated) */
 void or_srgb_tables(float toLinear[256], uint8_t roundTrip[256]);
 /* RemapPolarFoveatedKernel: polar[remap[i]] = sRGB round trip of results[i], depth = tMax[i] */

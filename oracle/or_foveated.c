@@ -337,6 +337,61 @@ static void pinhole_uvw(const o_tri* T, ov3 cam, ov3 dir, ov3 du, ov3 dv, float 
     b[2] = w * dinv;
 }
 
+/* IntersectTriangleThreadDoF::calcUVW (prim_tests.h:366-392) after IntersectTriangleTile<true>::setup
+ * (:263-291) at lens position lensUV and direction offset dirUV */
+static void dof_uvw(const o_tri* T, ov3 cam, const dof_tile* tileLens, ov3 lcfc, ov2 lensUV, ov2 dirUV, float* b) {
+    ov3 lu = tileLens->lensU, lv = tileLens->lensV;
+    ov3 normal = cross3(T->e0, T->e1);
+    ov3 v0ToLens = sub3(cam, T->v0);
+    float dDA0 = dot3(neg3(lu), normal), dDA1 = dot3(neg3(lv), normal);
+    float denomC = dot3(neg3(lcfc), normal);
+    float denom = fmaf(dDA1, dirUV.y, fmaf(dDA0, dirUV.x, denomC));
+    ov3 o = axpy2(v0ToLens, lensUV.x, lu, lensUV.y, lv);
+    ov3 rd = axpy2(lcfc, dirUV.x, lu, dirUV.y, lv);
+    ov3 e = cross3_fma(neg3(rd), o);
+    float v = dot3_fma(T->e1, e);
+    float w = -dot3_fma(T->e0, e);
+    float u = (denom - v) - w;
+    float dinv = 1.0f / denom;
+    b[0] = u * dinv;
+    b[1] = v * dinv;
+    b[2] = w * dinv;
+}
+
+int or_hits_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
+                 const uint32_t* triIds, const o_tri* tris, const o_dof* dof, const uint32_t* counts, float* uvwOut) {
+    if (A < 1 || A > 32 || !counts) return -1;
+    ov3 cam = v3(m[12], m[13], m[14]);
+    uint32_t rank = 0, rankTile = 0, row = 0;
+    dof_tile tileLens;
+    memset(&tileLens, 0, sizeof(tileLens));
+    for (uint32_t s = 0; s < sampleCount; s++) {
+        uint32_t tile = s / 128;
+        if (dof && s % 128 == 0 && counts[tile]) {
+            const o_beam* mid = &beams[(uint64_t)tile * 128 + 64];
+            dof_tile_load(m, m3mul(m, mid->du), m3mul(m, mid->dv), dof->radius, dof->focalDistance, &tileLens);
+            row = dof_row(counts, tile, &rank, &rankTile);
+        }
+        ov3 dir = m3mul(m, beams[s].c);
+        for (uint32_t i = 0; i < A; i++) {
+            float* b = uvwOut + ((uint64_t)s * A + i) * 3;
+            uint32_t tri = triIds[(uint64_t)s * A + i];
+            b[0] = b[1] = b[2] = 0.0f;
+            if (counts[tile] == 0 || tri == 0xffffffffu) continue;
+            if (dof) {
+                ov2 lensUV, dirUV;
+                dof_sample_uvs(dof->lensTable, row, A, i, s % 128, offsets[i], tileLens.focalToLensScale, &lensUV,
+                               &dirUV);
+                dof_uvw(&tris[tri], cam, &tileLens, dof_lens_to_focal(m, dir, dof->focalDistance), lensUV, dirUV, b);
+            } else {
+                pinhole_uvw(&tris[tri], cam, dir, m3mul(m, beams[s].du), m3mul(m, beams[s].dv), offsets[i].x,
+                            offsets[i].y, b);
+            }
+        }
+    }
+    return 0;
+}
+
 int or_tmax_view(const o_beam* beams, uint32_t sampleCount, const float m[16], const ov2* offsets, uint32_t A,
                  const uint32_t* triIds, const o_tri* tris, const o_dof* dof, const uint32_t* counts, float* tMaxOut) {
     if (A < 1 || A > 32 || !counts) return -1;
@@ -364,23 +419,7 @@ int or_tmax_view(const o_beam* beams, uint32_t sampleCount, const float m[16], c
         if (dof) {
             ov2 lensUV, dirUV;
             dof_sample_uvs(dof->lensTable, row, A, 0, s % 128, offsets[0], tileLens.focalToLensScale, &lensUV, &dirUV);
-            ov3 lcfc = dof_lens_to_focal(m, dir, dof->focalDistance);
-            ov3 lu = tileLens.lensU, lv = tileLens.lensV;
-            ov3 normal = cross3(T->e0, T->e1);
-            ov3 v0ToLens = sub3(cam, T->v0);
-            float dDA0 = dot3(neg3(lu), normal), dDA1 = dot3(neg3(lv), normal);
-            float denomC = dot3(neg3(lcfc), normal);
-            float denom = fmaf(dDA1, dirUV.y, fmaf(dDA0, dirUV.x, denomC));
-            ov3 o = axpy2(v0ToLens, lensUV.x, lu, lensUV.y, lv);
-            ov3 rd = axpy2(lcfc, dirUV.x, lu, dirUV.y, lv);
-            ov3 e = cross3_fma(neg3(rd), o);
-            float v = dot3_fma(T->e1, e);
-            float w = -dot3_fma(T->e0, e);
-            float u = (denom - v) - w;
-            float dinv = 1.0f / denom;
-            b[0] = u * dinv;
-            b[1] = v * dinv;
-            b[2] = w * dinv;
+            dof_uvw(T, cam, &tileLens, dof_lens_to_focal(m, dir, dof->focalDistance), lensUV, dirUV, b);
         } else {
             pinhole_uvw(T, cam, dir, m3mul(m, beams[s].du), m3mul(m, beams[s].dv), offsets[0].x, offsets[0].y, b);
         }

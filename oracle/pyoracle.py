@@ -281,6 +281,19 @@ def tmax_view(beams, m, offsets, A, tri_ids, tris, counts, dof=None):
     return out
 
 
+def hits_view(beams, m, offsets, A, tri_ids, tris, counts, dof=None):
+    """Barycentrics (u, v, w) of every subsample's hit on its own ray: [samples, A, 3]."""
+    beams = np.ascontiguousarray(beams)
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.zeros((len(beams), A, 3), np.float32)
+    d, keep = _dof(dof)
+    rc = lib().or_hits_view(_p(beams), c_uint32(len(beams)), _p(m), _p(np.ascontiguousarray(offsets, np.float32)),
+                            c_uint32(A), _p(np.ascontiguousarray(tri_ids, np.uint32)), _p(tris),
+                            ctypes.byref(d) if d else None, _p(np.ascontiguousarray(counts, np.uint32)), _p(out))
+    assert rc == 0
+    return out
+
+
 def srgb_tables():
     lin = np.zeros(256, np.float32)
     rt = np.zeros(256, np.uint8)

@@ -118,6 +118,17 @@ def test_dof_ids_match_brute_force(H, O, scene1):
     assert hits.sum() > 1000
     # depth = distance along the unnormalised direction
     assert np.allclose(depth.reshape(-1)[pick][hits & (got == bid)], bt[hits & (got == bid)], rtol=1e-4)
+    # barycentrics on the lens ray (or_hits_view): the barycentric point is the lens ray's
+    # point at the hit depth
+    uvw = O.hits_view(view.beams, m, offsets, A, ids, tris, counts, dof).reshape(-1, 3).astype(np.float64)
+    sel = pick[hits & (got == bid)]
+    k = ids.reshape(-1)[sel]
+    v0 = tris["v0"][k].astype(np.float64)
+    pb = v0 + uvw[sel, 1:2] * tris["edge0"][k] + uvw[sel, 2:3] * tris["edge1"][k]
+    assert np.allclose(uvw[sel].sum(1), 1, atol=1e-5)
+    pr = o[sel].astype(np.float64) + depth.reshape(-1)[sel, None].astype(np.float64) * d[sel].astype(np.float64)
+    err = np.linalg.norm(pb - pr, axis=1) / np.maximum(1, np.linalg.norm(pr - o[sel], axis=1))
+    assert err.max() <= 1e-4, err.max()
 
 
 def test_vanishing_lens_gives_pinhole_ids(H, O, scene1):
@@ -147,6 +158,7 @@ def test_dof_heightfield_gpu_matches_oracle(H, O, ctx, scene1, A, radius, focal)
     m = _camera(H)
     view = _view(H, radius, focal)
     gs, gv = H.GpuScene(ctx, scene1), H.GpuView(ctx, view, A)
+    gv.set_hit_output(True)
     gv.render(gs, m)
     ref = O.render_view(scene1, view, A, camera_to_world=m)
     counts, lists = gv.read_tile_lists()
@@ -158,6 +170,10 @@ def test_dof_heightfield_gpu_matches_oracle(H, O, ctx, scene1, A, radius, focal)
     assert gv.stats().survivorPairs == ref["survivors"]
     assert rgba_delta(gv.read_sample_colors(), ref["colors"]).max() <= 1
     assert rgba_delta(gv.read_image(), ref["image"]).max() <= 1
+    # per-subsample depth and barycentrics along the lens rays
+    from test_gpu_parity import check_hits
+    same = check_hits(O, gv, ref, view, A, m, dof=(radius, focal, H.lens_table(A)))
+    assert same == view.sample_count * A * 4
 
 
 @pytest.mark.gpu

@@ -1,7 +1,9 @@
 """GPU render path vs the CPU oracle on the same seeded scenes and cameras.
 
 Bar: BVH refit, tile triangle lists, per-subsample triangle ids and subsample masks
-bit-exact; colours within 1/255 per channel (libm pow/sqrt may differ by an ulp)."""
+bit-exact; per-subsample depth and barycentrics within 1e-5 relative (the same float
+operations, so they come out equal); colours within 1/255 per channel (libm pow/sqrt may
+differ by an ulp)."""
 
 import numpy as np
 import pytest
@@ -11,16 +13,40 @@ from conftest import rgba_delta
 pytestmark = pytest.mark.gpu
 
 
-def _render(H, ctx, scene, view, A, limit=None, m=None):
+def _render(H, ctx, scene, view, A, limit=None, m=None, hits=False):
     gs = H.GpuScene(ctx, scene)
     gv = H.GpuView(ctx, view, A)
     if limit:
         gv.set_traversal_stack_limit(limit)
+    if hits:
+        gv.set_hit_output(True)
     gv.render(gs, m)
     return gs, gv
 
 
-def _check_frame(O, gs, gv, scene, view, A, m=None, colors=True):
+def check_hits(O, gv, ref, view, A, m=None, dof=None):
+    """GPU per-subsample (t, u, v, w) against the oracle's depth and barycentrics:
+    within 1e-5 relative (1e-7 absolute for barycentrics near 0); returns the count of
+    bit-identical values."""
+    m = view.camera_to_world if m is None else m
+    got = gv.read_hits()
+    n = view.sample_count
+    want_t = ref["depth"].reshape(n, A)
+    want_b = O.hits_view(view.beams, m, ref["offsets"], A, ref["tri_ids"], ref["tris"], ref["counts"], dof)
+    hit = ref["tri_ids"].reshape(n, A) != 0xFFFFFFFF
+    assert np.array_equal(got[..., 0] != np.float32(np.finfo(np.float32).max), hit), "hit/miss pattern differs"
+    assert np.allclose(got[..., 0][hit], want_t[hit], rtol=1e-5, atol=0), "depth differs"
+    assert np.all(got[..., 0][~hit] == want_t[~hit])
+    assert np.allclose(got[..., 1:], want_b, rtol=1e-5, atol=1e-7), "barycentrics differ"
+    assert np.all(got[..., 1:][~hit] == 0)
+    # barycentrics of a hit sum to 1 and lie in [0, 1] up to rounding
+    b = got[..., 1:][hit].astype(np.float64)
+    assert np.all(np.abs(b.sum(1) - 1) < 1e-4) and b.min() > -1e-4 and b.max() < 1 + 1e-4
+    return int((got[..., 0].view(np.uint32) == want_t.view(np.uint32)).sum() +
+               (got[..., 1:].view(np.uint32) == want_b.view(np.uint32)).sum())
+
+
+def _check_frame(O, gs, gv, scene, view, A, m=None, colors=True, hits=False):
     ref = O.render_view(scene, view, A, camera_to_world=m, with_colors=colors)
     assert np.array_equal(gs.read_nodes().view(np.uint8), ref["nodes"].view(np.uint8)), "refit bounds differ"
     assert np.array_equal(gs.read_triangles().view(np.uint8), ref["tris"].view(np.uint8)), "triangles differ"
@@ -38,13 +64,18 @@ def _check_frame(O, gs, gv, scene, view, A, m=None, colors=True):
         assert rgba_delta(col, ref["colors"]).max() <= 1
         img = gv.read_image()
         assert rgba_delta(img, ref["image"]).max() <= 1
+    if hits:
+        same = check_hits(O, gv, ref, view, A, m)
+        assert same == view.sample_count * A * 4, "depth and barycentrics are expected bit for bit"
     return ref
 
 
 @pytest.mark.parametrize("A", [1, 4])
 def test_heightfield_640x480(H, O, ctx, scene1, view1, A):
-    gs, gv = _render(H, ctx, scene1, view1, A)
-    _check_frame(O, gs, gv, scene1, view1, A)
+    # the CPU reference configuration at full size (640x480, A = 1), with per-subsample
+    # depth and barycentrics
+    gs, gv = _render(H, ctx, scene1, view1, A, hits=True)
+    _check_frame(O, gs, gv, scene1, view1, A, hits=True)
 
 
 @pytest.mark.parametrize("A", [2, 8, 16, 32])
@@ -71,8 +102,25 @@ def test_deep_stack_fallback_same_lists(H, O, ctx, scene2, view2_small, limit):
 def test_room_full_eye_1080x1200(H, O, ctx, scene2):
     # the benchmark's view at full size: eye 0, 1080x1200, barrel distortion, A = 4
     view = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 1080, 1200)
-    gs, gv = _render(H, ctx, scene2, view, 4)
-    _check_frame(O, gs, gv, scene2, view, 4)
+    gs, gv = _render(H, ctx, scene2, view, 4, hits=True)
+    _check_frame(O, gs, gv, scene2, view, 4, hits=True)
+
+
+def test_hit_output_off_by_default_and_readback_guarded(H, ctx, scene1, view1_small):
+    gs, gv = _render(H, ctx, scene1, view1_small, 2)
+    with pytest.raises(H.HvvrError, match="hit output"):
+        gv.read_hits()
+    gv.set_hit_output(True)
+    gv.render(gs)
+    a = gv.read_hits()
+    gv.set_hit_output(False)
+    gv.render(gs)
+    with pytest.raises(H.HvvrError, match="hit output"):
+        gv.read_hits()
+    # the frame's ids do not depend on the hit output
+    gv.set_hit_output(True)
+    gv.render(gs)
+    assert np.array_equal(a, gv.read_hits())
 
 
 def test_render_is_deterministic(H, ctx, scene2, view2_small):

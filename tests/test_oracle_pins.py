@@ -233,12 +233,177 @@ def test_lists_contain_every_hit_and_ids_match_double(H, O, scene1, view1_small)
     assert checked > 1000 and agree == checked and skipped < checked // 10
 
 
+def test_hit_barycentrics_and_depth_match_double(H, O, scene1, view1_small):
+    # per-subsample barycentrics (or_hits_view: calcUVW on the subsample's own ray) against
+    # Moller-Trumbore in double: the reference's v and w weigh vertices 1 and 2 (edge0,
+    # edge1), u = 1 - v - w vertex 0; and the barycentric point is the ray point at depth t
+    A = 4
+    m = view1_small.camera_to_world
+    ref = O.render_view(scene1, view1_small, A, with_colors=False)
+    uvw = O.hits_view(view1_small.beams, m, ref["offsets"], A, ref["tri_ids"], ref["tris"], ref["counts"])
+    rng = np.random.default_rng(5)
+    occupied = np.nonzero(ref["counts"])[0]
+    tiles = list(rng.choice(occupied, size=min(10, len(occupied)), replace=False))
+    origin, dirs = _rays(H, view1_small, m, A, tiles)
+    tris = ref["tris"]
+    v0 = tris["v0"].astype(np.float64)
+    v1 = v0 + tris["edge0"].astype(np.float64)
+    v2 = v0 + tris["edge1"].astype(np.float64)
+    checked = 0
+    errs = []
+    for t, d in zip(tiles, dirs):
+        ids = ref["tri_ids"][t * 128:(t + 1) * 128].reshape(-1)
+        depth = ref["depth"][t * 128:(t + 1) * 128].reshape(-1).astype(np.float64)
+        b = uvw[t * 128:(t + 1) * 128].reshape(-1, 3).astype(np.float64)
+        hit, tt, u_mt, v_mt = _brute(origin, d, tris)
+        for r in np.nonzero(ids != BAD)[0]:
+            k = ids[r]
+            if not hit[r, k]:
+                continue  # a grazing hit the double test rounds the other way
+            w_mt = 1 - u_mt[r, k] - v_mt[r, k]
+            if min(u_mt[r, k], v_mt[r, k], w_mt) < 1e-4:
+                continue
+            want = np.array([w_mt, u_mt[r, k], v_mt[r, k]])
+            errs.append(np.abs(b[r] - want).max())
+            assert abs(depth[r] - tt[r, k]) <= 1e-5 * tt[r, k]
+            p_bary = b[r, 0] * v0[k] + b[r, 1] * v1[k] + b[r, 2] * v2[k]
+            p_ray = origin + depth[r] * d[r]
+            assert np.linalg.norm(p_bary - p_ray) <= 1e-5 * max(1.0, np.linalg.norm(p_ray - origin))
+            checked += 1
+    assert checked > 2000
+    errs = np.asarray(errs)
+    print("bary err max %.3g p99 %.3g median %.3g" % (errs.max(), np.percentile(errs, 99), np.median(errs)))
+    # float32 barycentrics from the differential form lose a few bits to cancellation; a
+    # swapped or dropped term is off by far more
+    assert errs.max() <= 1e-4 and np.percentile(errs, 99) <= 1e-5
+    # misses carry zero barycentrics
+    miss = ref["tri_ids"] == BAD
+    assert np.all(uvw[miss] == 0)
+
+
 def test_tile_lists_have_no_duplicates(O, scene1, view1_small):
     ref = O.render_view(scene1, view1_small, 1, with_colors=False)
     starts = np.concatenate([[0], np.cumsum(ref["counts"])]).astype(np.int64)
     for t in np.nonzero(ref["counts"])[0]:
         seg = ref["lists"][starts[t]:starts[t + 1]]
         assert len(np.unique(seg)) == len(seg)
+
+
+# ---------------------------------------------------------------------------------------
+# tile-list emission order on a hand-built BVH (traverse_avx.h:412-527)
+# ---------------------------------------------------------------------------------------
+def _hand_bvh(H):
+    """Eight nodes in depth-first order, leaf children in the low slots (a leaf child k
+    spans data[k]..data[k+1], an internal child k sits at offset data[1 + k]). Triangle
+    ranges follow node order, so the traversal order shows up as a permutation.
+
+      N0: leaf A [0,2) z[-3,-2]   leaf B [2,3) z[-9,-8]   N1 z[-6,-4]     N2 z[-3,-1]
+      N1: leaf H [3,5)            leaf I [5,6)            N6 z[-6,-5]     N7 z[-5,-4.5]
+      N6: leaf K [6,8)            N7: leaf J [8,9)
+      N2: leaf C [9,11) z[-2,-1.5]  N3 z[-3,-2.5]   N4 z[-3,-2.5] (the same distance key)
+      N3: leaf G [11,12)
+      N4: leaf D [12,13)          N5 z[-2.9,-2.6] (its only internal child)
+      N5: leaf E [13,15)          leaf F [15,16)
+    Every box spans x, y in [-0.2, 0.2]; unused slots are empty boxes (-inf)."""
+    nodes = np.zeros(8, H.DTYPES["node"])
+    nodes["bounds"] = -np.inf
+
+    def child(n, k, z0, z1):
+        b = nodes["bounds"][n]
+        b[0, k], b[1, k], b[2, k], b[3, k] = 0.2, 0.2, 0.2, 0.2  # max x, -min x, max y, -min y
+        b[4, k], b[5, k] = z1, -z0
+
+    def node(n, leaves, internals):
+        """leaves: [(z0, z1, first, end)] in slots 0..; internals: [(z0, z1, child)] after them."""
+        data = np.zeros(5, np.uint32)
+        k = 0
+        for z0, z1, first, end in leaves:
+            child(n, k, z0, z1)
+            data[k], data[k + 1] = first, end
+            nodes["leafMask"][n] |= 1 << k
+            k += 1
+        for z0, z1, c in internals:
+            child(n, k, z0, z1)
+            data[1 + k] = c - n
+            k += 1
+        nodes["data"][n] = data
+
+    node(0, [(-3, -2, 0, 2), (-9, -8, 2, 3)], [(-6, -4, 1), (-3, -1, 4)])
+    node(1, [(-6, -5.5, 3, 5), (-6, -5.5, 5, 6)], [(-6, -5, 2), (-5, -4.5, 3)])
+    node(2, [(-6, -5, 6, 8)], [])
+    node(3, [(-5, -4.5, 8, 9)], [])
+    node(4, [(-2, -1.5, 9, 11)], [(-3, -2.5, 5), (-3, -2.5, 6)])
+    node(5, [(-3, -2.5, 11, 12)], [])
+    node(6, [(-2.9, -2.6, 12, 13)], [(-2.9, -2.6, 7)])
+    node(7, [(-2.9, -2.8, 13, 15), (-2.7, -2.6, 15, 16)], [])
+    return nodes
+
+
+def _wide_tile(H, O):
+    fr = np.zeros(1, H.DTYPES["sfrustum"])
+    fr["directions"] = [[-0.5, -0.5, -1], [0.5, -0.5, -1], [0.5, 0.5, -1], [-0.5, 0.5, -1]]
+    _, pre = O.frustum_setup(fr, np.eye(4, dtype=np.float32).reshape(16))
+    return pre
+
+
+def _entry(node, key, mask=0):
+    return (np.float32(key), np.uint32(node | (mask << 28)))
+
+
+def test_tile_emission_order_hand_built_bvh(H, O):
+    # Distance keys along -z are -zMax of a child box, and the stack pops the smallest key
+    # first; children are inserted in slot order and a later insert goes above an equal
+    # key. Expected, by hand: N0 emits A (0 1) and B (2) in slot order, pushes N1 (key 4)
+    # and N2 (key 1); N2 emits C (9 10) and pushes N3, then N4 (both key 2.5: N4 above);
+    # N4 emits D (12) and, N5 being its only internal child, tests N5 at once (E 13 14,
+    # F 15) although N3's key (2.5) is below N5's (2.6); then N3 (G 11), then N1 (H 3 4,
+    # I 5), which pushes N6 (key 5) then N7 (key 4.5): N7 (J 8) before N6 (K 6 7).
+    nodes = _hand_bvh(H)
+    pre = _wide_tile(H, O)
+    assert O.test_children(pre[0:1], nodes[0:1]) == 0b1111
+    got = O.tile_traverse(nodes, np.array([_entry(0, 0.0)], O.O_STACK), pre[0:1])
+    assert got.tolist() == [0, 1, 2, 9, 10, 12, 13, 14, 15, 11, 3, 4, 5, 8, 6, 7]
+
+
+def test_tile_emission_order_leaf_entry_suppression(H, O):
+    # A block stack as block traversal leaves it for N0 (traverse_avx.h:358-391): one entry
+    # for N0's leaf children (mask = leaf slots) and one per internal child, sorted with the
+    # closest on top: N1 (4), N0's leaves (2), N2 (1). Popping N0's leaf entry emits A and B
+    # but must not push N1 and N2 again (traverse_avx.h:477-483). N2's children (2.5) are
+    # inserted below N0's leaf entry (2), so N0's leaves come right after C:
+    # C (9 10), A B (0 1 2), D E F (12 13 14 15), G (11), H I (3 4 5), J (8), K (6 7).
+    nodes = _hand_bvh(H)
+    pre = _wide_tile(H, O)
+    stack = np.array([_entry(1, 4.0), _entry(0, 2.0, 0b0011), _entry(4, 1.0)], O.O_STACK)
+    got = O.tile_traverse(nodes, stack, pre[0:1])
+    assert got.tolist() == [9, 10, 0, 1, 2, 12, 13, 14, 15, 11, 3, 4, 5, 8, 6, 7]
+    assert len(set(got.tolist())) == 16
+
+
+def test_exact_depth_tie_later_list_entry_wins(H, O):
+    # Two coplanar copies of one triangle: the second test sees depth * denom - t == +0
+    # when 1/denom is exact (denom a power of two), passes the sign test
+    # (prim_tests.h:209-225) and takes the sample: the later list entry wins a tie.
+    v0 = np.array([-8.0, -8.0, -4.0], np.float32)
+    tris = np.zeros(3, H.DTYPES["tri_intersect"])
+    for i in range(3):
+        tris["v0"][i] = v0
+        tris["edge0"][i] = [16, 0, 0]
+        tris["edge1"][i] = [0, 16, 0]
+    tris["v0"][2] = [-8.0, -8.0, -6.0]  # a farther parallel copy
+    beams = np.zeros(128, H.DTYPES["beam"])
+    beams["centerRay"] = [0, 0, -1]
+    fr = np.zeros(1, H.DTYPES["sfrustum"])
+    fr["directions"] = [[-0.01, -0.01, -1], [0.01, -0.01, -1], [0.01, 0.01, -1], [-0.01, 0.01, -1]]
+    m = np.eye(4, dtype=np.float32).reshape(16)
+    world, _ = O.frustum_setup(fr, m)
+    off = np.zeros((1, 2), np.float32)
+    # denom = -dir . (e0 x e1) = 256, t = 4 * 256: depth = 4 exactly
+    for order, want in (([0, 1, 2], 1), ([1, 0, 2], 0), ([2, 0, 1], 1), ([0, 2, 1], 1)):
+        ids, depth, _ = O.intersect_view(beams, world, m, off, 1, np.array([3], np.uint32),
+                                         np.array(order, np.uint32), tris)
+        assert np.all(ids == want), (order, ids[0])
+        assert np.all(depth == 4.0)
 
 
 def test_block_stack_sorted_closest_on_top(O, scene1, view1):
@@ -326,6 +491,124 @@ def test_point_light_falloff_closed_form(H, O):
     spec = 0.04 * (1 / (np.pi * a * a)) / 4
     rad = albedo * np.array([0.15, 0.15, 0.165]) + albedo * 2 * f * (1 - spec) + 2 * f * spec
     assert np.allclose(lin[0], rad, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------------------
+# shading closed forms away from N = V = L = H, written from the published formulas:
+#   D = a^2 / (pi ((n.h)^2 (a^2 - 1) + 1)^2)            GGX / Trowbridge-Reitz (Walter et al. 2007)
+#   F = F0 + (1 - F0) (1 - l.h)^5                        Schlick
+#   G1(x) = x / (x (1 - k) + k), k = a / 2               Schlick's Smith approximation for GGX
+#   f = D F G1(n.l) G1(n.v) / (4 (n.l) (n.v))            Cook-Torrance
+# combined as PBR_BRDF1 does (shading_helpers.h:115-122): light * (base (1 - f) n.l + f), the
+# specular term without the n.l cosine, with a = roughness^2 and roughness = 1 - glossiness
+# ---------------------------------------------------------------------------------------
+def _cook_torrance(n, l, v, rough, f0=0.04):
+    a = rough * rough
+    k = a / 2
+    h = (l + v) / np.linalg.norm(l + v)
+    sat = lambda x: min(max(x, 0.0), 1.0)
+    nh, lh, nl, nv = sat(n @ h), sat(l @ h), sat(n @ l), sat(n @ v)
+    D = a * a / (np.pi * (nh * nh * (a * a - 1) + 1) ** 2)
+    F = f0 + (1 - f0) * (1 - lh) ** 5
+    # G1(nl) G1(nv) / (4 nl nv) without the division by zero at a grazing light
+    vis = 1 / ((nl * (1 - k) + k) * (nv * (1 - k) + k))
+    return D * F * vis / 4, nl
+
+
+def _big_floor(H, roughness, albedo, lights):
+    v = np.zeros(3, H.DTYPES["vertex"])
+    v["pos"] = [[-500, -500, -5], [500, -500, -5], [0, 500, -5]]
+    v["normal"] = np.array([0, 0, 1, 0], np.float16).view(np.uint16)
+    mat = np.zeros(1, H.DTYPES["material"])
+    mat["diffuse"] = [*albedo, 1]
+    mat["specular"] = [0.04, 0.04, 0.04, 1]
+    mat["glossiness"] = 1 - roughness
+    mat["textureIDsAndShadingModel"] = 1 | (0xFFFF << 16) | (0xFFFF << 32) | (0xFFFF << 48)
+    return H.HostScene.from_meshes([(v, [[0, 1, 2]], [0], None)], mat, lights)
+
+
+def _shade_ray(H, O, scene, ray, precision):
+    wv, tris, _ = O.refit(scene)
+    env, dropped = O.lighting_env(scene.lights)
+    assert dropped == 0
+    beams = np.zeros(1, H.DTYPES["beam"])
+    beams["centerRay"] = ray
+    beams["du"] = [1e-3, 0, 0]
+    beams["dv"] = [0, 1e-3, 0]
+    col, lin = O.resolve_view(beams, np.eye(4, dtype=np.float32).reshape(16), np.zeros((1, 2), np.float32), 1,
+                              np.zeros((1, 1), np.uint32), tris, scene.triShade, wv, scene.materials, env, precision)
+    return col[0], lin[0].astype(np.float64)
+
+
+def _unit(x):
+    x = np.asarray(x, np.float64)
+    return x / np.linalg.norm(x)
+
+
+# (camera ray, directional light direction): V along N with L at 60 deg; V and L at +-45 deg
+# (H = N); a grazing view; an out-of-plane pair; a light below the horizon (n.l = 0: the
+# reference still adds the specular term, whose visibility factor is then 1/k)
+GGX_CASES = [((0, 0, -1), _unit([np.sin(np.pi / 3), 0, 0.5])),
+             ((1, 0, -1), _unit([1, 0, 1])),
+             ((8, 0, -1), (0.0, 0.0, 1.0)),
+             ((-0.4, 0.2, -1), _unit([0.3, 0.5, 0.8])),
+             ((0.3, 0, -1), _unit([np.sin(np.radians(100)), 0, np.cos(np.radians(100))]))]
+
+
+@pytest.mark.parametrize("case", range(len(GGX_CASES)))
+@pytest.mark.parametrize("roughness", [0.2, 0.6])
+def test_ggx_closed_form_general_geometry(H, O, case, roughness):
+    ray, L = GGX_CASES[case]
+    albedo = np.array([0.7, 0.4, 0.2])
+    power = np.array([1.2, 1.0, 0.8])
+    light = np.zeros(1, H.DTYPES["light"])
+    light["type"] = 1
+    light["body"][0, :6] = [*L, *power]
+    scene = _big_floor(H, roughness, albedo, light)
+    ray = np.asarray(ray, np.float64)
+    P = ray * (5.0 / -ray[2])
+    n = np.array([0.0, 0.0, 1.0])
+    v = _unit(-P)
+    Lf = np.asarray(light["body"][0, :3], np.float64)  # the float32 light direction
+    rough = 1 - np.float64(np.float32(1 - roughness))
+    f, nl = _cook_torrance(n, Lf, v, rough)
+    rad = albedo * np.array([0.15, 0.15, 0.165]) + albedo * power * (1 - f) * nl + power * f
+    col, lin = _shade_ray(H, O, scene, ray.astype(np.float32), precision=1)
+    assert np.allclose(lin, rad, rtol=1e-5), (lin, rad)
+    col32, lin32 = _shade_ray(H, O, scene, ray.astype(np.float32), precision=0)
+    assert np.allclose(lin32, rad, rtol=2e-3)
+    want = (_srgb(_aces(rad)) * 255).astype(int)
+    got = np.frombuffer(np.uint32(col32).tobytes(), np.uint8)[:3].astype(int)
+    assert np.max(np.abs(got - want)) <= 1
+
+
+@pytest.mark.parametrize("angle,expect", [(0.0, 1.0), (25.0, None), (40.0, 0.0)])
+def test_spot_light_cone_closed_form(H, O, angle, expect):
+    # spot 4 m above the hit point (0, 0, -5), aimed `angle` degrees off the vertical;
+    # cone: outer 30 deg, inner 20 deg, f_angle = sat((cos a - cos outer) / (cos inner - cos outer))
+    # (SpotLightContribution, shading_helpers.h:139-152); distance falloff 2..10 m
+    albedo = np.array([0.5, 0.6, 0.7])
+    color = np.array([3.0, 2.0, 1.0])
+    cos_out, cos_in = np.cos(np.radians(30)), np.cos(np.radians(20))
+    d = [np.sin(np.radians(angle)), 0.0, -np.cos(np.radians(angle))]
+    light = np.zeros(1, H.DTYPES["light"])
+    light["type"] = 3
+    light["body"][0, :13] = [0, 0, -1, 10.0, *d, 1 / 8.0, *color, cos_out, 1 / (cos_in - cos_out)]
+    scene = _big_floor(H, 0.5, albedo, light)
+    b = light["body"][0].astype(np.float64)
+    cos_a = np.asarray(b[4:7]) @ np.array([0, 0, -1.0])
+    f_angle = min(max((cos_a - b[11]) * b[12], 0.0), 1.0)
+    if expect is not None:
+        assert f_angle == expect
+    else:
+        assert 0.3 < f_angle < 0.7
+    f_dist = (10.0 - 4.0) / 8.0
+    rough = 1 - np.float64(np.float32(0.5))
+    f, nl = _cook_torrance(np.array([0, 0, 1.0]), np.array([0, 0, 1.0]), np.array([0, 0, 1.0]), rough)
+    lc = color * f_dist
+    rad = albedo * np.array([0.15, 0.15, 0.165]) + (albedo * lc * (1 - f) * nl + lc * f) * f_angle
+    _, lin = _shade_ray(H, O, scene, np.array([0, 0, -1], np.float32), precision=1)
+    assert np.allclose(lin, rad, rtol=1e-6), (lin, rad)
 
 
 def test_lights_past_maxima_are_dropped(H, O):
