@@ -532,11 +532,17 @@ int hvvr_gpu_batch_autotune(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* c
         }
         std::sort(t.begin(), t.end());
         ms = t[t.size() / 2];
+        // a frame leaves survivors, the deepest stack and node visits per tile; fold them
+        // into each view's counters (the batch frame itself clears those slots)
+        for (hvvr_gpu_view* v : b->views)
+            launchTileWorkReduce(v, b->stream);
+        BATCH_HIP(hipStreamSynchronize(b->stream));
         work.clear();
         for (const hvvr_gpu_view* v : b->views) {
             uint64_t c[5];
             BATCH_HIP(hipMemcpy(c, v->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
             work.push_back(c[0]); // survivor pairs
+            work.push_back(c[2]); // deepest tile stack
             work.push_back(c[3]); // list entries
             work.push_back(c[4]); // node visits
         }

@@ -95,11 +95,11 @@ __global__ __launch_bounds__(64) void ldsStackProbe(const uint32_t* ops, const f
         *finalSize = st.size;
 }
 
-// capacity 64 (RegStack), 256 (RegStack4), 512 (RegStack8) or 2048 (LdsStack)
+// capacity 64 (RegStack), 256 (RegStack4), 512 (RegStack8), 1024 (RegStack16) or 2048 (LdsStack)
 int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t n, uint32_t* popped, float* finalT,
              uint32_t* finalS, uint32_t* finalSize) {
     if (capacity != REG_STACK_CAP && capacity != REG_STACK4_CAP && capacity != REG_STACK8_CAP &&
-        capacity != LDS_PROBE_CAP)
+        capacity != REG_STACK16_CAP && capacity != LDS_PROBE_CAP)
         return -5;
     uint32_t *dOps = nullptr, *dPopped = nullptr, *dS = nullptr, *dSize = nullptr;
     float *dKeys = nullptr, *dT = nullptr;
@@ -117,6 +117,8 @@ int probeRun(uint32_t capacity, const uint32_t* ops, const float* keys, uint32_t
             regStackProbe<RegStack4, REG_STACK4_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         else if (capacity == REG_STACK8_CAP)
             regStackProbe<RegStack8, REG_STACK8_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
+        else if (capacity == REG_STACK16_CAP)
+            regStackProbe<RegStack16, REG_STACK16_CAP><<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         else
             ldsStackProbe<<<1, 64>>>(dOps, dKeys, n, dPopped, dT, dS, dSize);
         if (hipDeviceSynchronize() != hipSuccess)

@@ -128,6 +128,11 @@ def test_batch_autotune_keeps_results(H, ctx, scene2):
     best, ms = batch.autotune(gs, frames_per_trial=2)
     assert ms > 0 and best.largeGroups >= 1
     assert batch.launch_shape().as_dict() == best.as_dict()
+    # the tuner compares per-view survivors, deepest stack, list entries and node visits,
+    # folded from the per-tile work of the frame; a batch frame leaves them nonzero
+    for i, v in enumerate(views):
+        st = v.stats()
+        assert st.survivorPairs > 0 and st.nodeVisits > 0 and st.listEntries > 0, f"view {i}: no work counted"
     batch.render(gs)
     batch.synchronize()
     for i, v in enumerate(views):

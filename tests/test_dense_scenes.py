@@ -104,3 +104,37 @@ def test_dense_scene_server_grows_lists(H, ctx, dense, dense_view):
     srv.close()
     gv.close()
     gs.close()
+
+
+def test_giant_configuration_lists_match(H, O, ctx):
+    # the 2M-triangle room (8,192 meshes) at 256x256: some tiles need more than the huge
+    # configuration's 4,608 records or 512 stack entries and are built by the giant one
+    # (10,240 records, a sixteen-register stack); lists, ids and masks match the oracle bit
+    # for bit on the first frame after the list growth and again on a frame that follows
+    # the per-tile hints
+    scene = H.HostScene(H.SCENE_ROOM, 0, 8192)
+    view = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 256, 256)
+    ref = O.render_view(scene, view, 1, with_colors=False)
+    gs = H.GpuScene(ctx, scene)
+    gv = H.GpuView(ctx, view, 1)
+    gv.render(gs)
+    need, cap, overflowed = gv.list_demand()
+    assert need == int(ref["counts"].sum())
+    if overflowed:
+        gv.reserve_lists(need)
+    rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+    for frame in range(2):
+        gv.render(gs)
+        st = gv.stats()
+        print(f"frame {frame}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
+              f"walk {st.walkTiles}, deepest stack {st.maxTileStack}", flush=True)
+        assert st.overflow == 0 and st.giantTiles > 0, "no tile reached the giant configuration"
+        counts, lists = gv.read_tile_lists()
+        assert np.array_equal(counts, ref["counts"]), f"frame {frame}: tile list lengths differ"
+        assert np.array_equal(lists, ref["lists"]), f"frame {frame}: tile list contents/order differ"
+        tri, mask = gv.read_gbuffer()
+        bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
+        assert len(bad) == 0, f"frame {frame}: {len(bad)} samples differ, first {bad[:5]}"
+        assert st.survivorPairs == ref["survivors"]
+    gv.close()
+    gs.close()

@@ -1,7 +1,7 @@
 """GPU unit probes of wave-level building blocks, against Python models.
 
 The traversal stacks (hvvr_amd/csrc/gpu/reg_stack.h), one register (64 entries), four
-and eight registers (256, 512) and LDS (2048 here): random sequences of sorted inserts and pops that keep
+eight and sixteen registers (256, 512, 1024) and LDS (2048 here): random sequences of sorted inserts and pops that keep
 them shallow, past each register boundary and at capacity; every popped record and the
 final contents must match a list model of the reference's insertion rule
 (traverse_avx.h:499-520: a new entry goes below the run of top entries whose key is
@@ -54,13 +54,13 @@ CASES = {"small": (300, 0.55), "deep": (400, 0.6), "full": (1500, 0.75), "ties":
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cap", [64, 256, 512, 2048])
+@pytest.mark.parametrize("cap", [64, 256, 512, 1024, 2048])
 @pytest.mark.parametrize("case", list(CASES))
 def test_register_stack_matches_model(case, cap):
     lib = ctypes.CDLL(LIB)
     rng = np.random.default_rng(list(CASES).index(case) + cap)
     n, bias = CASES[case]
-    if cap in (256, 512):
+    if cap in (256, 512, 1024):
         n *= cap // 85  # enough pushes to cross every register boundary
     elif cap == 2048:
         n *= 24  # the LDS stack: deep enough for many 64-entry chunks
