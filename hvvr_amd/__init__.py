@@ -536,6 +536,14 @@ class GpuView:
         j = np.asarray(jitter, np.float32)
         _check_gpu(self.lib.hvvr_gpu_view_render(self.handle, scene.handle, _ptr(m), _ptr(j)))
 
+    def render_shared_scratch(self, scene: GpuScene, camera_to_world=None, jitter=(0.0, 0.0)):
+        """render() with the tile-list scratch in the library's per-GPU shared array
+        (hvvr_gpu_view_render_shared_scratch): allocates nothing; calls must not overlap."""
+        m = np.ascontiguousarray(self.view.camera_to_world if camera_to_world is None else camera_to_world,
+                                 np.float32).reshape(16)
+        j = np.asarray(jitter, np.float32)
+        _check_gpu(self.lib.hvvr_gpu_view_render_shared_scratch(self.handle, scene.handle, _ptr(m), _ptr(j)))
+
     def set_traversal_stack_limit(self, entries):
         _check_gpu(self.lib.hvvr_gpu_view_set_traversal_stack_limit(self.handle, c_uint32(entries)))
 

@@ -7,8 +7,27 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string.h>
 #include <vector>
+
+// Shared scratch of hvvr_gpu_view_render_shared_scratch, one copy per GPU for the life of
+// the process: the record scratch of the queued tile-list configurations
+// (kSharedRecordWords words, as many persistent workgroups as it holds), then the four tile
+// queues, [4][tileCount] words. Kernels and host copies use the device address from
+// hipGetSymbolAddress (sharedScratchOf below), never this name.
+enum : uint32_t {
+    kSharedRecordWords = 15u << 18,                       // 15 MiB
+    kSharedMaxTiles = 1u << 16,                           // 8,388,608 samples
+    kSharedScratchWords = kSharedRecordWords + 4u * kSharedMaxTiles // 16 MiB in all
+};
+static __device__ __align__(256) uint32_t scratch[kSharedScratchWords];
+
+namespace {
+enum { kMaxDevices = 64 };
+std::mutex gScratchLock;
+uint32_t* gScratchPtr[kMaxDevices] = {}; // device address of `scratch` on each GPU
+} // namespace
 
 namespace hvvr {
 
@@ -136,8 +155,25 @@ int hvvr_gpu_context_create(int device, hvvr_gpu_context** out) {
         return fail(std::string("device is ") + prop.gcnArchName + ", this build targets gfx950",
                     HVVR_GPU_ERR_NO_DEVICE);
     HVVR_HIP_CHECK(hipSetDevice(device));
+    if (device >= kMaxDevices)
+        return fail("device index beyond the library's device table", HVVR_GPU_ERR_NO_DEVICE);
+    uint32_t* shared = nullptr;
+    {
+        // the shared scratch's address on this device, looked up once per device with the
+        // device current (each GPU holds its own copy of the array)
+        std::lock_guard<std::mutex> lock(gScratchLock);
+        if (!gScratchPtr[device]) {
+            void* p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+            gScratchPtr[device] = static_cast<uint32_t*>(p);
+        }
+        shared = gScratchPtr[device];
+    }
     hvvr_gpu_context* c = new hvvr_gpu_context();
     c->device = device;
+    c->sharedScratch = shared;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     for (auto& vs : c->viewStreams)
         ok = ok && hipStreamCreateWithFlags(&vs, hipStreamNonBlocking) == hipSuccess;
@@ -494,7 +530,7 @@ int hvvr_gpu_view_set_jitter_spin(hvvr_gpu_view* v, int enable) {
 // one frame of one view; `fov` (a foveated frame) switches on the screen cull planes, the
 // t-max output and the polar reconstruction in place of the remap
 static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter, hipStream_t st,
-                     const hvvr_gpu_foveated_frame* fov = nullptr) {
+                     const hvvr_gpu_foveated_frame* fov = nullptr, const TileListScratch* scratch = nullptr) {
     hvvr_gpu_context* c = v->ctx;
     matrix4x4 m;
     memcpy(&m, m16, sizeof(m));
@@ -516,9 +552,9 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     mark(1);
     launchBlockTraverse(v, s, st);
     mark(2);
-    launchTileLists(v, s, st);
+    launchTileLists(v, s, st, scratch);
     mark(3);
-    launchTileListFallback(v, s, st);
+    launchTileListFallback(v, s, st, scratch);
     if (v->dof)
         launchOccupiedRank(v, st);
     mark(4);
@@ -565,6 +601,32 @@ int hvvr_gpu_view_render(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, 
             return rc;
     }
     return renderOne(v, s, m16, jitter, renderStream(v, s));
+}
+
+int hvvr_gpu_view_render_shared_scratch(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, const float* jitter) {
+    if (!v || !s || !m16)
+        return fail("null argument");
+    if (v->ctx != s->ctx)
+        return fail("view and scene belong to different contexts");
+    if (!v->ctx->sharedScratch)
+        return fail("the context has no shared scratch", HVVR_GPU_ERR_STATE);
+    if (v->tileCount > kSharedMaxTiles)
+        return fail("view of " + std::to_string(v->tileCount) + " tiles: the shared scratch holds the tile queues of " +
+                    std::to_string(kSharedMaxTiles) + " tiles (" + std::to_string(kSharedMaxTiles * TILE_SIZE) +
+                    " samples) at most");
+    if (!s->everUpdated || s->dirty) {
+        int rc = hvvr_gpu_scene_update(s); // kernels only: allocates nothing
+        if (rc)
+            return rc;
+    }
+    // Nothing in the shared scratch has to start at zero: a queue entry is read only
+    // below its length, which the frame's frustum setup zeroes in the view's counters,
+    // and a workgroup writes its record scratch for a tile before reading it.
+    TileListScratch sc;
+    sc.rec = v->ctx->sharedScratch;
+    sc.queues = v->ctx->sharedScratch + kSharedRecordWords;
+    tileListGroupsFor(kSharedRecordWords, sc.groups);
+    return renderOne(v, s, m16, jitter, renderStream(v, s), nullptr, &sc);
 }
 
 static int viewSync(hvvr_gpu_view* v);

@@ -93,6 +93,9 @@ struct hvvr_gpu_context {
     hipStream_t viewStreams[hvvr::kViewStreams] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t joins[hvvr::kViewStreams] = {};
+    // device address of the library's shared scratch on this context's GPU
+    // (hvvr_gpu_view_render_shared_scratch)
+    uint32_t* sharedScratch = nullptr;
 };
 
 struct hvvr_gpu_scene {
@@ -192,12 +195,28 @@ struct hvvr_gpu_view {
 
 namespace hvvr {
 
+// Per-frame scratch of the tile-list kernels taken from outside the view (the shared
+// scratch of hvvr_gpu_view_render_shared_scratch): the queued configurations' record
+// scratch, `groups[L]` persistent workgroups' worth for the large, huge and giant
+// configurations (L = 0, 1, 2), and the four tile queues, [4][tileCount].
+struct TileListScratch {
+    uint32_t* rec;
+    uint32_t* queues;
+    uint32_t groups[3];
+};
+// the persistent workgroups of the large, huge and giant configurations whose record
+// scratch fits `words` 32-bit words (at most as many as a view's own scratch holds)
+void tileListGroupsFor(size_t words, uint32_t groups[3]);
+
 // kernel launchers (all asynchronous on `stream`)
 void launchSceneRefit(hvvr_gpu_scene* s, hipStream_t stream);
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream);
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
-void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
-void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
+// `scratch` null: the view's own record scratch and tile queues
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
+                     const TileListScratch* scratch = nullptr);
+void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
+                            const TileListScratch* scratch = nullptr);
 void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream);
 // counters[0], [2] and [4] from the view's per-tile work of its last frame
 void launchTileWorkReduce(hvvr_gpu_view* v, hipStream_t stream);

@@ -1171,23 +1171,47 @@ size_t tileListScratchWords() {
     return std::max(large, std::max(huge, giant));
 }
 
-void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
-    const LaunchShape shape;
-    const uint32_t T = v->tileCount;
-    tileListKernel<<<T, 64, 0, stream>>>(tileListArgs(v, s, 0));
-    // 28 KB of LDS: five workgroups per CU; 63 KB: two; 140 KB: one
-    tileListQueueKernel<TileListLarge><<<clampGroups(std::min(shape.largeGroups, uint32_t(kMaxLargeGroups)), T), 64,
-                                         0, stream>>>(tileListArgs(v, s, 1));
-    tileListQueueKernel<TileListHuge><<<clampGroups(std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups)), T), 64, 0,
-                                        stream>>>(tileListArgs(v, s, 2));
-    tileListQueueKernel<TileListGiant><<<clampGroups(std::min(shape.giantGroups, uint32_t(kMaxGiantGroups)), T), 64,
-                                         0, stream>>>(tileListArgs(v, s, 3));
+void tileListGroupsFor(size_t words, uint32_t groups[3]) {
+    const size_t per[3] = {recScratchWords<TileListLarge>(), recScratchWords<TileListHuge>(),
+                           recScratchWords<TileListGiant>()};
+    const uint32_t most[3] = {kMaxLargeGroups, kMaxHugeGroups, kMaxGiantGroups};
+    for (int k = 0; k < 3; k++)
+        groups[k] = uint32_t(std::min<size_t>(most[k], words / per[k]));
 }
 
-void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream) {
+// the arguments of configuration `level` with the record scratch and queues of `scratch`
+static TileListArgs tileListArgsWith(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level,
+                                     const TileListScratch* scratch) {
+    TileListArgs g = tileListArgs(v, s, level);
+    if (scratch) {
+        const size_t T = v->tileCount;
+        g.recScratch = scratch->rec;
+        g.queue = level > 0 ? scratch->queues + (level - 1) * T : nullptr;
+        g.nextQueue = scratch->queues + level * T;
+    }
+    return g;
+}
+
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream, const TileListScratch* scratch) {
     const LaunchShape shape;
     const uint32_t T = v->tileCount;
-    const TileWalkArgs a = tileWalkArgs(v, s);
+    const uint32_t large = scratch ? scratch->groups[0] : std::min(shape.largeGroups, uint32_t(kMaxLargeGroups));
+    const uint32_t huge = scratch ? scratch->groups[1] : std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups));
+    const uint32_t giant = scratch ? scratch->groups[2] : std::min(shape.giantGroups, uint32_t(kMaxGiantGroups));
+    tileListKernel<<<T, 64, 0, stream>>>(tileListArgsWith(v, s, 0, scratch));
+    // 28 KB of LDS: five workgroups per CU; 63 KB: two; 140 KB: one
+    tileListQueueKernel<TileListLarge><<<clampGroups(large, T), 64, 0, stream>>>(tileListArgsWith(v, s, 1, scratch));
+    tileListQueueKernel<TileListHuge><<<clampGroups(huge, T), 64, 0, stream>>>(tileListArgsWith(v, s, 2, scratch));
+    tileListQueueKernel<TileListGiant><<<clampGroups(giant, T), 64, 0, stream>>>(tileListArgsWith(v, s, 3, scratch));
+}
+
+void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
+                            const TileListScratch* scratch) {
+    const LaunchShape shape;
+    const uint32_t T = v->tileCount;
+    TileWalkArgs a = tileWalkArgs(v, s);
+    if (scratch)
+        a.queue = scratch->queues + 3 * size_t(T);
     for (int emit = 0; emit < 2; emit++)
         tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a, emit); // 32 KB of LDS each
 }

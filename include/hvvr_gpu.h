@@ -111,6 +111,17 @@ int hvvr_gpu_view_destroy(hvvr_gpu_view* view);
  * spin is enabled, see hvvr_gpu_view_set_jitter_spin). Asynchronous. */
 int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float* cameraToWorld,
                          const float* jitter);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc): hvvr_gpu_view_render with
+ * the tile-list kernels' per-frame scratch (the queued configurations' record scratch and
+ * the four tile queues) taken from one 16 MiB device array that the library holds per GPU
+ * for the life of the process, instead of the view's own buffers. Allocates nothing; the
+ * results are those of hvvr_gpu_view_render. Views of more than 65,536 tiles (8,388,608
+ * samples) are rejected (HVVR_GPU_ERR_INVALID). Every call on one GPU uses the same array:
+ * calls whose work overlaps on the device (views on different streams) race, so serialise
+ * them (one stream, or wait for the previous call's frame). */
+int hvvr_gpu_view_render_shared_scratch(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float* cameraToWorld,
+                                        const float* jitter);
 /* Several views of one scene in one submission (batched multi-view). */
 int hvvr_gpu_render_views(hvvr_gpu_scene* scene, uint32_t count, hvvr_gpu_view* const* views,
                           const float* cameraToWorld /* [count][16] */, const float* jitter /* [count][2] or NULL */);
