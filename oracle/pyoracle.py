@@ -216,17 +216,19 @@ def refit(scene):
     return wv, tris, nodes
 
 
-def render_beams(scene, beams, tile_frusta, block_frusta, A, m, lens=(0.0, 1.0), cull_planes=None, precision=0):
+def render_beams(scene, beams, tile_frusta, block_frusta, A, m, lens=(0.0, 1.0), cull_planes=None, precision=0,
+                 offsets=None):
     """Oracle frame of a beam set (no remap): refit, frustum setup (tile frusta outside a
     cull plane skipped), traversal, intersect, resolve with empty tiles cleared, and the
-    per-sample t-max of the foveated path."""
+    per-sample t-max of the foveated path. `offsets`: the frame's subsample offsets (the
+    unspun spiral when None)."""
     m = np.asarray(m, np.float32).reshape(16)
     wv, tris, nodes = refit(scene)
     world_tiles, tile_pre = frustum_setup(tile_frusta, m, cull_planes)
     _, block_pre = frustum_setup(block_frusta, m)
     counts, lists = build_tile_lists(nodes, block_pre, tile_pre)
     from hvvr_amd import lens_table, subsample_offsets
-    offsets = subsample_offsets(A)
+    offsets = subsample_offsets(A) if offsets is None else np.asarray(offsets, np.float32)
     radius, focal = float(lens[0]), float(lens[1])
     dof = (radius, focal, lens_table(A)) if radius > 0 else None
     tri_ids, depth, surv = intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris, dof)

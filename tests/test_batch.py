@@ -153,3 +153,76 @@ def test_batch_rejects_foveated_and_dof_views(H, ctx, scene2):
     with pytest.raises(H.HvvrError):
         batch.render(gs)
     batch.close()
+
+
+def test_batch_1m_scene_refit_graph_matches_oracle(H, O, ctx):
+    # the batched multi-view configuration at reduced view sizes: 16 views of the
+    # 1M-triangle room (4,096 meshes), areas log-uniform between 128x128 and 480x270, MSAA
+    # 1/2/4/8, half of them FP16-shaded, every frame 10% of the vertices displaced by
+    # N(0, 0.02 m) and refit inside the replayed graph. After the last frame every view
+    # matches the oracle of the displaced scene: refit bounds, tile lists, triangle ids and
+    # masks bit for bit, FP32 colours within 1/255 of the FP32 oracle and FP16 colours
+    # within 2/255 of the FP64 one.
+    from concurrent.futures import ThreadPoolExecutor
+
+    from conftest import rgba_delta
+
+    rng = np.random.default_rng(4)
+    scene = H.HostScene(H.SCENE_ROOM, 0, 4096)
+    lo, hi = np.log(128 * 128), np.log(480 * 270)
+    specs = []
+    for k in range(16):
+        area = np.exp(rng.uniform(lo, hi))
+        h = int(round(np.sqrt(area / (16 / 9))))
+        specs.append((int(round(h * 16 / 9)), h, [1, 2, 4, 8][k % 4], k % 2 == 1))
+    hviews = [H.HostView(H.VIEW_PRESET_ROOM_EYE, k & 1, w, h) for k, (w, h, _, _) in enumerate(specs)]
+    gs = H.GpuScene(ctx, scene)
+    views = []
+    for hv, (_, _, a, half) in zip(hviews, specs):
+        gv = H.GpuView(ctx, hv, a)
+        if half:
+            gv.set_shading_precision(16)
+        views.append(gv)
+    cams = [_moved(hv.camera_to_world, k % 5) for k, hv in enumerate(hviews)]
+    batch = H.GpuBatch(ctx, views)
+    batch.set_refit(True)
+    # size every view's lists for this scene first (growth allocates, outside the graph)
+    batch.render(gs, cams)
+    batch.synchronize()
+    for gv in views:
+        need, cap, of = gv.list_demand()
+        if of:
+            gv.reserve_lists(need + need // 4)
+    batch.set_graph(True)
+    verts = scene.verts.copy()
+    n_move = len(verts) // 10
+    for frame in range(3):
+        first = int(rng.integers(0, len(verts) - n_move))
+        verts["pos"][first:first + n_move] += rng.normal(0, 0.02, size=(n_move, 3)).astype(np.float32)
+        gs.set_vertices(first, verts[first:first + n_move])
+        batch.render(gs, cams)
+        batch.synchronize()
+    displaced = H.HostScene.__new__(H.HostScene)
+    displaced.__dict__.update(scene.__dict__)
+    displaced.verts = verts
+
+    def oracle(i):
+        return O.render_view(displaced, hviews[i], specs[i][2], camera_to_world=cams[i],
+                             precision=1 if specs[i][3] else 0)
+
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        refs = list(pool.map(oracle, range(len(views))))
+    assert np.array_equal(gs.read_nodes().view(np.uint8), refs[0]["nodes"].view(np.uint8)), "refit bounds differ"
+    for i, (gv, ref) in enumerate(zip(views, refs)):
+        w, h, a, half = specs[i]
+        what = f"view {i} ({w}x{h}, A={a}, {'fp16' if half else 'fp32'})"
+        assert not gv.list_demand()[2], f"{what}: lists overflowed"
+        counts, lists = gv.read_tile_lists()
+        assert np.array_equal(counts, ref["counts"]) and np.array_equal(lists, ref["lists"]), f"{what}: lists differ"
+        tri, mask = gv.read_gbuffer()
+        rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+        assert np.array_equal(tri, rtri) and np.array_equal(mask, rmask), f"{what}: ids or masks differ"
+        d = rgba_delta(gv.read_sample_colors(), ref["colors"])
+        assert d.max() <= (2 if half else 1), f"{what}: colour differs by {d.max()}/255"
+        assert rgba_delta(gv.read_image(), ref["image"]).max() <= (2 if half else 1), f"{what}: image differs"
+    batch.close()

@@ -258,23 +258,27 @@ def _stage_checks(O, gv, feb, fr, taa_prev):
     # decoded byte re-encoded, an integer in exact arithmetic, so a 1-ulp difference
     # between the GPU's and the host's powf / acosf / logf moves it across the integer and
     # flips the last bit (likewise alpha = 255 x a sum of weights that is 1 in exact
-    # arithmetic). Bound: 1/255 per channel, on a minority of channels.
-    def close(got, want):
+    # arithmetic). Bound: 1/255 per channel, on at most 8% of the channels (measured on
+    # the room scene at 270x300, 540x600 and 1080x1200, pinhole and thin lens: at most
+    # 6.0% after the temporal filter, 3.6% after a blur, 0.3% after the contrast pass).
+    def close(got, want, what):
         d = rgba_delta(got, want)
-        return d.max() <= 1 and (d > 0).mean() < 0.15
+        frac = float((d > 0).mean())
+        print(f"{what}: max {int(d.max())}, channels off by one {frac:.4f}", flush=True)
+        return d.max() <= 1 and frac < 0.08
     w, h = gv.view.width, gv.view.height
     filt = O.foveated_reproject(gpc, gpd, g1, g2, taa_prev, w, h, fr["sample_to_eye"], fr["eye_to_sample_previous"],
                                 feb.emap, feb.max_eccentricity)
     gf = gv.read_foveated("filtered")
-    assert close(gf, filt)
+    assert close(gf, filt, "reprojection + temporal")
     b1 = O.foveated_blur(gf, (0, 1), fr["sample_to_eye"])
     gb1 = gv.read_foveated("blur1")
-    assert close(gb1, b1)
+    assert close(gb1, b1, "blur 1")
     b2 = O.foveated_blur(gb1, (1, 0), fr["sample_to_eye"])
     gb2 = gv.read_foveated("blur2")
-    assert close(gb2, b2)
+    assert close(gb2, b2, "blur 2")
     img = O.foveated_contrast(gf, gb2, fr["sample_to_eye"], feb.max_eccentricity)
-    assert close(gv.read_image(), img)
+    assert close(gv.read_image(), img, "contrast")
     return gf
 
 
@@ -293,3 +297,35 @@ def test_foveated_reconstruction_matches_oracle(H, O, ctx, scene2, feb, screen):
         prev = _stage_checks(O, gv, feb, fr, prev)
     # the second frame did use history: reprojected pixels differ from the no-history answer
     assert np.any(gv.read_foveated("filtered") != gv.read_foveated("previous"))
+
+
+@pytest.mark.gpu
+def test_foveated_full_screen_jitter_dof_matches_oracle(H, O, ctx, scene2, feb, screen):
+    # the foveated + depth-of-field configuration at full size: feb2017 polar samples, 4
+    # subsamples spun by the frame jitter, lens radius 1 cm at a focal distance U[1, 10] m,
+    # gaze U(+-20 deg) per frame, reconstruction to 1080x1200; two frames, the second
+    # blending the first's history
+    A = 4
+    rng = np.random.default_rng(3)
+    focal = float(rng.uniform(1.0, 10.0))
+    gv = H.GpuFoveatedView(ctx, feb, 1080, 1200, A, lens_radius=0.01, focal_distance=focal)
+    gv.set_jitter_spin(True)
+    gs = H.GpuScene(ctx, scene2)
+    hist = H.foveated_history()
+    prev = np.zeros((1200, 1080), np.uint32)
+    for k in range(2):
+        yaw, pitch = rng.uniform(-20.0, 20.0, 2)
+        jitter = rng.random(2).astype(np.float32)
+        fr = H.foveated_frame(screen.camera_to_world, _gaze(yaw, pitch), screen.sample_to_camera,
+                              screen.sample_bounds, hist)
+        gv.render_foveated(gs, fr, jitter)
+        # the spin the view applies: jitter.x * 6.28 in float (gpu_samples.h:52-54)
+        spin = float(np.float32(jitter[0]) * np.float32(6.28))
+        offsets = H.subsample_offsets(A, spin)
+        assert not np.array_equal(offsets, H.subsample_offsets(A))
+        ref = O.render_beams(scene2, feb.beams, feb.tile_frusta, feb.block_frusta, A, fr["eye_to_world"],
+                             (0.01, focal), fr["cull_planes"], offsets=offsets)
+        print(f"frame {k}: gaze ({yaw:.1f}, {pitch:.1f}) deg, jitter {jitter}, focal {focal:.2f} m", flush=True)
+        _check_samples(O, gv, ref, A)
+        assert np.array_equal(gv.read_foveated("previous"), prev)
+        prev = _stage_checks(O, gv, feb, fr, prev)

@@ -195,3 +195,24 @@ def test_batched_views_match_solo_renders(H, ctx, scene1, scene2, view1_small, v
         assert np.array_equal(g[0], g2[0]) and np.array_equal(g[1], g2[1])
         assert np.array_equal(c, gv.read_sample_colors())
         assert np.array_equal(im, gv.read_image())
+
+
+@pytest.mark.parametrize("split", [2, 3])
+def test_split_colour_remap(H, O, ctx, scene2, split):
+    # split colour samples (camera.cpp:21-64, remap.cu:35-77): `split` samples per pixel,
+    # sample n feeding channel n % split, side by side across the pixel; split 2 merges
+    # pentile-style by pixel parity, split 3 takes R, G and B from their own samples
+    w, h = 270, 300
+    base = H.grid_samples(w, h, 0.22, 0.24)
+    samples = np.repeat(base, split)
+    for c in range(split):
+        samples["position"][c::split, 0] += np.float32((c - (split - 1) / 2) / (split * w))
+    sx = np.tan(np.radians(50.0)) / 1.46
+    view = H.HostView.from_samples(samples, w, h, (-sx, -sx * h / w, sx, sx * h / w), split=split)
+    assert view.split == split and view.valid_sample_count == w * h * split
+    m = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, w, h).camera_to_world
+    gs, gv = _render(H, ctx, scene2, view, 4, m=m)
+    ref = _check_frame(O, gs, gv, scene2, view, 4, m=m)
+    # every pixel draws on all of its channel samples
+    assert (view.remap >= 0).all() and len(np.unique(view.remap)) == w * h * split
+    assert (ref["image"] != 0).mean() > 0.9

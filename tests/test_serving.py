@@ -111,3 +111,113 @@ def ctypes_bytes(H, gv):
     out = ctypes.c_uint64()
     H._check_gpu(gv.lib.hvvr_gpu_view_device_bytes(gv.handle, ctypes.byref(out)))
     return out.value
+
+
+@pytest.mark.gpu
+def test_served_images_match_oracle(H, O, ctx, scene1, scene2, shapes):
+    # a grid request on each scene and a foveated one, served together: the grid images
+    # are within 1/255 of the oracle's frame; the foveated image equals a fresh foveated
+    # view's render of the same frame (no history: every request is a first frame), whose
+    # samples match the oracle bit for bit and whose reconstruction stages match it
+    from conftest import rgba_delta
+    from test_foveated import _check_samples, _stage_checks
+
+    hf, room, fov = shapes
+    gscenes = [H.GpuScene(ctx, scene1), H.GpuScene(ctx, scene2)]
+    reqs = _requests(H, scene1, scene2, shapes)[:3]
+    images, st, _ = _serve(H, ctx, gscenes, shapes, reqs, 1 << 40, 4)
+    assert st.completed == 3 and st.failed == 0
+    for (shape, sc, m, _), img, view, A, scene in zip(reqs[:2], images[:2], (hf, room), (4, 2), (scene1, scene2)):
+        ref = O.render_view(scene, view, A, camera_to_world=m)
+        d = rgba_delta(img, ref["image"])
+        assert d.max() <= 1, f"served {shape} image differs from the oracle by {d.max()}/255"
+    _, sc, m, eye = reqs[2]
+    fr = H.foveated_frame(m, eye, room.sample_to_camera, room.sample_bounds, H.foveated_history())
+    gv = H.GpuFoveatedView(ctx, fov, room.width, room.height, 4)
+    gv.render_foveated(gscenes[sc], fr)
+    assert np.array_equal(gv.read_image(), images[2]), "served foveated image differs from a direct render"
+    ref = O.render_beams(scene2, fov.beams, fov.tile_frusta, fov.block_frusta, 4, fr["eye_to_world"], (0.0, 1.0),
+                         fr["cull_planes"])
+    _check_samples(O, gv, ref, 4)
+    _stage_checks(O, gv, fov, fr, np.zeros((room.height, room.width), np.uint32))
+
+
+@pytest.mark.gpu
+def test_serving_mix_four_scenes_matches_oracle(H, O, ctx):
+    # the serving mix at reduced sizes: the four resident room scenes (100k, 262k, 1M and
+    # 2M triangles), FP16 shading, grid requests of several sizes and MSAA 1-8 and polar
+    # foveated ones, all in flight together. Grid images are within 2/255 of the FP64
+    # oracle with ids and masks bit for bit behind them (a fresh view of the same request
+    # reads back the GBuffer); foveated images equal a direct render whose samples match
+    # the oracle bit for bit.
+    from concurrent.futures import ThreadPoolExecutor
+
+    from conftest import rgba_delta
+
+    meshes = [391, 1024, 4096, 8192]
+    hscenes = [H.HostScene(H.SCENE_ROOM, 0, n) for n in meshes]
+    gscenes = [H.GpuScene(ctx, s) for s in hscenes]
+    grid = [(H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 200, 150), 1), (H.HostView(H.VIEW_PRESET_ROOM_EYE, 1, 160, 160), 8),
+            (H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 256, 144), 4), (H.HostView(H.VIEW_PRESET_ROOM_EYE, 1, 128, 96), 2)]
+    fov = H.HostFoveated(H.HostFoveated.DEFAULT)
+    screen = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 270, 300)
+    srv = H.Server(ctx, 8 << 30, 6, shading_bits=16)
+    gsid = [srv.add_shape(v, a) for v, a in grid]
+    fsid = srv.add_foveated_shape(fov, screen, 2)
+    reqs = []  # (scene, grid index or -1, camera, eye)
+    for k in range(8):
+        sc = k % 4
+        m = _camera(grid[k % 4][0].camera_to_world, k)
+        if k % 2 == 0:
+            reqs.append((sc, (k // 2) % 4, m, None))
+        else:
+            g = np.array([0.1 * np.sin(k), 0.05 * k, -1.0], np.float32)
+            reqs.append((sc, -1, _camera(screen.camera_to_world, k), g / np.linalg.norm(g)))
+    out = []
+    for sc, gi, m, eye in reqs:
+        out.append(srv.submit(gscenes[sc], gsid[gi] if gi >= 0 else fsid, m, eye))
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == len(reqs) and st.failed == 0
+    assert st.peakInFlight >= 2
+    for rid, _ in out:
+        assert srv.done(rid)[0]
+    srv.close()
+
+    def oracle(k):
+        sc, gi, m, eye = reqs[k]
+        if gi >= 0:
+            return O.render_view(hscenes[sc], grid[gi][0], grid[gi][1], camera_to_world=m, precision=1)
+        fr = H.foveated_frame(m, eye, screen.sample_to_camera, screen.sample_bounds, H.foveated_history())
+        return fr, O.render_beams(hscenes[sc], fov.beams, fov.tile_frusta, fov.block_frusta, 2, fr["eye_to_world"],
+                                  (0.0, 1.0), fr["cull_planes"], precision=1)
+
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        refs = list(pool.map(oracle, range(len(reqs))))
+    for k, ((sc, gi, m, eye), (_, img), ref) in enumerate(zip(reqs, out, refs)):
+        what = f"request {k} ({meshes[sc]} meshes, {'grid %d' % gi if gi >= 0 else 'foveated'})"
+        if gi >= 0:
+            view, a = grid[gi]
+            d = rgba_delta(img, ref["image"])
+            assert d.max() <= 2, f"{what}: image differs from the FP64 oracle by {d.max()}/255"
+            gv = H.GpuView(ctx, view, a)
+            gv.set_shading_precision(16)
+            gv.reserve_lists(int(ref["counts"].sum()))
+            gv.render(gscenes[sc], m)
+            assert np.array_equal(gv.read_image(), img), f"{what}: served image differs from a direct render"
+            tri, mask = gv.read_gbuffer()
+        else:
+            fr, ref = ref
+            gv = H.GpuFoveatedView(ctx, fov, screen.width, screen.height, 2)
+            gv.set_shading_precision(16)
+            gv.reserve_lists(max(int(ref["counts"].sum()), 1))
+            gv.render_foveated(gscenes[sc], fr)
+            assert np.array_equal(gv.read_image(), img), f"{what}: served image differs from a direct render"
+            counts, lists = gv.read_tile_lists()
+            assert np.array_equal(counts, ref["counts"]) and np.array_equal(lists, ref["lists"]), what
+            tri, mask = gv.read_gbuffer()
+        rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+        assert np.array_equal(tri, rtri) and np.array_equal(mask, rmask), f"{what}: ids or masks differ"
+        gv.close()
+    for g in gscenes:
+        g.close()
