@@ -33,59 +33,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, views, budget_s=20.0, threads=None):
-    """The CPU oracle (C restatement of the reference's traversal + intersect + resolve)
-    timed on a bounded sample of the stereo frame: blocks in a seeded shuffled order, one
-    block per task on a thread pool (the reference's CPU stage runs one task per few blocks on
-    hardware_concurrency threads, render.cpp:344-364, traversal.cpp:249-254), until the
-    time budget is used. ctypes releases the GIL, so the C work runs in parallel."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(scene, views, budget_s=20.0, threads=None):
+    """The reference's CPU stage beside a scalar restatement of its GPU kernels, timed on
+    a bounded sample of the stereo frame (oracle/cpu_baseline.cpp): per task of 2 blocks
+    (traversal.cpp:249-254), frustum setup, the AVX2 block and tile traversal
+    (traverse_avx.h:104-527) with FTZ/DAZ, then the scalar C intersect and resolve of the
+    blocks' samples, on a thread pool. The reference sizes its pool by
+    hardware_concurrency (raycaster.cpp:33-37); this process uses the CPUs it is given
+    (OMP_NUM_THREADS, else its affinity mask), and reports both."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
-    threads = threads or max(1, min(16, os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    wv, tris, nodes = O.refit(scene)
-    env, _ = O.lighting_env(scene.lights)
-    offsets = H.subsample_offsets(A)
-    t_refit = time.perf_counter() - t0
-    tasks = [(e, b) for e in range(len(views)) for b in range(views[e].block_count)]
-    order = np.random.default_rng(0).permutation(len(tasks))
-
-    def block(task):
-        e, b = task
-        v = views[e]
-        m = v.camera_to_world
-        tiles = slice(b * 64, (b + 1) * 64)
-        world, tpre = O.frustum_setup(v.tile_frusta[tiles], m)
-        _, bpre = O.frustum_setup(v.block_frusta[b:b + 1], m)
-        counts, lists = O.build_tile_lists(nodes, bpre, tpre)
-        beams = v.beams[b * 8192:(b + 1) * 8192]
-        ids, _, _ = O.intersect_view(beams, world, m, offsets, A, counts, lists, tris)
-        O.resolve_view(beams, m, offsets, A, ids, tris, scene.triShade, wv, scene.materials, env, 0)
-        return min(8192, v.valid_sample_count - b * 8192) * A  # padding samples are not counted
-
-    samples = 0
-    done = 0
-    t_start = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as pool:
-        # waves of `threads` blocks, passing over both eyes again if time is left
-        while time.perf_counter() - t_start < budget_s:
-            for i in range(0, len(order), threads):
-                batch = [tasks[j] for j in order[i:i + threads]]
-                samples += sum(pool.map(block, batch))
-                done += len(batch)
-                if time.perf_counter() - t_start > budget_s:
-                    break
-    wall = time.perf_counter() - t_start
-    value = samples / wall / 1e6
-    total = len(tasks)
-    return {"value": round(value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{done} block renders ({total} blocks per stereo frame, 8192 samples x {A} subsamples "
-                      f"each, seeded shuffled order over both eyes) in {wall:.1f} s on {threads} threads; C oracle: "
-                      f"frustum setup + block/tile traversal + intersect + resolve; scene refit ({t_refit:.2f} s, "
-                      f"one thread) excluded"}
+    available = len(os.sched_getaffinity(0))
+    threads = threads or min(available, int(os.environ.get("OMP_NUM_THREADS") or available))
+    st = O.cpu_baseline_run(scene, views, A, threads=threads, budget_s=budget_s)
+    value = st.subsamples / st.seconds / 1e6
+    busy = st.frustumSeconds + st.traverseSeconds + st.intersectSeconds + st.resolveSeconds
+    split = {k: round(getattr(st, k + "Seconds") / busy, 3) for k in ("frustum", "traverse", "intersect", "resolve")}
+    blocks = sum(v.block_count for v in views)
+    return {"value": round(value, 4), "unit": "Msamples/s", "cores": int(st.threads), "kind": "port",
+            "cpu_model": cpu_model(), "hardware_concurrency": int(st.hardwareConcurrency),
+            "cpus_available": available, "stage_share": split,
+            "sample": f"{st.blocks} block renders ({blocks} blocks per stereo frame, 8192 samples x {A} subsamples "
+                      f"each, tasks of 2 blocks in a seeded order over both eyes) in {st.seconds:.1f} s on "
+                      f"{st.threads} threads: frustum setup + AVX2 block/tile traversal (FTZ/DAZ) + scalar C "
+                      f"intersect + resolve; scene refit excluded"}
 
 
 def copy_calibration(mib):
@@ -117,27 +99,56 @@ def measured_traffic(kernel):
     return None if entry is None else int(entry["read_bytes"] + entry["write_bytes"])
 
 
+def measured_counters(kernel):
+    """The committed SQ/L2 counter summary's row for `kernel` (l2_hit, valu_busy,
+    valu_util, salu_share, ...), or {} if it has none."""
+    path = os.path.join(ROOT, "profiles", "pmc_counters_stereo_1080x1200_a4.md")
+    if not os.path.exists(path):
+        return {}
+    head = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == "kernel":
+            head = cells
+        elif head and cells and cells[0].split("<")[0] == kernel.split("<")[0] and len(cells) == len(head):
+            return {h: float(c) for h, c in zip(head[1:], cells[1:])}
+    return {}
+
+
 def roofline(stage_ms, survivors, entries, visits):
     """Roofline of the dominant kernel, per launch (one launch per eye).
-    tile_lists: HBM-class bytes = 128 B per BVH node visit + 4 B per list entry written
-    (the node reads mostly hit L2/MALL, which `traffic` shows);
-    intersect: FP32 flops = N_pair (80 + 16 A), N_pair = tile-cull survivors x 128."""
+    tile_lists: algorithmic bytes = 128 B per BVH node visit + 4 B per list entry written,
+    against HBM; the node reads mostly hit L2, so beside it stand the HBM bytes the
+    counters measured (`traffic`, its fraction of 8 TB/s) and the L2 hit rate: the kernel
+    is bound by the issue and dependency latency of its stack walk, not by either;
+    intersect: FP32 VALU flops = N_pair (80 + 16 A), N_pair = tile-cull survivors x 128,
+    against the FP32 vector figure."""
     dom = STAGES[int(np.argmax(stage_ms))]
-    if dom == "intersect":
-        flops = survivors * (80 + 16 * A)
-        ach = flops / (stage_ms[STAGES.index("intersect")] * 1e-3)
-        return {"kernel": "intersectKernel<4>", "bound": "mfma", "achieved": round(ach / 1e12, 3),
-                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32, 4),
-                "traffic": measured_traffic("intersectKernel<4>"),
-                "note": f"FP32 VALU work (no MFMA on this path) against the 157.3 TFLOP/s FP32 figure; "
-                        f"F = N_pair*(80+16A), N_pair={int(survivors)} per launch"}
-    bytes_ = 128 * visits + 4 * entries
     t = stage_ms[STAGES.index(dom)] * 1e-3
+    if dom == "intersect":
+        name = "intersectKernel<4>"
+        flops = survivors * (80 + 16 * A)
+        ach = flops / t
+        pmc = measured_counters(name)
+        return {"kernel": name, "bound": "valu", "achieved": round(ach / 1e12, 3),
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32, 4),
+                "traffic": measured_traffic(name), "valu_busy_pct": pmc.get("valu_busy"),
+                "valu_lane_util_pct": pmc.get("valu_util"),
+                "note": f"FP32 VALU work (no matrix cores on this path) against the 157.3 TFLOP/s FP32 vector "
+                        f"figure (78.6 without packed FP32, which the build disables); F = N_pair*(80+16A), "
+                        f"N_pair={int(survivors)} per launch"}
+    bytes_ = 128 * visits + 4 * entries
     name = {"tile_lists": "tileListKernel", "resolve": "resolveKernel<4>"}.get(dom, dom)
+    traffic = measured_traffic(name)
+    pmc = measured_counters(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(bytes_ / t / 1e9, 3), "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s", "frac": round(bytes_ / t / PEAK_HBM, 5), "traffic": measured_traffic(name),
-            "note": f"latency-bound traversal; algorithmic bytes = 128 B x {int(visits)} node visits + 4 B x "
-                    f"{int(entries)} list entries per launch"}
+            "unit": "GB/s", "frac": round(bytes_ / t / PEAK_HBM, 5), "traffic": traffic,
+            "hbm_counter_frac": None if traffic is None else round(traffic / t / PEAK_HBM, 5),
+            "l2_hit_pct": pmc.get("l2_hit"), "salu_share": pmc.get("salu_share"),
+            "valu_busy_pct": pmc.get("valu_busy"),
+            "note": f"issue- and latency-bound stack walk; algorithmic bytes = 128 B x {int(visits)} node visits + "
+                    f"4 B x {int(entries)} list entries per launch, mostly L2 hits (l2_hit_pct); the HBM bytes "
+                    f"the counters measured are `traffic` (hbm_counter_frac of 8 TB/s)"}
 
 
 def serving_bench(args):

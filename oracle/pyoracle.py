@@ -376,3 +376,78 @@ def emap_apply_inverse(emap, e_rad):
     L.or_emap_apply_inverse.restype = ctypes.c_float
     e = (ctypes.c_float * 6)(*np.asarray(emap, np.float32))
     return L.or_emap_apply_inverse(e, ctypes.c_float(e_rad))
+
+
+# ---------------------------------------------------------------------------------------
+# CPU baseline (oracle/_build/libcpu_baseline.so): the AVX2 traversal on a thread pool
+# ---------------------------------------------------------------------------------------
+BASELINE_LIB = os.path.join(_HERE, "_build", "libcpu_baseline.so")
+_baseline = None
+
+
+class CbView(ctypes.Structure):
+    _fields_ = [("beams", c_void_p), ("tileFrusta", c_void_p), ("blockFrusta", c_void_p), ("blockCount", c_uint32),
+                ("validSampleCount", c_uint32), ("cameraToWorld", ctypes.c_float * 16)]
+
+
+class CbStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("subsamples", c_uint64), ("blocks", c_uint64),
+                ("frustumSeconds", ctypes.c_double), ("traverseSeconds", ctypes.c_double),
+                ("intersectSeconds", ctypes.c_double), ("resolveSeconds", ctypes.c_double), ("threads", c_uint32),
+                ("hardwareConcurrency", c_uint32), ("status", ctypes.c_int32)]
+
+
+def baseline_lib():
+    global _baseline
+    if _baseline is None:
+        lib()  # liboracle.so first (libcpu_baseline.so links it)
+        if not os.path.exists(BASELINE_LIB):
+            build()
+        L = ctypes.CDLL(BASELINE_LIB)
+        L.cb_build_tile_lists.restype = ctypes.c_int64
+        _baseline = L
+    return _baseline
+
+
+def avx_build_tile_lists(nodes, block_pre, tile_pre):
+    """build_tile_lists through the AVX2 traversal of the CPU baseline."""
+    nodes = np.ascontiguousarray(nodes)
+    counts = np.zeros(len(tile_pre), np.uint32)
+    L = baseline_lib()
+    total = L.cb_build_tile_lists(_p(nodes), _p(block_pre), _p(tile_pre), c_uint32(len(block_pre)), _p(counts), None,
+                                  c_uint64(0))
+    if total < 0:
+        raise RuntimeError("AVX2 traversal overflowed")
+    lists = np.zeros(max(total, 1), np.uint32)
+    total2 = L.cb_build_tile_lists(_p(nodes), _p(block_pre), _p(tile_pre), c_uint32(len(block_pre)), _p(counts),
+                                   _p(lists), c_uint64(len(lists)))
+    assert total2 == total
+    return counts, lists[:total]
+
+
+def cpu_baseline_run(scene, views, A, threads=0, budget_s=20.0, seed=0):
+    """Times the CPU baseline over the views' blocks (tasks of 2 blocks, seeded order,
+    repeated) for budget_s seconds on `threads` threads (0 = hardware_concurrency).
+    Returns the CbStats."""
+    from hvvr_amd import subsample_offsets
+    wv, tris, nodes = refit(scene)
+    env, _ = lighting_env(scene.lights)
+    offsets = np.ascontiguousarray(subsample_offsets(A), np.float32)
+    keep = [wv, tris, nodes, env, offsets]
+    cviews = (CbView * len(views))()
+    for cv, v in zip(cviews, views):
+        arrs = [np.ascontiguousarray(a) for a in (v.beams, v.tile_frusta, v.block_frusta)]
+        keep += arrs
+        cv.beams, cv.tileFrusta, cv.blockFrusta = (a.ctypes.data for a in arrs)
+        cv.blockCount, cv.validSampleCount = v.block_count, v.valid_sample_count
+        cv.cameraToWorld[:] = [float(x) for x in np.asarray(v.camera_to_world, np.float32).reshape(16)]
+    shade = np.ascontiguousarray(scene.triShade)
+    mats = np.ascontiguousarray(scene.materials)
+    keep += [shade, mats]
+    st = CbStats()
+    rc = baseline_lib().cb_run(_p(nodes), _p(tris), _p(shade), _p(wv), _p(mats), _p(env), cviews,
+                               c_uint32(len(views)), _p(offsets), c_uint32(A), c_uint32(threads),
+                               ctypes.c_double(budget_s), c_uint32(seed), ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("CPU baseline: a tile traversal overflowed its stack")
+    return st
