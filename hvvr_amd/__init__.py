@@ -215,14 +215,18 @@ class HostView:
             self.handle = None
 
     @classmethod
-    def from_samples(cls, samples, width, height, viewport, split=1, lens_radius=0.0, focal_distance=1.0):
+    def from_samples(cls, samples, width, height, viewport, split=1, lens_radius=0.0, focal_distance=1.0,
+                     spherical=None):
+        """spherical = (fovXDegrees, fovYDegrees): rays from a spherical section
+        (Camera::setSphericalWarpSettings) instead of the viewport's perspective."""
         lib = load_host()
         samples = np.ascontiguousarray(samples, DTYPES["sample"])
         vp = np.asarray(viewport, np.float32)
+        fx, fy = spherical if spherical is not None else (0.0, 0.0)
         h = c_void_p()
-        _check_host(lib.hvvr_host_view_create_from_samples(
+        _check_host(lib.hvvr_host_view_create_spherical(
             _ptr(samples), c_uint32(len(samples)), c_uint32(split), c_uint32(width), c_uint32(height), _ptr(vp),
-            c_float(lens_radius), c_float(focal_distance), byref(h)))
+            c_float(fx), c_float(fy), c_float(lens_radius), c_float(focal_distance), byref(h)))
         return cls(_handle=h)
 
 

@@ -3,10 +3,14 @@
 // modelviewer/modelviewer.cpp: create models, lights and a camera, set samples and a
 // render target, call render() every frame).
 //
-//   render_scene <scene.bin> <out.rgba> [frames] [foveated]
+//   render_scene <scene.bin> <out.rgba> [frames] [foveated | spherical=FX,FY | share-samples]
 //
 // "foveated" renders in GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT mode with the feb2017
-// sample pattern (raycaster_spec.h:38-47), gaze straight ahead.
+// sample pattern (raycaster_spec.h:38-47), gaze straight ahead. "spherical=FX,FY" sets
+// Camera::setSphericalWarpSettings(FX, FY) before the samples (rays from a spherical
+// section of FX x FY degrees, as modelviewer.cpp:168 does for its wide view).
+// "share-samples" adds a second camera that takes the first one's sample hierarchy
+// through getSampleData / setSampleData (camera.h:96-97); its image follows the first's.
 //
 // scene.bin (little endian, packed):
 //   char[4] "HVSC", u32 version = 1
@@ -62,7 +66,8 @@ struct Reader {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s scene.bin out.rgba [frames] [foveated]\n", argv[0]);
+        fprintf(stderr, "usage: %s scene.bin out.rgba [frames] [foveated | spherical=FX,FY | share-samples]\n",
+                argv[0]);
         return 2;
     }
     int frames = argc > 3 ? atoi(argv[3]) : 1;
@@ -112,9 +117,13 @@ int main(int argc, char** argv) {
             moved = r.getTransform();
         fclose(f);
 
+        const std::string mode = argc > 4 ? argv[4] : "";
+        float fovX = 0.0f, fovY = 0.0f;
+        if (mode.rfind("spherical=", 0) == 0 && sscanf(mode.c_str() + 10, "%f,%f", &fovX, &fovY) != 2)
+            throw std::runtime_error("spherical=FX,FY: two angles in degrees");
         RayCasterSpecification spec;
         spec.msaaRate = msaa;
-        if (argc > 4 && std::string(argv[4]) == "foveated") {
+        if (mode == "foveated") {
             spec.mode = RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT;
             spec.foveatedSamplePattern.degreeTrackingError = 7.0f;
             spec.foveatedSamplePattern.fovealMARDegrees = 1.0f / 20.0f;
@@ -141,9 +150,21 @@ int main(int argc, char** argv) {
         target.height = height;
         target.stride = width;
         camera->setRenderTarget(target);
+        if (fovX > 0.0f)
+            camera->setSphericalWarpSettings(fovX, fovY);
         std::vector<Sample> samples = getGridSamples(width, height);
         camera->setSamples(samples.data(), uint32_t(samples.size()), 1);
         camera->setCameraToWorld(cameraToWorld);
+        std::vector<uint32_t> image2;
+        if (mode == "share-samples") {
+            Camera* second = raycaster.createCamera(camera->getViewport(), 0.0f);
+            image2.assign(image.size(), 0);
+            ImageResourceDescriptor t2 = target;
+            t2.data = image2.data();
+            second->setRenderTarget(t2);
+            second->setSampleData(camera->getSampleData());
+            second->setCameraToWorld(cameraToWorld);
+        }
 
         FILE* out = fopen(argv[2], "wb");
         if (!out)
@@ -156,6 +177,8 @@ int main(int argc, char** argv) {
             ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
         }
         fwrite(image.data(), 4, image.size(), out);
+        if (!image2.empty())
+            fwrite(image2.data(), 4, image2.size(), out);
         if (moveModel && !handles.empty()) {
             handles[0]->setTransform(moved);
             raycaster.render(0.0);

@@ -171,8 +171,8 @@ int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint3
         applyBarrelDistortion(samples, cam.barrelK1, cam.barrelK2);
     hvvr_host_view* v = new hvvr_host_view();
     v->sampleToCam = sampleToCamera(cam.viewportLower, cam.viewportUpper);
-    buildSampleSetup(samples.data(), uint32_t(samples.size()), 1, cam.width, cam.height, v->sampleToCam, cam.lens,
-                     v->setup);
+    buildSampleSetup(samples.data(), uint32_t(samples.size()), 1, cam.width, cam.height,
+                     Sample2Dto3DMappingSettings(v->sampleToCam, cam.lens), v->setup);
     v->cameraToWorld = matrix4x4(cam.cameraToWorld);
     v->lens = cam.lens;
     *out = v;
@@ -182,12 +182,26 @@ int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint3
 int hvvr_host_view_create_from_samples(const void* samples, uint32_t count, uint32_t split, uint32_t width,
                                        uint32_t height, const float vp[4], float lensRadius, float focalDistance,
                                        hvvr_host_view** out) {
+    return hvvr_host_view_create_spherical(samples, count, split, width, height, vp, 0.0f, 0.0f, lensRadius,
+                                           focalDistance, out);
+}
+
+int hvvr_host_view_create_spherical(const void* samples, uint32_t count, uint32_t split, uint32_t width,
+                                    uint32_t height, const float vp[4], float fovXDegrees, float fovYDegrees,
+                                    float lensRadius, float focalDistance, hvvr_host_view** out) {
     if (!out || !samples || !vp || split < 1 || split > 3)
         return fail("invalid argument");
+    if (!(fovXDegrees >= 0.0f && fovYDegrees >= 0.0f && fovXDegrees <= 360.0f && fovYDegrees <= 180.0f))
+        return fail("spherical section: angles in [0, 360] x [0, 180] degrees");
     hvvr_host_view* v = new hvvr_host_view();
     ThinLens lens = {lensRadius, focalDistance};
     v->sampleToCam = sampleToCamera(vector2(vp[0], vp[1]), vector2(vp[2], vp[3]));
-    buildSampleSetup((const Sample*)samples, count, split, width, height, v->sampleToCam, lens, v->setup);
+    // Camera::get2DSampleMappingSettings (camera.cpp:206-212): a nonzero horizontal angle
+    // selects the spherical section
+    Sample2Dto3DMappingSettings mapping =
+        fovXDegrees > 0.0f ? Sample2Dto3DMappingSettings::sphericalSection(v->sampleToCam, lens, fovXDegrees, fovYDegrees)
+                           : Sample2Dto3DMappingSettings(v->sampleToCam, lens);
+    buildSampleSetup((const Sample*)samples, count, split, width, height, mapping, v->setup);
     v->lens = lens;
     *out = v;
     return 0;

@@ -46,23 +46,13 @@ static_assert(sizeof(Sample) == 28, "Sample layout");
 void applyBarrelDistortion(std::vector<Sample>& samples, float k1, float k2);
 
 // Camera-space sample hierarchy ready for rendering (raycaster/camera.h:28-47).
-struct SampleSetup {
-    uint32_t validSampleCount = 0;
-    uint32_t sampleCount = 0; // padded to a multiple of BLOCK_SIZE
-    uint32_t blockCount = 0;
-    uint32_t splitColorSamples = 1;
-    uint32_t width = 0, height = 0;
-    std::vector<DirectionalBeam> beams;         // [sampleCount]
-    std::vector<SimpleRayFrustum> tileFrusta;   // [blockCount * TILES_PER_BLOCK]
-    std::vector<SimpleRayFrustum> blockFrusta;  // [blockCount]
-    std::vector<int32_t> remap;                 // [width * height * split], -1 = no sample
-    vector2 boundsLower, boundsUpper;
-};
+using SampleSetup = SampleData;
 // camera.cpp:315-319
 matrix3x3 sampleToCamera(vector2 viewportLower, vector2 viewportUpper);
-// camera.cpp:21-64 + sample_hierarchy.cpp:149-293 (perspective mapping)
+// camera.cpp:21-64 + sample_hierarchy.cpp:27-190: the perspective mapping, or the
+// spherical section when settings.type says so
 void buildSampleSetup(const Sample* samples, uint32_t count, uint32_t split, uint32_t width, uint32_t height,
-                      const matrix3x3& sampleToCam, ThinLens lens, SampleSetup& out);
+                      const Sample2Dto3DMappingSettings& settings, SampleSetup& out);
 
 // ---------------------------------------------------------------------------------
 // polar foveated path (raycaster/foveated.cpp, samples.cpp:44-142, frusta.cu:26-121)

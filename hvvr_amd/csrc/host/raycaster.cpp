@@ -119,8 +119,38 @@ void Camera::setSamples(const Sample* samples, uint32_t count, uint32_t splitCol
         throw RaycasterError("Camera::setSamples: splitColorSamples must be 1, 2 or 3");
     _samples.assign(samples, samples + count);
     _split = splitColorSamples;
+    _sampleDataGiven = false;
     _viewDirty = true;
     // the reference builds the sample hierarchy here (camera.cpp:135-156)
+    rebuildView();
+}
+
+void Camera::setSphericalWarpSettings(float fovXDegrees, float fovYDegrees) {
+    if (!(fovXDegrees >= 0.0f && fovYDegrees >= 0.0f && fovXDegrees <= 360.0f && fovYDegrees <= 180.0f))
+        throw RaycasterError("Camera::setSphericalWarpSettings: angles in [0, 360] x [0, 180] degrees");
+    _fovXDegrees = fovXDegrees;
+    _fovYDegrees = fovYDegrees;
+    if (!_sampleDataGiven)
+        _viewDirty = true;
+}
+
+// camera.cpp:206-212
+Sample2Dto3DMappingSettings Camera::get2DSampleMappingSettings() const {
+    if (_fovXDegrees > 0.0f)
+        return Sample2Dto3DMappingSettings::sphericalSection(getSampleToCamera(), _lens, _fovXDegrees, _fovYDegrees);
+    return Sample2Dto3DMappingSettings(getSampleToCamera(), _lens);
+}
+
+void Camera::setSampleData(const SampleData& sampleData) {
+    const SampleData& d = sampleData;
+    if (d.blockCount * 8192u != d.sampleCount || d.beams.size() != d.sampleCount ||
+        d.tileFrusta.size() != size_t(d.blockCount) * 64 || d.blockFrusta.size() != d.blockCount ||
+        d.validSampleCount > d.sampleCount || d.splitColorSamples < 1 || d.splitColorSamples > 3 ||
+        d.remap.size() != size_t(d.width) * d.height * d.splitColorSamples)
+        throw RaycasterError("Camera::setSampleData: inconsistent sample data");
+    _sampleData = sampleData;
+    _sampleDataGiven = true;
+    _viewDirty = true;
     rebuildView();
 }
 
@@ -134,22 +164,31 @@ void Camera::rebuildView() {
         _view = nullptr;
     }
     _validSampleCount = 0;
-    if (_samples.empty() || _renderTarget.width == 0 || _renderTarget.height == 0) {
+    const bool haveSamples = _sampleDataGiven ? _sampleData.validSampleCount > 0 : !_samples.empty();
+    if (!haveSamples || _renderTarget.width == 0 || _renderTarget.height == 0) {
         _viewDirty = true; // retried on the next render once samples and a target exist
         return;
     }
-    for (const Sample& s : _samples)
-        if (s.pixelLocation.x >= _renderTarget.width || s.pixelLocation.y >= _renderTarget.height)
-            throw RaycasterError("Camera::setSamples: sample pixel location outside the render target");
+    if (_sampleDataGiven) {
+        if (_sampleData.width != _renderTarget.width || _sampleData.height != _renderTarget.height)
+            throw RaycasterError("Camera::setSampleData: the sample data maps a " + std::to_string(_sampleData.width) +
+                                 "x" + std::to_string(_sampleData.height) + " render target, not " +
+                                 std::to_string(_renderTarget.width) + "x" + std::to_string(_renderTarget.height));
+    } else {
+        for (const Sample& s : _samples)
+            if (s.pixelLocation.x >= _renderTarget.width || s.pixelLocation.y >= _renderTarget.height)
+                throw RaycasterError("Camera::setSamples: sample pixel location outside the render target");
+    }
 
     if (_raycaster._spec.mode == RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT) {
         rebuildFoveatedView();
         return;
     }
 
-    SampleSetup setup;
-    buildSampleSetup(_samples.data(), uint32_t(_samples.size()), _split, _renderTarget.width,
-                     _renderTarget.height, getSampleToCamera(), _lens, setup);
+    if (!_sampleDataGiven)
+        buildSampleSetup(_samples.data(), uint32_t(_samples.size()), _split, _renderTarget.width,
+                         _renderTarget.height, get2DSampleMappingSettings(), _sampleData);
+    const SampleData& setup = _sampleData;
 
     hvvr_gpu_view_desc d = {};
     d.msaaRate = _raycaster._spec.msaaRate;
@@ -220,7 +259,13 @@ void Camera::rebuildFoveatedView() {
 void Camera::foveatedFrame(hvvr_gpu_foveated_frame& out) {
     // sample bounds of the camera's own samples (camera.cpp:36-42)
     vector2 lo(1.0f, 1.0f), hi(0.0f, 0.0f);
+    if (_sampleDataGiven) {
+        lo = _sampleData.boundsLower;
+        hi = _sampleData.boundsUpper;
+    }
     for (const Sample& s : _samples) {
+        if (_sampleDataGiven)
+            break;
         lo = vector2(std::min(lo.x, s.position.x), std::min(lo.y, s.position.y));
         hi = vector2(std::max(hi.x, s.position.x), std::max(hi.y, s.position.y));
     }

@@ -6,7 +6,8 @@
 //   * clustering into blocks of 8192 / tiles of 128 (sample_clustering.h:17-49);
 //   * padded 2-D tile/block bounds (sample_hierarchy.cpp:154-190), 3-D camera-space
 //     frusta (get3DFrustumFrom2D, :46-123) and per-sample DirectionalBeams
-//     (generateFrom2D, :129-152);
+//     (generateFrom2D, :129-152), through the viewport's perspective transform or a
+//     spherical section (sphericalUVToDirection, :27-35);
 //   * the image-location -> sample remap (camera.cpp:57-63).
 
 #include "host_internal.h"
@@ -111,13 +112,36 @@ struct Bounds2D {
     float yMax() const { return -d[3]; }
 };
 
-// sample_hierarchy.cpp:46-123, perspective mapping
-SimpleRayFrustum frustumFrom2D(const Bounds2D& b, const matrix3x3& s2c, ThinLens lens) {
+// A sample-space position as a direction on a spherical section: yaw and pitch linear in
+// the position, 0 at the centre (sample_hierarchy.cpp:27-35).
+vector3 sphericalUVToDirection(vector2 uv, float fovXDegrees, float fovYDegrees) {
+    const float radiansPerDegree = 3.14159265358979f / 180.0f;
+    float yaw = (uv.x - 0.5f) * (fovXDegrees * radiansPerDegree);
+    float pitch = -(uv.y - 0.5f) * (fovYDegrees * radiansPerDegree);
+    return normalize(vector3(sinf(yaw) * cosf(pitch), sinf(pitch), -cosf(yaw) * cosf(pitch)));
+}
+
+// sample_hierarchy.cpp:46-123
+SimpleRayFrustum frustumFrom2D(const Bounds2D& b, const Sample2Dto3DMappingSettings& m) {
+    const matrix3x3& s2c = m.sampleToCamera;
+    const ThinLens lens = m.thinLens;
     const float farPlane = -100.0f;
     vector3 nearPoints[4] = {vector3(-lens.radius, -lens.radius, 0.0f), vector3(+lens.radius, -lens.radius, 0.0f),
                              vector3(+lens.radius, +lens.radius, 0.0f), vector3(-lens.radius, +lens.radius, 0.0f)};
     vector3 rayDirections[4] = {s2c * vector3(b.xMin(), b.yMax(), 1.0f), s2c * vector3(b.xMax(), b.yMax(), 1.0f),
                                 s2c * vector3(b.xMax(), b.yMin(), 1.0f), s2c * vector3(b.xMin(), b.yMin(), 1.0f)};
+    if (m.type == Sample2Dto3DMappingSettings::MappingType::SphericalSection) {
+        // the corner rays of the padded sample-space rectangle on the sphere, from the
+        // lens square (the reference fits no far plane here)
+        const vector2 uv[4] = {vector2(b.xMin(), b.yMax()), vector2(b.xMax(), b.yMax()), vector2(b.xMax(), b.yMin()),
+                               vector2(b.xMin(), b.yMin())};
+        SimpleRayFrustum f;
+        for (int i = 0; i < 4; i++) {
+            f.origins[i] = nearPoints[i];
+            f.directions[i] = sphericalUVToDirection(uv[i], m.fovXDegrees, m.fovYDegrees);
+        }
+        return f;
+    }
     vector3 focus[4];
     for (int i = 0; i < 4; i++)
         focus[i] = rayDirections[i] * (lens.focalDistance / -rayDirections[i].z);
@@ -148,8 +172,11 @@ SimpleRayFrustum frustumFrom2D(const Bounds2D& b, const matrix3x3& s2c, ThinLens
 } // namespace
 
 void buildSampleSetup(const Sample* raw, uint32_t count, uint32_t split, uint32_t width, uint32_t height,
-                      const matrix3x3& s2c, ThinLens lens, SampleSetup& out) {
+                      const Sample2Dto3DMappingSettings& mapping, SampleSetup& out) {
     out = SampleSetup();
+    out.settings2DTo3D = mapping;
+    const matrix3x3& s2c = mapping.sampleToCamera;
+    const bool spherical = mapping.type == Sample2Dto3DMappingSettings::MappingType::SphericalSection;
     out.splitColorSamples = split;
     out.width = width;
     out.height = height;
@@ -194,16 +221,23 @@ void buildSampleSetup(const Sample* raw, uint32_t count, uint32_t split, uint32_
                 uint32_t slot = b * BLOCK_SIZE + t * TILE_SIZE + i;
                 Unpacked u = unpackSample(s);
                 DirectionalBeam& beam = out.beams[slot];
-                beam.centerRay = s2c * vector3(u.center, 1.0f);
-                beam.du = s2c * vector3(u.majorAxis, 0.0f);
-                beam.dv = s2c * vector3(u.minorAxis, 0.0f);
+                if (spherical) {
+                    const float fx = mapping.fovXDegrees, fy = mapping.fovYDegrees;
+                    beam.centerRay = sphericalUVToDirection(u.center, fx, fy);
+                    beam.du = sphericalUVToDirection(u.center + u.majorAxis, fx, fy) - beam.centerRay;
+                    beam.dv = sphericalUVToDirection(u.center + u.minorAxis, fx, fy) - beam.centerRay;
+                } else {
+                    beam.centerRay = s2c * vector3(u.center, 1.0f);
+                    beam.du = s2c * vector3(u.majorAxis, 0.0f);
+                    beam.dv = s2c * vector3(u.minorAxis, 0.0f);
+                }
                 sampleIndex = std::min(sampleIndex + 1, maxIndex);
             }
             // the infinite cull rect leaves the bounds unchanged
             block.merge(tile);
-            out.tileFrusta[size_t(b) * TILES_PER_BLOCK + t] = frustumFrom2D(tile, s2c, lens);
+            out.tileFrusta[size_t(b) * TILES_PER_BLOCK + t] = frustumFrom2D(tile, mapping);
         }
-        out.blockFrusta[b] = frustumFrom2D(block, s2c, lens);
+        out.blockFrusta[b] = frustumFrom2D(block, mapping);
     }
 
     out.remap.assign(size_t(width) * height * split, -1);
@@ -213,6 +247,11 @@ void buildSampleSetup(const Sample* raw, uint32_t count, uint32_t split, uint32_
         if (idx < out.remap.size())
             out.remap[idx] = int32_t(i);
     }
+}
+
+SampleData::SampleData(const Sample* rawSamples, uint32_t rawSampleCount, uint32_t splitColorSamples,
+                       const Sample2Dto3DMappingSettings& settings2DTo3D, uint32_t rtWidth, uint32_t rtHeight) {
+    buildSampleSetup(rawSamples, rawSampleCount, splitColorSamples, rtWidth, rtHeight, settings2DTo3D, *this);
 }
 
 } // namespace hvvr

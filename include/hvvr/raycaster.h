@@ -87,6 +87,51 @@ struct RayCasterSpecification {
     int device = 0;
 };
 
+// How sample-space positions in [0,1]^2 become camera-space rays
+// (sample_hierarchy.h:73-95): the perspective transform of the camera's viewport, or a
+// spherical section of fovXDegrees x fovYDegrees (yaw and pitch linear in the sample
+// position, sample_hierarchy.cpp:27-35).
+struct Sample2Dto3DMappingSettings {
+    enum class MappingType { Perspective, SphericalSection };
+    MappingType type = MappingType::Perspective;
+    matrix3x3 sampleToCamera = matrix3x3::identity();
+    ThinLens thinLens = {0.0f, 1.0f};
+    float fovXDegrees = 0.0f;
+    float fovYDegrees = 0.0f;
+    Sample2Dto3DMappingSettings() {}
+    Sample2Dto3DMappingSettings(const matrix3x3& s2c, ThinLens lens) : sampleToCamera(s2c), thinLens(lens) {}
+    static Sample2Dto3DMappingSettings sphericalSection(const matrix3x3& s2c, ThinLens lens, float fovXDegrees,
+                                                       float fovYDegrees) {
+        Sample2Dto3DMappingSettings r(s2c, lens);
+        r.type = MappingType::SphericalSection;
+        r.fovXDegrees = fovXDegrees;
+        r.fovYDegrees = fovYDegrees;
+        return r;
+    }
+};
+
+// A camera's sample hierarchy (camera.h:28-47): samples clustered into blocks of 8192 and
+// tiles of 128 (padded to whole blocks), camera-space beams and tile / block frusta, and
+// the image-location -> sample map. Built from raw samples by the constructor, or taken
+// from another camera (Camera::getSampleData / setSampleData).
+struct SampleData {
+    Sample2Dto3DMappingSettings settings2DTo3D;
+    uint32_t validSampleCount = 0;
+    uint32_t sampleCount = 0; // padded to a multiple of 8192
+    uint32_t blockCount = 0;
+    uint32_t splitColorSamples = 1;
+    uint32_t width = 0, height = 0;             // the render target the remap addresses
+    std::vector<DirectionalBeam> beams;         // [sampleCount]
+    std::vector<SimpleRayFrustum> tileFrusta;   // [blockCount * 64]
+    std::vector<SimpleRayFrustum> blockFrusta;  // [blockCount]
+    std::vector<int32_t> remap;                 // imageLocationToSampleIndex: [width * height * split], -1 = none
+    vector2 boundsLower, boundsUpper;           // sampleBounds
+    SampleData() = default;
+    // camera.cpp:21-64
+    SampleData(const Sample* rawSamples, uint32_t rawSampleCount, uint32_t splitColorSamples,
+               const Sample2Dto3DMappingSettings& settings2DTo3D, uint32_t rtWidth, uint32_t rtHeight);
+};
+
 // SAH BVH4 topology for a mesh, reordering triangles and vertices (shared/bvh.h:15).
 bool GenerateTopology(MeshData& mesh);
 // Grid samples of a width x height render target (samples.cpp:21-42).
@@ -140,6 +185,15 @@ public:
     void setFocalDepth(float depth);
     void setRenderTarget(const ImageResourceDescriptor& target);
     void setSamples(const Sample* samples, uint32_t count, uint32_t splitColorSamples);
+    // Nonzero angles: rays from a spherical section of fovXDegrees x fovYDegrees instead of
+    // the viewport's perspective transform (camera.cpp:206-224); applies from the next
+    // setSamples (or render, if samples are set).
+    void setSphericalWarpSettings(float fovXDegrees, float fovYDegrees);
+    // The sample hierarchy in use (camera.cpp:135-160). setSampleData replaces it as a
+    // whole (its remap addresses the render target's size); setSamples rebuilds it.
+    void setSampleData(const SampleData& sampleData);
+    const SampleData& getSampleData() const { return _sampleData; }
+    Sample2Dto3DMappingSettings get2DSampleMappingSettings() const;
     uint32_t getSampleCount() const { return _validSampleCount; }
     matrix3x3 getSampleToCamera() const;
     void setCameraToWorld(const transform& cameraToWorld) { _cameraToWorld = cameraToWorld; }
@@ -163,6 +217,9 @@ protected:
     ImageResourceDescriptor _renderTarget;
     transform _cameraToWorld = transform::identity();
     std::vector<Sample> _samples;
+    SampleData _sampleData;
+    bool _sampleDataGiven = false; // setSampleData: render from _sampleData as given
+    float _fovXDegrees = 0.0f, _fovYDegrees = 0.0f;
     uint32_t _split = 1;
     uint32_t _validSampleCount = 0;
     bool _viewDirty = true;

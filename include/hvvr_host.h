@@ -66,6 +66,13 @@ int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint3
 int hvvr_host_view_create_from_samples(const void* samples, uint32_t count, uint32_t split, uint32_t width,
                                        uint32_t height, const float viewportLowerUpper[4], float lensRadius,
                                        float focalDistance, hvvr_host_view** out);
+/* As hvvr_host_view_create_from_samples, with rays from a spherical section of
+ * fovXDegrees x fovYDegrees when fovXDegrees > 0 (Camera::setSphericalWarpSettings,
+ * camera.cpp:206-224; sample_hierarchy.cpp:27-35, 68-80, 144-150); the viewport still sets
+ * the sample-to-camera matrix the view reports. */
+int hvvr_host_view_create_spherical(const void* samples, uint32_t count, uint32_t split, uint32_t width,
+                                    uint32_t height, const float viewportLowerUpper[4], float fovXDegrees,
+                                    float fovYDegrees, float lensRadius, float focalDistance, hvvr_host_view** out);
 void hvvr_host_view_destroy(hvvr_host_view* view);
 /* counts[0..6] = validSampleCount, sampleCount, blockCount, width, height, split, tileCount */
 int hvvr_host_view_counts(const hvvr_host_view* view, uint64_t counts[7]);

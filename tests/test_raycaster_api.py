@@ -98,14 +98,14 @@ def _write_scene(path, models, lights, moved=None):
             f.write(struct.pack("<I8f", 1, *t, *r, s))
 
 
-def _oracle_image(H, O, models, lights, transforms):
+def _oracle_image(H, O, models, lights, transforms, spherical=None):
     meshes, mat_offset, all_mats = [], 0, []
     for ((v, tris, mats), materials, _), (t, r, s) in zip(models, transforms):
         meshes.append((v, tris, mats + mat_offset, H.transform_matrix(t, r, s)))
         all_mats.append(materials)
         mat_offset += len(materials)
     scene = H.HostScene.from_meshes(meshes, np.concatenate(all_mats), lights)
-    view = H.HostView.from_samples(H.grid_samples(W, H_), W, H_, VIEWPORT)
+    view = H.HostView.from_samples(H.grid_samples(W, H_), W, H_, VIEWPORT, spherical=spherical)
     m = H.transform_matrix(*CAMERA)
     return O.render_view(scene, view, MSAA, camera_to_world=m)["image"]
 
@@ -175,3 +175,27 @@ def test_cpp_api_foveated_matches_c_abi(H, ctx, tmp_path):
     want = gv.read_image()
     assert (want != 0).mean() > 0.9
     assert np.array_equal(img, want)
+
+
+@pytest.mark.gpu
+def test_cpp_api_spherical_warp_and_shared_sample_data(H, O, tmp_path):
+    # Camera::setSphericalWarpSettings(100, 75) before setSamples: the image is the
+    # oracle's frame of the same spherical-section view; and a second camera given the
+    # first one's sample hierarchy (getSampleData / setSampleData) renders the same image
+    models, lights = _scene(H)
+    path, out = tmp_path / "scene.bin", tmp_path / "out.rgba"
+    _write_scene(path, models, lights)
+    r = subprocess.run([BIN, str(path), str(out), "1", "spherical=100,75"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(out, np.uint32).reshape(H_, W)
+    base = [m[2] for m in models]
+    want = _oracle_image(H, O, models, lights, base, spherical=(100.0, 75.0)).reshape(H_, W)
+    assert not np.array_equal(want, _oracle_image(H, O, models, lights, base).reshape(H_, W))
+    assert rgba_delta(img, want).max() <= 1
+    r = subprocess.run([BIN, str(path), str(out), "1", "share-samples"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    imgs = np.fromfile(out, np.uint32).reshape(2, H_, W)
+    assert (imgs[0] != 0xFF000000).mean() > 0.3
+    assert np.array_equal(imgs[0], imgs[1])
