@@ -152,6 +152,19 @@ class HostScene:
         lib.hvvr_host_scene_destroy(h)
 
     @classmethod
+    def from_bin(cls, path):
+        """A scene file in the reference's binary format (include/hvvr/model_import.h)."""
+        h = c_void_p()
+        _check_host(load_host().hvvr_host_scene_load_bin(os.fsencode(path), byref(h)))
+        return cls(_handle=h)
+
+    @staticmethod
+    def save_bin(path, kind=SCENE_HEIGHTFIELD_BOXES, seed=0, mesh_count=0):
+        """Writes the seeded scene of HostScene(kind, seed, mesh_count) as a .bin file."""
+        _check_host(load_host().hvvr_host_scene_save_bin(c_int(kind), c_uint64(seed), c_uint32(mesh_count),
+                                                         os.fsencode(path)))
+
+    @classmethod
     def from_meshes(cls, meshes, materials, lights=None):
         """meshes: list of (verts[vertex dtype], tri_indices[n,3] local, tri_material[n], transform16 or None)."""
         lib = load_host()

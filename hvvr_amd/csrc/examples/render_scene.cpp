@@ -3,7 +3,15 @@
 // modelviewer/modelviewer.cpp: create models, lights and a camera, set samples and a
 // render target, call render() every frame).
 //
-//   render_scene <scene.bin> <out.rgba> [frames] [foveated | spherical=FX,FY | share-samples]
+//   render_scene <scene> <out.rgba> [frames] [foveated | spherical=FX,FY | share-samples]
+//                [size=WxH] [msaa=N] [camera=TX,TY,TZ,QX,QY,QZ,QW]
+//
+// <scene> is either a file in the reference's binary scene format (model_import.h; loaded
+// with model_import::load and createObjects, as modelviewer.cpp:71-77 does), rendered at
+// size (default 640x480) and msaa (default 4) from `camera` (translation and rotation
+// quaternion, default identity) through the viewport (-W/H, -1)..(W/H, 1) that
+// modelviewer.cpp:160 sets, or an "HVSC" test scene file (below), which carries its own
+// camera, size and MSAA rate.
 //
 // "foveated" renders in GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT mode with the feb2017
 // sample pattern (raycaster_spec.h:38-47), gaze straight ahead. "spherical=FX,FY" sets
@@ -30,8 +38,10 @@
 // the model-0 move when moveModel is 1. Prints one JSON line with host-inclusive frame
 // times (render() through the image copy into the CPU render target).
 
+#include "hvvr/model_import.h"
 #include "hvvr/raycaster.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -66,13 +76,37 @@ struct Reader {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s scene.bin out.rgba [frames] [foveated | spherical=FX,FY | share-samples]\n",
+        fprintf(stderr,
+                "usage: %s scene out.rgba [frames] [foveated | spherical=FX,FY | share-samples] [size=WxH] "
+                "[msaa=N] [camera=TX,TY,TZ,QX,QY,QZ,QW]\n",
                 argv[0]);
         return 2;
     }
-    int frames = argc > 3 ? atoi(argv[3]) : 1;
-    if (frames < 1)
-        frames = 1;
+    int frames = 1;
+    std::string mode;
+    uint32_t optW = 640, optH = 480, optMsaa = 4;
+    float cam[7] = {0, 0, 0, 0, 0, 0, 1};
+    for (int i = 3; i < argc; i++) {
+        const std::string a = argv[i];
+        if (!a.empty() && a.find_first_not_of("0123456789") == std::string::npos) {
+            frames = std::max(1, atoi(a.c_str()));
+        } else if (a.rfind("size=", 0) == 0) {
+            if (sscanf(a.c_str() + 5, "%ux%u", &optW, &optH) != 2 || !optW || !optH) {
+                fprintf(stderr, "size=WxH: two positive integers\n");
+                return 2;
+            }
+        } else if (a.rfind("msaa=", 0) == 0) {
+            optMsaa = uint32_t(atoi(a.c_str() + 5));
+        } else if (a.rfind("camera=", 0) == 0) {
+            if (sscanf(a.c_str() + 7, "%f,%f,%f,%f,%f,%f,%f", cam, cam + 1, cam + 2, cam + 3, cam + 4, cam + 5,
+                       cam + 6) != 7) {
+                fprintf(stderr, "camera=TX,TY,TZ,QX,QY,QZ,QW: seven numbers\n");
+                return 2;
+            }
+        } else {
+            mode = a;
+        }
+    }
     try {
         FILE* f = fopen(argv[1], "rb");
         if (!f)
@@ -80,14 +114,15 @@ int main(int argc, char** argv) {
         Reader r{f};
         char magic[4];
         r.read(magic, 4);
-        if (memcmp(magic, "HVSC", 4) != 0 || r.get<uint32_t>() != 1)
+        const bool testScene = memcmp(magic, "HVSC", 4) == 0;
+        if (testScene && r.get<uint32_t>() != 1)
             throw std::runtime_error("not a version-1 HVSC scene file");
 
         struct ModelIn {
             MeshData mesh;
             transform xf;
         };
-        std::vector<ModelIn> models(r.get<uint32_t>());
+        std::vector<ModelIn> models(testScene ? r.get<uint32_t>() : 0u);
         for (ModelIn& m : models) {
             uint32_t nv = r.get<uint32_t>(), nt = r.get<uint32_t>(), nm = r.get<uint32_t>();
             m.xf = r.getTransform();
@@ -105,19 +140,32 @@ int main(int argc, char** argv) {
             m.mesh.materials.resize(nm);
             r.read(m.mesh.materials.data(), nm * sizeof(SimpleMaterial));
         }
-        std::vector<LightUnion> lights(r.get<uint32_t>());
-        r.read(lights.data(), lights.size() * sizeof(LightUnion));
-        uint32_t width = r.get<uint32_t>(), height = r.get<uint32_t>(), msaa = r.get<uint32_t>();
-        float vp[4];
-        r.read(vp, sizeof(vp));
-        transform cameraToWorld = r.getTransform();
-        uint32_t moveModel = r.get<uint32_t>();
+        std::vector<LightUnion> lights;
+        uint32_t width = optW, height = optH, msaa = optMsaa;
+        float vp[4] = {-float(optW) / float(optH), -1.0f, float(optW) / float(optH), 1.0f};
+        transform cameraToWorld(vector3(cam[0], cam[1], cam[2]), quaternion(cam[3], cam[4], cam[5], cam[6]), 1.0f);
+        uint32_t moveModel = 0;
         transform moved = transform::identity();
-        if (moveModel)
-            moved = r.getTransform();
+        if (testScene) {
+            lights.resize(r.get<uint32_t>());
+            r.read(lights.data(), lights.size() * sizeof(LightUnion));
+            width = r.get<uint32_t>();
+            height = r.get<uint32_t>();
+            msaa = r.get<uint32_t>();
+            r.read(vp, sizeof(vp));
+            cameraToWorld = r.getTransform();
+            moveModel = r.get<uint32_t>();
+            if (moveModel)
+                moved = r.getTransform();
+        }
         fclose(f);
+        model_import::Model binModel;
+        if (!testScene) {
+            std::string err;
+            if (!model_import::load(argv[1], binModel, &err))
+                throw std::runtime_error(std::string(argv[1]) + ": " + err);
+        }
 
-        const std::string mode = argc > 4 ? argv[4] : "";
         float fovX = 0.0f, fovY = 0.0f;
         if (mode.rfind("spherical=", 0) == 0 && sscanf(mode.c_str() + 10, "%f,%f", &fovX, &fovY) != 2)
             throw std::runtime_error("spherical=FX,FY: two angles in degrees");
@@ -140,6 +188,8 @@ int main(int argc, char** argv) {
         }
         for (const LightUnion& l : lights)
             raycaster.createLight(l);
+        if (!testScene && !model_import::createObjects(raycaster, binModel))
+            throw std::runtime_error("createObjects failed");
 
         Camera* camera = raycaster.createCamera(FloatRect(vector2(vp[0], vp[1]), vector2(vp[2], vp[3])), 0.0f);
         std::vector<uint32_t> image(size_t(width) * height, 0);

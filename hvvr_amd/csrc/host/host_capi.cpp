@@ -1,6 +1,7 @@
 // C interface over the host-side building blocks (include/hvvr_host.h).
 
 #include "../../../include/hvvr_host.h"
+#include "hvvr/model_import.h"
 #include "host_internal.h"
 
 #include <string.h>
@@ -79,6 +80,49 @@ int hvvr_host_scene_create(int kind, uint64_t seed, uint32_t meshCount, hvvr_hos
         return fail("unknown scene kind");
     }
     return finishScene(gen.meshes, gen.modelToWorld, gen.lights, out);
+}
+
+int hvvr_host_scene_save_bin(int kind, uint64_t seed, uint32_t meshCount, const char* path) {
+    if (!path)
+        return fail("null path");
+    SceneGenResult gen;
+    if (kind == HVVR_SCENE_HEIGHTFIELD_BOXES)
+        generateHeightfieldBoxesScene(seed, gen);
+    else if (kind == HVVR_SCENE_ROOM && meshCount > 0)
+        generateRoomScene(seed, meshCount, gen);
+    else
+        return fail("unknown scene kind or no meshes");
+    model_import::Model model;
+    model.lights = gen.lights;
+    std::string err;
+    for (size_t i = 0; i < gen.meshes.size(); i++) {
+        if (!GenerateTopology(gen.meshes[i], &err))
+            return fail(err);
+        model.meshes.push_back({gen.transforms[i], std::move(gen.meshes[i])});
+    }
+    if (!model_import::saveBin(path, model, &err))
+        return fail(err);
+    return 0;
+}
+
+int hvvr_host_scene_load_bin(const char* path, hvvr_host_scene** out) {
+    if (!path || !out)
+        return fail("null argument");
+    model_import::Model model;
+    std::string err;
+    if (!model_import::loadBin(path, model, &err))
+        return fail(std::string(path) + ": " + err);
+    // the stored meshes carry their topology: flattened as they are (createModel +
+    // setTransform of model_import.cpp:47-59)
+    std::vector<SceneModel> models;
+    for (const model_import::Mesh& m : model.meshes)
+        models.push_back({&m.data, matrix4x4(m.transform)});
+    hvvr_host_scene* s = new hvvr_host_scene();
+    buildScene(models, s->flat);
+    s->flat.lights = model.lights;
+    s->modelCount = uint32_t(models.size());
+    *out = s;
+    return 0;
 }
 
 int hvvr_host_scene_create_from_meshes(uint32_t meshCount, const uint32_t* vertexStart, const void* verts,

@@ -127,6 +127,7 @@ struct Pcg32 {
 struct SceneGenResult {
     std::vector<MeshData> meshes;
     std::vector<matrix4x4> modelToWorld;
+    std::vector<transform> transforms; // the same placements as transforms (what .bin files store)
     std::vector<LightUnion> lights;
 };
 // 64x64-cell heightfield (8192 triangles) + 256 boxes (3072 triangles), per-triangle GGX

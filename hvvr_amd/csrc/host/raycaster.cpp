@@ -343,6 +343,43 @@ void Raycaster::destroyCamera(Camera* camera) {
     eraseOwned(_cameras, camera, "Raycaster::destroyCamera");
 }
 
+size_t getTextureSize(uint32_t strideElements, uint32_t height, TextureFormat format) {
+    const size_t n = size_t(strideElements) * height;
+    switch (format) {
+        case TextureFormat::r8g8b8a8_unorm_srgb:
+        case TextureFormat::r8g8b8a8_unorm:
+        case TextureFormat::r11g11b10_float:
+        case TextureFormat::r32_float:
+            return n * 4;
+        case TextureFormat::r16g16b16a16_unorm:
+        case TextureFormat::r16g16b16a16_float:
+            return n * 8;
+        case TextureFormat::r32g32b32a32_float:
+            return n * 16;
+        default:
+            return 0;
+    }
+}
+
+Texture::Texture(const TextureData& data, uint32_t id) : _textureID(id), _data(data) {
+    const size_t bytes = getTextureSize(data.strideElements, data.height, data.format);
+    if (data.format == TextureFormat::none || data.format >= TextureFormat::count || data.strideElements < data.width)
+        throw RaycasterError("Raycaster::createTexture: invalid texture description");
+    if (bytes && !data.data)
+        throw RaycasterError("Raycaster::createTexture: null texels");
+    _texels.assign(data.data, data.data + bytes);
+    _data.data = _texels.data();
+}
+
+Texture* Raycaster::createTexture(const TextureData& textureData) {
+    _textures.emplace_back(new Texture(textureData, _nextTextureID++));
+    return _textures.back().get();
+}
+
+void Raycaster::destroyTexture(Texture* texture) {
+    eraseOwned(_textures, texture, "Raycaster::destroyTexture");
+}
+
 Light* Raycaster::createLight(const LightUnion& lightUnion) {
     _lights.emplace_back(new Light(lightUnion));
     _lightsDirty = true;

@@ -188,6 +188,7 @@ void generateHeightfieldBoxesScene(uint64_t seed, SceneGenResult& out) {
         }
         out.meshes.push_back(std::move(m));
         out.modelToWorld.push_back(matrix4x4::identity());
+        out.transforms.push_back(transform::identity());
     }
     // 256 axis-aligned boxes, one model each, 12 triangles with per-triangle materials
     for (uint32_t b = 0; b < 256; b++) {
@@ -207,6 +208,7 @@ void generateHeightfieldBoxesScene(uint64_t seed, SceneGenResult& out) {
         }
         out.meshes.push_back(std::move(m));
         out.modelToWorld.push_back(matrix4x4::identity());
+        out.transforms.push_back(transform::identity());
     }
     out.lights.push_back(directionalLight(vector3(0.4f, 0.8f, 0.45f), vector3(2.5f, 2.4f, 2.2f)));
 }
@@ -227,7 +229,8 @@ void generateRoomScene(uint64_t seed, uint32_t meshCount, SceneGenResult& out) {
         float scale = float(exp(0.7 * rng.normal()));
         quaternion rot = randomRotation(rng);
         vector3 pos(rng.uniformf(0.0f, 30.0f), rng.uniformf(0.0f, 15.0f), rng.uniformf(0.0f, 30.0f));
-        out.modelToWorld.push_back(matrix4x4(transform(pos, rot, scale)));
+        out.transforms.push_back(transform(pos, rot, scale));
+        out.modelToWorld.push_back(matrix4x4(out.transforms.back()));
         out.meshes.push_back(std::move(m));
     }
     out.lights.push_back(directionalLight(vector3(-0.3f, 0.9f, 0.3f), vector3(1.2f, 1.2f, 1.1f)));

@@ -137,7 +137,44 @@ bool GenerateTopology(MeshData& mesh);
 // Grid samples of a width x height render target (samples.cpp:21-42).
 std::vector<Sample> getGridSamples(size_t width, size_t height);
 
+// texture.h:16-80: texel formats, a texture's description and its byte size
+enum class TextureFormat : uint32_t {
+    none = 0,
+    r8g8b8a8_unorm_srgb,
+    r8g8b8a8_unorm,
+    r16g16b16a16_unorm,
+    r32g32b32a32_float,
+    r16g16b16a16_float,
+    r11g11b10_float,
+    r32_float,
+    count
+};
+struct TextureData {
+    const uint8_t* data;
+    TextureFormat format;
+    uint32_t width;
+    uint32_t height;
+    uint32_t strideElements;
+};
+// bytes of strideElements x height texels; 0 for `none` or an unknown format
+size_t getTextureSize(uint32_t strideElements, uint32_t height, TextureFormat format);
+
 class Raycaster;
+
+// A texture the raycaster holds (texture.h:36-54); getID() is the index materials use in
+// their texture fields. Shading in this build is untextured: materials shade with their
+// constant colours whatever texture they name.
+class Texture {
+public:
+    Texture(const TextureData& data, uint32_t id);
+    uint32_t getID() const { return _textureID; }
+    const TextureData& getData() const { return _data; }
+
+protected:
+    uint32_t _textureID;
+    TextureData _data;         // data points into _texels
+    std::vector<uint8_t> _texels;
+};
 
 class Light {
 public:
@@ -243,6 +280,9 @@ public:
 
     Camera* createCamera(const FloatRect& viewport, float apertureRadius);
     void destroyCamera(Camera* camera);
+    // raycaster.h:51-52: the texels are copied
+    Texture* createTexture(const TextureData& textureData);
+    void destroyTexture(Texture* texture);
     Light* createLight(const LightUnion& lightUnion);
     void destroyLight(Light* light);
     Model* createModel(MeshData&& meshData);
@@ -263,6 +303,8 @@ protected:
 
     RayCasterSpecification _spec;
     std::vector<std::unique_ptr<Camera>> _cameras;
+    std::vector<std::unique_ptr<Texture>> _textures;
+    uint32_t _nextTextureID = 0;
     std::vector<std::unique_ptr<Light>> _lights;
     std::vector<std::unique_ptr<Model>> _models;
     bool _sceneDirty = false;

@@ -45,6 +45,13 @@ int hvvr_host_scene_create_from_meshes(uint32_t meshCount, const uint32_t* verte
                                        const uint32_t* triMaterial, const float* transforms16,
                                        uint32_t materialCount, const void* materials, uint32_t lightCount,
                                        const void* lights, hvvr_host_scene** out);
+/* Scene files in the reference's binary format (include/hvvr/model_import.h,
+ * samples_shared/model_import_bin.cpp): save writes the seeded scene of
+ * hvvr_host_scene_create (its meshes after GenerateTopology, their transforms, the
+ * lights); load flattens a file's meshes with their stored topology, transforms and lights
+ * (textures are checked and skipped: shading here is untextured). */
+int hvvr_host_scene_save_bin(int kind, uint64_t seed, uint32_t meshCount, const char* path);
+int hvvr_host_scene_load_bin(const char* path, hvvr_host_scene** out);
 void hvvr_host_scene_destroy(hvvr_host_scene* scene);
 /* counts[0..6] = verts, tris, nodes, materials, transforms, lights, models */
 int hvvr_host_scene_counts(const hvvr_host_scene* scene, uint64_t counts[7]);

@@ -118,6 +118,9 @@ def test_scene_file_is_rejected_when_malformed(tmp_path):
     bad = tmp_path / "bad.bin"
     bad.write_bytes(b"NOPE")
     r = subprocess.run([BIN, str(bad), str(tmp_path / "o.rgba")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "truncated header" in r.stderr
+    bad.write_bytes(b"HVSC" + struct.pack("<I", 2))
+    r = subprocess.run([BIN, str(bad), str(tmp_path / "o.rgba")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "HVSC" in r.stderr
 
 
