@@ -4,8 +4,10 @@
 // render target, call render() every frame).
 //
 //   render_scene <scene> <out.rgba> [frames] [foveated | spherical=FX,FY | share-samples]
-//                [size=WxH] [msaa=N] [camera=TX,TY,TZ,QX,QY,QZ,QW]
+//                [size=WxH] [msaa=N] [camera=TX,TY,TZ,QX,QY,QZ,QW] [dump=PREFIX[,text]]
 //
+// dump=PREFIX writes Raycaster::dumpSceneAndRays(camera, PREFIX) after the first frame
+// (binary OFF / BFF / RFF files, or the text forms with ",text").
 // <scene> is either a file in the reference's binary scene format (model_import.h; loaded
 // with model_import::load and createObjects, as modelviewer.cpp:71-77 does), rendered at
 // size (default 640x480) and msaa (default 4) from `camera` (translation and rotation
@@ -86,6 +88,8 @@ int main(int argc, char** argv) {
     std::string mode;
     uint32_t optW = 640, optH = 480, optMsaa = 4;
     float cam[7] = {0, 0, 0, 0, 0, 0, 1};
+    std::string dumpPrefix;
+    bool dumpBinary = true;
     for (int i = 3; i < argc; i++) {
         const std::string a = argv[i];
         if (!a.empty() && a.find_first_not_of("0123456789") == std::string::npos) {
@@ -94,6 +98,13 @@ int main(int argc, char** argv) {
             if (sscanf(a.c_str() + 5, "%ux%u", &optW, &optH) != 2 || !optW || !optH) {
                 fprintf(stderr, "size=WxH: two positive integers\n");
                 return 2;
+            }
+        } else if (a.rfind("dump=", 0) == 0) {
+            dumpPrefix = a.substr(5);
+            const size_t comma = dumpPrefix.rfind(",text");
+            if (comma != std::string::npos && comma + 5 == dumpPrefix.size()) {
+                dumpPrefix.resize(comma);
+                dumpBinary = false;
             }
         } else if (a.rfind("msaa=", 0) == 0) {
             optMsaa = uint32_t(atoi(a.c_str() + 5));
@@ -225,6 +236,8 @@ int main(int argc, char** argv) {
             raycaster.render(0.0);
             auto t1 = std::chrono::steady_clock::now();
             ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+            if (i == 0 && !dumpPrefix.empty())
+                raycaster.dumpSceneAndRays(camera, dumpPrefix, dumpBinary);
         }
         fwrite(image.data(), 4, image.size(), out);
         if (!image2.empty())

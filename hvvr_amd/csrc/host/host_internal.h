@@ -55,6 +55,13 @@ void buildSampleSetup(const Sample* samples, uint32_t count, uint32_t split, uin
                       const Sample2Dto3DMappingSettings& settings, SampleSetup& out);
 
 // ---------------------------------------------------------------------------------
+// scene and ray dumps (raycaster/render.cpp:28-201); binary forms append 'b' to the name
+// ---------------------------------------------------------------------------------
+void dumpRaysToRFF(const std::string& filename, const std::vector<SimpleRay>& rays, bool binary);
+void dumpSceneToOFF(const std::string& filename, const std::vector<const Model*>& models, bool binary);
+void dumpBVH(const std::string& filename, const std::vector<const Model*>& models, bool binary);
+
+// ---------------------------------------------------------------------------------
 // polar foveated path (raycaster/foveated.cpp, samples.cpp:44-142, frusta.cu:26-121)
 // ---------------------------------------------------------------------------------
 // Eye-space polar samples, ring-major: sample r * samplesPerRing + i is azimuth column i

@@ -468,6 +468,24 @@ void Raycaster::updateScene() {
     check(hvvr_gpu_scene_update(_scene), "Raycaster: scene update failed");
 }
 
+void Raycaster::dumpSceneAndRays(Camera* camera, const std::string& prefix, bool binary) {
+    if (!camera || !camera->_view)
+        throw RaycasterError("Raycaster::dumpSceneAndRays: the camera has no view yet (render it first)");
+    if (_spec.mode == RayCasterGPUMode::GPU_FOVEATED_POLAR_SPACE_CUDA_RECONSTRUCT)
+        throw RaycasterError("Raycaster::dumpSceneAndRays: the foveated mode has no image-order rays");
+    std::vector<const Model*> models;
+    for (const auto& m : _models)
+        models.push_back(m.get());
+    dumpBVH(prefix + "bvh_dump.bff", models, binary);
+    dumpSceneToOFF(prefix + "scene_dump.off", models, binary);
+    const uint64_t n = uint64_t(camera->_validSampleCount) * _spec.msaaRate;
+    std::vector<SimpleRay> rays(n);
+    const matrix4x4 m = camera->getCameraToWorld();
+    check(hvvr_gpu_view_dump_rays(camera->_view, (const float*)&m, 1, rays.data(), n),
+          "Raycaster::dumpSceneAndRays: ray dump failed");
+    dumpRaysToRFF(prefix + "ray_dump.rff", rays, binary);
+}
+
 void Raycaster::render(double elapsedTime) {
     (void)elapsedTime;
     updateScene();

@@ -290,6 +290,12 @@ public:
     void destroyModel(Model* model);
     void setSceneDirty();
     void render(double elapsedTime);
+    // dumpSceneAndRays (render.cpp:175-201, compiled in there by DUMP_SCENE_AND_RAYS): the
+    // models' BVH topology (<prefix>bvh_dump.bff), world-space triangles
+    // (<prefix>scene_dump.off) and the camera's subsample rays in image order
+    // (<prefix>ray_dump.rff) for camera-to-world as last set; binary forms add a 'b' to each
+    // name. The camera must have been rendered (its sample setup is on the GPU).
+    void dumpSceneAndRays(Camera* camera, const std::string& prefix = "", bool binary = true);
     void reinit(RayCasterGPUMode mode);
 
     // GPU handles, for callers that mix this API with the C ABI (hvvr_gpu.h)

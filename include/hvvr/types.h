@@ -382,6 +382,13 @@ struct DirectionalBeam {
 };
 static_assert(sizeof(DirectionalBeam) == 36, "DirectionalBeam layout");
 // SimpleRayFrustum (graphics_types.h:182-185)
+// SimpleRay (graphics_types.h:175-178): one ray of a ray dump
+struct SimpleRay {
+    vector3 origin;
+    vector3 direction;
+};
+static_assert(sizeof(SimpleRay) == 24, "SimpleRay layout");
+
 struct SimpleRayFrustum {
     vector3 origins[4];
     vector3 directions[4];

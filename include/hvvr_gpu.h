@@ -277,6 +277,15 @@ int hvvr_gpu_view_read_gbuffer(hvvr_gpu_view* view, uint32_t* triIndex, uint32_t
 int hvvr_gpu_view_read_hits(hvvr_gpu_view* view, float* hits);
 /* Per-tile triangle lists of the last frame in tile order: counts[tileCount]; lists
  * concatenated into triIndices (capacity `capacity` entries); *total = list entries. */
+/* GPUCamera::dumpRays + DumpRaysKernel (gpu_camera.cu, intersect.cu:407-458): every
+ * subsample's world-space ray for camera-to-world `cameraToWorld` (float[16]), as SimpleRay
+ * {float3 origin, float3 direction} (24 B), sample s's subsample k at s*msaaRate + k. Sample
+ * order, or with scanlineOrder the image order of the remap (locations without a sample
+ * get NaN rays). origin = the camera position (the reference's thin-lens origin is a TODO,
+ * so depth of field is not applied). capacity: rays the buffer holds (validSampleCount
+ * x msaaRate needed). Synchronous. */
+int hvvr_gpu_view_dump_rays(hvvr_gpu_view* view, const float* cameraToWorld, int scanlineOrder, void* rays,
+                            uint64_t capacity);
 int hvvr_gpu_view_read_tile_lists(hvvr_gpu_view* view, uint32_t* counts, uint32_t* triIndices, uint64_t capacity,
                                   uint64_t* total);
 typedef struct {
