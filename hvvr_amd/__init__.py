@@ -308,6 +308,14 @@ def foveated_frame(camera_to_world, eye_dir, sample_to_camera, sample_bounds, hi
                 eye_to_sample_previous=out[41:57].copy())
 
 
+def preset_params(preset, eye=0, width=0, height=0):
+    """(viewport lower x y upper x y, (barrel k1, k2), (lens radius, focal distance)) of a view preset"""
+    p = np.zeros(8, np.float32)
+    _check_host(load_host().hvvr_host_view_preset_params(c_int(preset), c_uint32(eye), c_uint32(width),
+                                                          c_uint32(height), _ptr(p)))
+    return p[:4].copy(), (float(p[4]), float(p[5])), (float(p[6]), float(p[7]))
+
+
 def subsample_offsets(msaa_rate, spin=0.0):
     out = np.zeros((msaa_rate, 2), np.float32)
     _check_host(load_host().hvvr_host_subsample_offsets(c_uint32(msaa_rate), c_float(spin), _ptr(out)))
@@ -629,6 +637,61 @@ class GpuView:
         if self.handle:
             self.lib.hvvr_gpu_view_destroy(self.handle)
             self.handle = c_void_p()
+
+
+class _SetupDesc(ctypes.Structure):
+    _fields_ = [("msaaRate", c_uint32), ("width", c_uint32), ("height", c_uint32), ("split", c_uint32),
+                ("sampleToCamera", c_float * 9), ("lensRadius", c_float), ("focalDistance", c_float),
+                ("fovXDegrees", c_float), ("fovYDegrees", c_float)]
+
+
+class _DeviceSetupShape:
+    """The host-side description of a view whose sample hierarchy the device built: its
+    counts, and its beams, frusta and remap read back (for comparisons)."""
+
+    def __init__(self, gv, sample_to_camera, lens, camera_to_world, bounds):
+        shape = np.zeros(8, np.uint32)
+        _check_gpu(gv.lib.hvvr_gpu_view_get_shape(gv.handle, _ptr(shape)))
+        (_, self.sample_count, self.valid_sample_count, self.tile_count, self.block_count, self.width, self.height,
+         self.split) = (int(x) for x in shape)
+        self.beams = np.zeros(self.sample_count, DTYPES["beam"])
+        self.tile_frusta = np.zeros(self.tile_count, DTYPES["sfrustum"])
+        self.block_frusta = np.zeros(self.block_count, DTYPES["sfrustum"])
+        self.remap = np.zeros(self.width * self.height * self.split, np.int32)
+        _check_gpu(gv.lib.hvvr_gpu_view_read_sample_setup(gv.handle, _ptr(self.beams), _ptr(self.tile_frusta),
+                                                          _ptr(self.block_frusta), _ptr(self.remap)))
+        self.sample_to_camera = np.asarray(sample_to_camera, np.float32)
+        self.lens = np.asarray(lens, np.float32)
+        self.camera_to_world = np.asarray(camera_to_world, np.float32).reshape(16)
+        self.sample_bounds = bounds
+
+
+class GpuDeviceSetupView(GpuView):
+    """A view whose sample hierarchy is built on the device (hvvr_gpu_view_create_grid /
+    _create_from_samples): Camera::setSamples without the host setup."""
+
+    def __init__(self, ctx: GpuContext, width, height, msaa_rate=1, viewport=(-1.0, -1.0, 1.0, 1.0), samples=None,
+                 split=1, barrel=(0.0, 0.0), lens=(0.0, 1.0), spherical=None, camera_to_world=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.msaa_rate = msaa_rate
+        s2c = np.zeros(9, np.float32)
+        _check_host(load_host().hvvr_host_sample_to_camera(_ptr(np.asarray(viewport, np.float32)), _ptr(s2c)))
+        fx, fy = spherical if spherical is not None else (0.0, 0.0)
+        d = _SetupDesc(msaa_rate, width, height, split, (c_float * 9)(*[float(x) for x in s2c]), float(lens[0]),
+                       float(lens[1]), float(fx), float(fy))
+        self.handle = c_void_p()
+        bounds = np.zeros(4, np.float32)
+        if samples is None:
+            _check_gpu(self.lib.hvvr_gpu_view_create_grid(ctx.handle, byref(d), c_float(barrel[0]), c_float(barrel[1]),
+                                                          byref(self.handle), _ptr(bounds)))
+        else:
+            samples = np.ascontiguousarray(samples, DTYPES["sample"])
+            _check_gpu(self.lib.hvvr_gpu_view_create_from_samples(ctx.handle, byref(d), _ptr(samples),
+                                                                  c_uint32(len(samples)), byref(self.handle),
+                                                                  _ptr(bounds)))
+        m = np.eye(4, dtype=np.float32).reshape(16) if camera_to_world is None else camera_to_world
+        self.view = _DeviceSetupShape(self, s2c, lens, m, bounds)
 
 
 class _FoveatedViewShape:

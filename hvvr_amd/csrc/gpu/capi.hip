@@ -387,6 +387,15 @@ int hvvr_gpu_scene_read_vertices(hvvr_gpu_scene* s, void* dst) {
 }
 
 int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvvr_gpu_view** out) {
+    return makeView(ctx, d, false, out);
+}
+
+} // extern "C"
+
+// A view and its buffers. The sample hierarchy (beams, camera-space frusta, remap) is
+// uploaded from d's host arrays, or with `deviceSetup` only allocated (the remap for
+// width x height x split entries) for the device sample setup to fill.
+int hvvr::makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool deviceSetup, hvvr_gpu_view** out) {
     if (!ctx || !d || !out)
         return fail("null argument");
     if (!validRate(d->msaaRate))
@@ -395,7 +404,7 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         return fail("sampleCount must be a positive multiple of 8192");
     if (d->validSampleCount > d->sampleCount)
         return fail("validSampleCount exceeds sampleCount");
-    if (!d->beams || !d->tileFrusta || !d->blockFrusta)
+    if (!deviceSetup && (!d->beams || !d->tileFrusta || !d->blockFrusta))
         return fail("view description has null arrays");
     if (d->split < 1 || d->split > 3 || d->width == 0 || d->height == 0)
         return fail("invalid image size or split");
@@ -403,8 +412,8 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         return fail("lensRadius must be finite and >= 0");
     if (d->lensRadius > 0.0f && (!(d->focalDistance > 0.0f) || !std::isfinite(d->focalDistance)))
         return fail("depth of field needs a finite focalDistance > 0");
-    uint64_t remapCount = d->remap ? uint64_t(d->width) * d->height * d->split : 0;
-    for (uint64_t i = 0; i < remapCount; i++)
+    uint64_t remapCount = d->remap || deviceSetup ? uint64_t(d->width) * d->height * d->split : 0;
+    for (uint64_t i = 0; d->remap && i < remapCount; i++)
         if (d->remap[i] >= int32_t(d->sampleCount))
             return fail("remap entry out of range");
 
@@ -427,10 +436,17 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
         if (e == hipSuccess)
             e = r;
     };
-    step(upload(v->beams, d->beams, v->sampleCount, st));
-    step(upload(v->tileFrustaCam, d->tileFrusta, v->tileCount, st));
-    step(upload(v->blockFrustaCam, d->blockFrusta, v->blockCount, st));
-    step(upload(v->remap, d->remap, remapCount, st));
+    if (deviceSetup) {
+        step(v->beams.resize(v->sampleCount));
+        step(v->tileFrustaCam.resize(v->tileCount));
+        step(v->blockFrustaCam.resize(v->blockCount));
+        step(v->remap.resize(remapCount));
+    } else {
+        step(upload(v->beams, d->beams, v->sampleCount, st));
+        step(upload(v->tileFrustaCam, d->tileFrusta, v->tileCount, st));
+        step(upload(v->blockFrustaCam, d->blockFrusta, v->blockCount, st));
+        step(upload(v->remap, d->remap, remapCount, st));
+    }
     step(v->tileFrustaWorld.resize(v->tileCount));
     step(v->tilePre.resize(v->tileCount));
     step(v->blockPre.resize(v->blockCount));
@@ -472,6 +488,8 @@ int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, hvv
     *out = v;
     return HVVR_GPU_OK;
 }
+
+extern "C" {
 
 int hvvr_gpu_view_destroy(hvvr_gpu_view* v) {
     if (v) {

@@ -208,6 +208,9 @@ struct TileListScratch {
 // scratch fits `words` 32-bit words (at most as many as a view's own scratch holds)
 void tileListGroupsFor(size_t words, uint32_t groups[3]);
 
+// hvvr_gpu_view_create; with `deviceSetup` the sample hierarchy is allocated, not uploaded
+int makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool deviceSetup, hvvr_gpu_view** out);
+
 // kernel launchers (all asynchronous on `stream`)
 void launchSceneRefit(hvvr_gpu_scene* s, hipStream_t stream);
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream);

@@ -195,6 +195,30 @@ int hvvr_host_scene_copy(const hvvr_host_scene* s, int array, void* dst, uint64_
     }
 }
 
+int hvvr_host_view_preset_params(int preset, uint32_t eye, uint32_t width, uint32_t height, float params[8]) {
+    if (!params)
+        return fail("null output");
+    CameraPreset cam;
+    if (preset == HVVR_VIEW_PRESET_HEIGHTFIELD)
+        cam = heightfieldCamera();
+    else if (preset == HVVR_VIEW_PRESET_ROOM_EYE)
+        cam = roomEyeCamera(eye, width ? width : 1080, height ? height : 1200);
+    else
+        return fail("unknown view preset");
+    const float p[8] = {cam.viewportLower.x, cam.viewportLower.y, cam.viewportUpper.x, cam.viewportUpper.y,
+                        cam.barrelK1,        cam.barrelK2,        cam.lens.radius,     cam.lens.focalDistance};
+    memcpy(params, p, sizeof(p));
+    return 0;
+}
+
+int hvvr_host_sample_to_camera(const float vp[4], float out[9]) {
+    if (!vp || !out)
+        return fail("null argument");
+    const matrix3x3 m = sampleToCamera(vector2(vp[0], vp[1]), vector2(vp[2], vp[3]));
+    memcpy(out, &m, sizeof(m));
+    return 0;
+}
+
 int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint32_t height, hvvr_host_view** out) {
     if (!out)
         return fail("null output");

@@ -104,6 +104,34 @@ typedef struct {
 
 /* GPUCamera::updateConfig (gpu_camera.cu:149-192). */
 int hvvr_gpu_view_create(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* desc, hvvr_gpu_view** out);
+
+/* ---- sample setup on the device: Camera::setSamples (camera.cpp:21-64, 135-156) ---- */
+typedef struct {
+    uint32_t msaaRate;
+    uint32_t width, height, split;  /* render target the remap addresses; colour samples per pixel */
+    float sampleToCamera[9];        /* column-major 3x3: Camera::getSampleToCamera (camera.cpp:167-171) */
+    float lensRadius, focalDistance;/* ThinLens */
+    float fovXDegrees, fovYDegrees; /* fovXDegrees > 0: rays from a spherical section of this size
+                                       (Camera::setSphericalWarpSettings, camera.cpp:206-224) */
+} hvvr_gpu_sample_setup_desc;
+/* A view whose sample hierarchy is built on the device from samples (Sample[count], 28 B
+ * each, host memory): naiveXYCluster (sample_clustering.h:17-49) as stable sorts, then beams,
+ * padded tile and block bounds, frusta (sample_hierarchy.cpp:46-190) and the remap
+ * (camera.cpp:57-63); equal to the host setup bit for bit for the perspective mapping.
+ * boundsOut (may be NULL): float[4] sample bounds, lower x y, upper x y (camera.cpp:36-42). */
+int hvvr_gpu_view_create_from_samples(hvvr_gpu_context* ctx, const hvvr_gpu_sample_setup_desc* desc,
+                                      const void* samples, uint32_t count, hvvr_gpu_view** out, float* boundsOut);
+/* The same for the width x height grid (getGridSamples, samples.cpp:21-42) generated on the
+ * device, with a radial barrel distortion r' = r (1 + k1 r^2 + k2 r^4) when k1 or k2 is
+ * nonzero. split must be 1. */
+int hvvr_gpu_view_create_grid(hvvr_gpu_context* ctx, const hvvr_gpu_sample_setup_desc* desc, float k1, float k2,
+                              hvvr_gpu_view** out, float* boundsOut);
+/* A view's sample hierarchy: DirectionalBeam[sampleCount], SimpleRayFrustum[tileCount],
+ * SimpleRayFrustum[blockCount], int32[width*height*split] (any may be NULL). Synchronous. */
+int hvvr_gpu_view_read_sample_setup(hvvr_gpu_view* view, void* beams, void* tileFrusta, void* blockFrusta,
+                                    int32_t* remap);
+/* shape[8] = msaaRate, sampleCount, validSampleCount, tileCount, blockCount, width, height, split */
+int hvvr_gpu_view_get_shape(hvvr_gpu_view* view, uint32_t shape[8]);
 int hvvr_gpu_view_destroy(hvvr_gpu_view* view);
 /* Raycaster::intersectAndResolveBeamBatch + GPUCamera::intersectShadeResolve + remap
  * (render.cpp:418-451, gpu_camera.cu:271-293, remap.cu:79-124): one frame of one view.

@@ -68,6 +68,12 @@ enum {
     HVVR_VIEW_REMAP = 3         /* int32[width*height*split] */
 };
 int hvvr_host_view_create_preset(int preset, uint32_t eye, uint32_t width, uint32_t height, hvvr_host_view** out);
+/* a preset's camera: params[8] = viewport lower x y, upper x y, barrel k1 k2, lens radius,
+ * focal distance */
+int hvvr_host_view_preset_params(int preset, uint32_t eye, uint32_t width, uint32_t height, float params[8]);
+/* Camera::getSampleToCamera of a viewport (lower x y, upper x y), column-major 3x3
+ * (camera.cpp:167-171) */
+int hvvr_host_sample_to_camera(const float viewportLowerUpper[4], float out[9]);
 /* samples: Sample[count] (28 bytes each: pixel x,y u32; position x,y f32; minor axis x,y f32;
  * major axis length f32) */
 int hvvr_host_view_create_from_samples(const void* samples, uint32_t count, uint32_t split, uint32_t width,
