@@ -779,6 +779,18 @@ class Server:
         self.shapes.append((view.height, view.width))
         return sid.value
 
+    def add_grid_shape(self, width, height, msaa_rate, viewport, barrel=(0.0, 0.0), lens=(0.0, 1.0)):
+        """A grid shape whose views set up their samples on the device (hvvr_server_add_grid_shape)."""
+        s2c = np.zeros(9, np.float32)
+        _check_host(load_host().hvvr_host_sample_to_camera(_ptr(np.asarray(viewport, np.float32)), _ptr(s2c)))
+        d = _SetupDesc(msaa_rate, width, height, 1, (c_float * 9)(*[float(x) for x in s2c]), float(lens[0]),
+                       float(lens[1]), 0.0, 0.0)
+        sid = c_uint32()
+        _check_host(self.lib.hvvr_server_add_grid_shape(self.handle, byref(d), c_float(barrel[0]), c_float(barrel[1]),
+                                                        byref(sid)))
+        self.shapes.append((height, width))
+        return sid.value
+
     def add_foveated_shape(self, fov: HostFoveated, screen: HostView, msaa_rate):
         sid = c_uint32()
         _check_host(self.lib.hvvr_server_add_foveated_shape(self.handle, fov.handle, screen.handle,

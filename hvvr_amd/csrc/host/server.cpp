@@ -43,9 +43,12 @@ struct Shape {
     bool foveated = false;
     uint32_t msaa = 1;
     uint32_t width = 0, height = 0;
-    // grid shape
+    // grid shape: a host sample setup, or a grid built on the device per view
     SampleSetup setup;
     ThinLens lens = {0.0f, 1.0f};
+    bool deviceGrid = false;
+    hvvr_gpu_sample_setup_desc grid = {};
+    float k1 = 0.0f, k2 = 0.0f;
     // foveated shape
     FoveatedSetup fov;
     matrix3x3 screenSampleToCam = matrix3x3::identity();
@@ -138,7 +141,8 @@ struct hvvr_server {
         }
         PooledView* pv = new PooledView();
         pv->shape = shapeId;
-        int rc = hvvr_gpu_view_create(ctx, &d, &pv->view);
+        int rc = sh.deviceGrid ? hvvr_gpu_view_create_grid(ctx, &sh.grid, sh.k1, sh.k2, &pv->view, nullptr)
+                               : hvvr_gpu_view_create(ctx, &d, &pv->view);
         if (!rc && sh.foveated) {
             hvvr_gpu_foveated_desc f = {};
             f.polarRemap = (const uint32_t*)sh.fov.polarRemap.data();
@@ -373,6 +377,39 @@ int hvvr_server_add_shape(hvvr_server* s, const hvvr_host_view* v, uint32_t msaa
     int rc = hvvr_gpu_view_estimate_bytes(&d, nullptr, &sh.bytes);
     if (rc)
         return gpuFail("shape", rc);
+    if (sh.bytes > s->budget)
+        return serverFail("a view of this shape exceeds the server's device budget", HVVR_GPU_ERR_OVERFLOW);
+    s->shapes.push_back(std::move(sh));
+    *id = uint32_t(s->shapes.size() - 1);
+    return HVVR_GPU_OK;
+}
+
+int hvvr_server_add_grid_shape(hvvr_server* s, const hvvr_gpu_sample_setup_desc* g, float k1, float k2,
+                               uint32_t* id) {
+    if (!s || !g || !id || g->width == 0 || g->height == 0 || g->split != 1)
+        return serverFail("invalid grid shape");
+    Shape sh;
+    sh.deviceGrid = true;
+    sh.grid = *g;
+    sh.k1 = k1;
+    sh.k2 = k2;
+    sh.msaa = g->msaaRate;
+    sh.width = g->width;
+    sh.height = g->height;
+    sh.lens = {g->lensRadius, g->focalDistance};
+    const uint64_t count = uint64_t(g->width) * g->height;
+    hvvr_gpu_view_desc d = {};
+    d.msaaRate = g->msaaRate;
+    d.sampleCount = uint32_t((count + BLOCK_SIZE - 1) / BLOCK_SIZE * BLOCK_SIZE);
+    d.validSampleCount = uint32_t(count);
+    d.width = g->width;
+    d.height = g->height;
+    d.split = 1;
+    d.lensRadius = g->lensRadius;
+    int rc = hvvr_gpu_view_estimate_bytes(&d, nullptr, &sh.bytes);
+    if (rc)
+        return gpuFail("shape", rc);
+    sh.bytes += count * 4; // the remap (the estimate counts it when the description carries one)
     if (sh.bytes > s->budget)
         return serverFail("a view of this shape exceeds the server's device budget", HVVR_GPU_ERR_OVERFLOW);
     s->shapes.push_back(std::move(sh));

@@ -33,6 +33,11 @@ int hvvr_server_destroy(hvvr_server* server);
 /* A view shape requests name: the sample setup of a host view (width x height image) at an
  * MSAA rate. The server copies what it needs; the host view may be destroyed afterwards. */
 int hvvr_server_add_shape(hvvr_server* server, const hvvr_host_view* view, uint32_t msaaRate, uint32_t* shapeId);
+/* A grid shape whose views build their sample hierarchy on the device
+ * (hvvr_gpu_view_create_grid: width x height grid samples, barrel distortion k1 k2, the
+ * descriptor's viewport mapping and lens): adding it does no host-side sample setup. */
+int hvvr_server_add_grid_shape(hvvr_server* server, const hvvr_gpu_sample_setup_desc* grid, float k1, float k2,
+                               uint32_t* shapeId);
 /* A polar foveated shape: the polar sample set, reconstructed into `screen`'s image, whose
  * sample space and bounds give the eye transforms and cull planes. */
 int hvvr_server_add_foveated_shape(hvvr_server* server, const hvvr_host_foveated* fov, const hvvr_host_view* screen,

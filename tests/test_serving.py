@@ -221,3 +221,29 @@ def test_serving_mix_four_scenes_matches_oracle(H, O, ctx):
         gv.close()
     for g in gscenes:
         g.close()
+
+
+@pytest.mark.gpu
+def test_device_grid_shapes_serve_like_host_shapes(H, ctx, scene2):
+    # a grid shape whose views set up their samples on the device serves the same images
+    # as the same camera's host-built shape, and its footprint estimate is exact
+    w, h = 270, 300
+    host = H.HostView(H.VIEW_PRESET_ROOM_EYE, 1, w, h)
+    vp, barrel, lens = H.preset_params(H.VIEW_PRESET_ROOM_EYE, 1, w, h)
+    gs = H.GpuScene(ctx, scene2)
+    srv = H.Server(ctx, 8 << 30, 3)
+    a = srv.add_shape(host, 4)
+    b = srv.add_grid_shape(w, h, 4, vp, barrel, lens)
+    reqs = [(_camera(host.camera_to_world, k), k % 2) for k in range(6)]
+    out = [srv.submit(gs, (a, b)[which], m, None) for m, which in reqs]
+    out += [srv.submit(gs, b if which == 0 else a, m, None) for m, which in reqs]
+    srv.drain()
+    st = srv.stats()
+    assert st.completed == 12 and st.failed == 0
+    for k in range(6):
+        assert np.array_equal(out[k][1], out[6 + k][1]), f"request {k}: device and host shapes differ"
+        assert (out[k][1] != 0).mean() > 0.9
+    gv = H.GpuDeviceSetupView(ctx, w, h, 4, viewport=vp, barrel=barrel, lens=lens)
+    assert srv.shape_bytes(b) == ctypes_bytes(H, gv)
+    gv.close()
+    srv.close()
