@@ -354,6 +354,17 @@ class _SceneDesc(ctypes.Structure):
                 ("transformCount", c_uint32)]
 
 
+class _TextureDesc(ctypes.Structure):
+    _fields_ = [("format", c_uint32), ("width", c_uint32), ("height", c_uint32), ("strideElements", c_uint32),
+                ("data", c_void_p)]
+
+
+# TextureFormat (include/hvvr/raycaster.h) -> (code, numpy dtype of one channel, channels)
+TEXTURE_FORMATS = {"srgb8": (1, np.uint8, 4), "unorm8": (2, np.uint8, 4), "unorm16": (3, np.uint16, 4),
+                   "float4": (4, np.float32, 4), "half4": (5, np.float16, 4), "r11g11b10": (6, np.uint32, 1),
+                   "r32": (7, np.float32, 1)}
+
+
 class _ViewDesc(ctypes.Structure):
     _fields_ = [("msaaRate", c_uint32), ("sampleCount", c_uint32), ("validSampleCount", c_uint32),
                 ("beams", c_void_p), ("tileFrusta", c_void_p), ("blockFrusta", c_void_p), ("remap", c_void_p),
@@ -437,6 +448,33 @@ class GpuScene:
     def read_vertices(self):
         out = np.zeros(len(self.scene.verts), DTYPES["vertex"])
         _check_gpu(self.lib.hvvr_gpu_scene_read_vertices(self.handle, _ptr(out)))
+        return out
+
+    def set_textures(self, textures):
+        """Replace the scene's textures with [(format, texels), ...], format a key of
+        TEXTURE_FORMATS and texels [h, w, 4] ([h, w] for r11g11b10 and r32); texture i is
+        the index materials name. Mip chains are built on the device."""
+        keep, descs = [], (_TextureDesc * max(len(textures), 1))()
+        for i, (fmt, data) in enumerate(textures):
+            code, dt, ch = TEXTURE_FORMATS[fmt]
+            a = np.ascontiguousarray(data, dt)
+            if a.ndim != (3 if ch > 1 else 2) or (ch > 1 and a.shape[2] != ch):
+                raise ValueError(f"texture {i}: texels of format {fmt} are [h, w{', 4' if ch > 1 else ''}]")
+            keep.append(a)
+            descs[i] = _TextureDesc(code, a.shape[1], a.shape[0], a.shape[1], _ptr(a))
+        _check_gpu(self.lib.hvvr_gpu_scene_set_textures(self.handle, c_uint32(len(textures)), descs))
+
+    def texture_info(self, index):
+        """(width, height, levels, stored kind, chain bytes) of texture `index`"""
+        info = (c_uint32 * 4)()
+        nbytes = c_uint64(0)
+        _check_gpu(self.lib.hvvr_gpu_scene_texture_info(self.handle, c_uint32(index), info, byref(nbytes)))
+        return info[0], info[1], info[2], info[3], nbytes.value
+
+    def read_texture(self, index):
+        """the stored mip chain of texture `index` as bytes"""
+        out = np.zeros(self.texture_info(index)[4], np.uint8)
+        _check_gpu(self.lib.hvvr_gpu_scene_read_texture(self.handle, c_uint32(index), _ptr(out)))
         return out
 
     def close(self):

@@ -145,7 +145,9 @@ int batchFail(const std::string& msg, int code = HVVR_GPU_ERR_INVALID) {
 
 // everything the launches depend on besides the per-frame arguments
 std::vector<uint64_t> frameSignature(const hvvr_gpu_batch* b, const hvvr_gpu_scene* s) {
-    std::vector<uint64_t> sig = {uint64_t(uintptr_t(s)), uint64_t(b->refitEveryFrame), b->shape.largeGroups,
+    // (whether the scene has textures picks the resolve variant a captured graph holds)
+    std::vector<uint64_t> sig = {uint64_t(uintptr_t(s)), uint64_t(s->textureCount > 0), uint64_t(b->refitEveryFrame),
+                                 b->shape.largeGroups,
                                  b->shape.hugeGroups,    b->shape.giantGroups,          b->shape.walkGroups,
                                  b->shape.xcdSwizzle,    b->shape.streams};
     for (const hvvr_gpu_view* v : b->views) {
@@ -294,7 +296,7 @@ void fillArgs(Chain& c, const hvvr_gpu_scene* s, const matrix4x4* m) {
 }
 
 // one chain's work on its stream
-void enqueueChain(const hvvr_gpu_batch* b, Chain& c) {
+void enqueueChain(const hvvr_gpu_batch* b, Chain& c, bool tex) {
     const uint32_t n = uint32_t(c.views.size());
     hipStream_t st = c.stream;
     uint8_t* d = c.devArgs.ptr;
@@ -312,7 +314,7 @@ void enqueueChain(const hvvr_gpu_batch* b, Chain& c) {
         const uint32_t gn = uint32_t(g.members.size());
         launchIntersectBatch(int(g.msaaRate), reinterpret_cast<const IntersectArgs*>(d + g.intersectOff),
                              u32(g.startOff), gn, g.units, b->shape, st);
-        launchResolveBatch(int(g.msaaRate), g.half, reinterpret_cast<const ResolveArgs*>(d + g.resolveOff),
+        launchResolveBatch(int(g.msaaRate), g.half, tex, reinterpret_cast<const ResolveArgs*>(d + g.resolveOff),
                            u32(g.startOff), gn, g.units, b->shape, st);
     }
     if (!c.remapMembers.empty())
@@ -330,7 +332,7 @@ void enqueueFrame(hvvr_gpu_batch* b, hvvr_gpu_scene* s) {
     for (size_t k = 1; k < b->chains.size(); k++)
         (void)hipStreamWaitEvent(b->chains[k]->stream, b->fork, 0);
     for (auto& c : b->chains)
-        enqueueChain(b, *c);
+        enqueueChain(b, *c, s->textureCount > 0);
     for (size_t k = 1; k < b->chains.size(); k++) {
         (void)hipEventRecord(b->chains[k]->joined, b->chains[k]->stream);
         (void)hipStreamWaitEvent(b->stream, b->chains[k]->joined, 0);

@@ -3,6 +3,7 @@
 
 #include "../../../include/hvvr_gpu.h"
 #include "hvvr/types.h"
+#include "texture.h"
 
 #include <hip/hip_runtime.h>
 
@@ -112,6 +113,13 @@ struct hvvr_gpu_scene {
     hvvr::DeviceBuffer<uint32_t> taskChild, taskNodeChild;
     std::vector<hvvr::RefitLevel> levels; // deepest first
     hvvr::LightingEnvironment lighting = {};
+    // textures (hvvr_gpu_scene_set_textures): every texture's mip chain in one buffer
+    uint32_t textureCount = 0;
+    hvvr::DeviceBuffer<uint8_t> texels;
+    hvvr::DeviceBuffer<hvvr::TexInfo> texInfo;
+    hvvr::DeviceBuffer<float> texLut; // [512]: TextureSet::srgbLut, then lodLut
+    std::vector<hvvr::TexInfo> texInfoHost;
+    std::vector<uint64_t> texBase, texBytes; // each texture's chain: start and size in `texels`
     bool dirty = true;
     bool everUpdated = false;
     hipEvent_t ready = nullptr; // recorded on the context stream after each update

@@ -123,6 +123,7 @@ struct ResolveArgs {
     // foveated path: per sample, the view-axis distance of subsample 0's hit (infinity on
     // a miss; ResolveKernel's TMaxBuffer, resolve.cu:386-457); null = not written
     float* tMax;
+    TextureSet tex; // the scene's textures (count 0: none)
 };
 
 struct RemapArgs {
@@ -175,8 +176,9 @@ void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t m
                          hipStream_t stream);
 void launchIntersectBatch(int msaaRate, const IntersectArgs* all, const uint32_t* start, uint32_t viewCount,
                           uint32_t units, const LaunchShape& shape, hipStream_t stream);
-void launchResolveBatch(int msaaRate, bool half, const ResolveArgs* all, const uint32_t* start, uint32_t viewCount,
-                        uint32_t units, const LaunchShape& shape, hipStream_t stream);
+// tex: the scene has textures (the resolve variant that reads them)
+void launchResolveBatch(int msaaRate, bool half, bool tex, const ResolveArgs* all, const uint32_t* start,
+                        uint32_t viewCount, uint32_t units, const LaunchShape& shape, hipStream_t stream);
 void launchRemapBatch(const RemapArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
                       hipStream_t stream);
 

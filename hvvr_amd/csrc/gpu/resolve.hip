@@ -11,10 +11,11 @@
 //   remapKernel    RemapKernel (raycaster/remap.cu:35-77): sample order -> image order,
 //                  split-colour merge for 2 (pentile) or 3 channels.
 //
-// Texture lookups are out of scope (the benchmark scenes are untextured): the UV
-// derivatives the reference computes for tex2DGrad (resolve.cu:264-275) only feed
-// texture reads, so they are not computed; materials with a texture index other than
-// SimpleMaterial::badTextureIndex shade with their constant colours.
+// Textures (texture.h): a material naming textures (SimpleMaterial's diffuse / specular /
+// glossy fields) reads them with the UV derivatives the reference takes for tex2DGrad
+// (resolve.cu:258-275): barycentrics at the centroid and at the centroid moved by one unit
+// along each subsample axis, the interpolated UVs' differences. Untextured materials skip
+// that work and shade with their constant colours.
 
 #include "device_common.h"
 #include "dof.h"
@@ -54,18 +55,90 @@ __device__ inline vector3 pbrBrdf(vector3 L, vector3 N, vector3 V, vector3 base,
     return diffuse + lightColor * spec;
 }
 
-__device__ inline vector3 ggxShade(const SimpleMaterial& mat, vector3 pos, vector3 nrm, vector3 camPos,
+// A material's inputs to GGXShade (shading.h:84-161): the constants, or where the
+// material names a texture, its value at the sample (shading.h:99-127): the emissive
+// model's colour from the diffuse slot, else base colour, F0 and roughness = 1 - glossy.x.
+struct SurfaceInputs {
+    uint32_t model_;
+    vector3 emissiveColor_; // emissive model
+    vector3 emissive_;      // added to the radiance of the GGX model (never textured)
+    vector3 base_, F0_;
+    float roughness_;
+    __device__ uint32_t model() const { return model_; }
+    __device__ vector3 emissiveColor() const { return emissiveColor_; }
+    __device__ vector3 emissive() const { return emissive_; }
+    __device__ vector3 base() const { return base_; }
+    __device__ vector3 F0() const { return F0_; }
+    __device__ float roughness() const { return roughness_; }
+};
+// an untextured material's inputs, read where the shading uses them
+struct MaterialInputs {
+    const SimpleMaterial& m;
+    __device__ uint32_t model() const { return uint32_t(m.textureIDsAndShadingModel & 0xffffu); }
+    __device__ vector3 emissiveColor() const { return m.emissive.xyz(); }
+    __device__ vector3 emissive() const { return m.emissive.xyz(); }
+    __device__ vector3 base() const { return m.diffuse.xyz(); }
+    __device__ vector3 F0() const { return m.specular.xyz(); }
+    __device__ float roughness() const { return 1.0f - m.glossiness; }
+};
+
+__device__ inline bool textureIndexValid(uint32_t id, const TextureSet& tex) {
+    return id != SimpleMaterial::badTextureIndex && id < tex.count;
+}
+
+// whether shading this material reads a texture
+__device__ inline bool materialTextured(const SimpleMaterial& mat, const TextureSet& tex) {
+    const uint64_t code = mat.textureIDsAndShadingModel;
+    const uint32_t mask = SimpleMaterial::textureMask;
+    const uint32_t diffuse = uint32_t(code >> SimpleMaterial::diffuseBitShift) & mask;
+    if (uint32_t(code & 0xffffu) == ShadingModelEmissive)
+        return textureIndexValid(diffuse, tex);
+    return textureIndexValid(diffuse, tex) ||
+           textureIndexValid(uint32_t(code >> SimpleMaterial::specularBitShift) & mask, tex) ||
+           textureIndexValid(uint32_t(code >> SimpleMaterial::glossyBitShift) & mask, tex);
+}
+
+__device__ inline SurfaceInputs surfaceInputs(const SimpleMaterial& mat, const TextureSet& tex, vector2 uv,
+                                              vector2 ddx, vector2 ddy) {
+    SurfaceInputs in;
+    const uint64_t code = mat.textureIDsAndShadingModel;
+    in.model_ = uint32_t(code & 0xffffu);
+    in.emissiveColor_ = mat.emissive.xyz();
+    in.emissive_ = mat.emissive.xyz();
+    in.base_ = mat.diffuse.xyz();
+    in.F0_ = mat.specular.xyz();
+    in.roughness_ = 1.0f - mat.glossiness;
+    const uint32_t mask = SimpleMaterial::textureMask;
+    const uint32_t diffuse = uint32_t(code >> SimpleMaterial::diffuseBitShift) & mask;
+    const uint32_t specular = uint32_t(code >> SimpleMaterial::specularBitShift) & mask;
+    const uint32_t glossy = uint32_t(code >> SimpleMaterial::glossyBitShift) & mask;
+    auto read = [&](uint32_t id) { return texSampleGrad(tex.texels, tex.info[id], tex.srgbLut, tex.lodLut, uv, ddx, ddy); };
+    if (in.model_ == ShadingModelEmissive) {
+        if (textureIndexValid(diffuse, tex))
+            in.emissiveColor_ = read(diffuse).xyz();
+        return in;
+    }
+    if (textureIndexValid(diffuse, tex))
+        in.base_ = read(diffuse).xyz();
+    if (textureIndexValid(specular, tex))
+        in.F0_ = read(specular).xyz();
+    if (textureIndexValid(glossy, tex))
+        in.roughness_ = 1.0f - read(glossy).x;
+    return in;
+}
+
+template <class In>
+__device__ inline vector3 ggxShade(const In& in, vector3 pos, vector3 nrm, vector3 camPos,
                                    const LightingEnvironment& env) {
-    // GGXShade (shading.h:84-161), untextured
+    // GGXShade (shading.h:84-161)
     vector3 normal = normalize(nrm);
-    uint32_t model = uint32_t(mat.textureIDsAndShadingModel & 0xffffu);
-    if (model == ShadingModelEmissive)
-        return mat.emissive.xyz();
-    vector3 base = mat.diffuse.xyz();
+    if (in.model() == ShadingModelEmissive)
+        return in.emissiveColor();
+    vector3 base = in.base();
     vector3 V = normalize(camPos - pos);
-    vector3 F0 = mat.specular.xyz();
-    float roughness = 1.0f - mat.glossiness;
-    vector3 rad = mat.emissive.xyz();
+    vector3 F0 = in.F0();
+    float roughness = in.roughness();
+    vector3 rad = in.emissive();
     rad += base * vector3(0.15f, 0.15f, 0.165f);
     for (int i = 0; i < MAX_DIRECTIONAL_LIGHTS; i++) {
         if (i >= env.directionalLightCount)
@@ -144,17 +217,17 @@ __device__ inline void pbrBrdfHalf(const vector3& L, const vector3& N, const vec
     rad += __builtin_convertvector(u, float4v) + __builtin_convertvector(w, float4v) * sc;
 }
 
-__device__ inline vector3 ggxShadeHalf(const SimpleMaterial& mat, vector3 pos, vector3 nrm, vector3 camPos,
+template <class In>
+__device__ inline vector3 ggxShadeHalf(const In& in, vector3 pos, vector3 nrm, vector3 camPos,
                                        const LightingEnvironment& env) {
     vector3 normal = normalizeFast(nrm);
-    uint32_t model = uint32_t(mat.textureIDsAndShadingModel & 0xffffu);
-    if (model == ShadingModelEmissive)
-        return mat.emissive.xyz();
-    const half4v base = toHalf4(mat.diffuse.xyz());
+    if (in.model() == ShadingModelEmissive)
+        return in.emissiveColor();
+    const half4v base = toHalf4(in.base());
     vector3 V = normalizeFast(camPos - pos);
-    const half4v F0 = toHalf4(mat.specular.xyz());
-    float roughness = 1.0f - mat.glossiness;
-    vector3 amb = mat.emissive.xyz() + mat.diffuse.xyz() * vector3(0.15f, 0.15f, 0.165f);
+    const half4v F0 = toHalf4(in.F0());
+    float roughness = in.roughness();
+    vector3 amb = in.emissive() + in.base() * vector3(0.15f, 0.15f, 0.165f);
     float4v rad = {amb.x, amb.y, amb.z, 0.0f};
     for (int i = 0; i < MAX_DIRECTIONAL_LIGHTS; i++) {
         if (i >= env.directionalLightCount)
@@ -195,7 +268,9 @@ __device__ inline float acesFilm(float x) {
     return saturate((x * (a * x + b)) / (x * (c * x + d) + e));
 }
 
-template <int A, bool DoF, bool Half>
+// Tex: the scene has textures (a variant of its own: the texture reads' registers would
+// cost untextured scenes occupancy)
+template <int A, bool DoF, bool Half, bool Tex>
 __device__ inline void resolveBody(const ResolveArgs& args, uint32_t tile) {
     __shared__ DofTile dofTile;
     const uint32_t tid = threadIdx.x;
@@ -273,8 +348,33 @@ __device__ inline void resolveBody(const ResolveArgs& args, uint32_t tile) {
         vector3 n1(halfToFloat(V1.normal[0]), halfToFloat(V1.normal[1]), halfToFloat(V1.normal[2]));
         vector3 n2(halfToFloat(V2.normal[0]), halfToFloat(V2.normal[1]), halfToFloat(V2.normal[2]));
         vector3 nrm = n0 * bx + n1 * by + n2 * bz;
-        vector3 c = Half ? ggxShadeHalf(args.materials[S.material], pos, nrm, cam, args.env)
-                         : ggxShade(args.materials[S.material], pos, nrm, cam, args.env);
+        const SimpleMaterial& mat = args.materials[S.material];
+        const bool textured = Tex && materialTextured(mat, args.tex);
+        vector2 uv(0.0f), dUVdX(0.0f), dUVdY(0.0f);
+        if (textured) {
+            // UV derivatives (resolve.cu:258-275): barycentrics one unit along each
+            // subsample axis from the centroid (for depth of field, along each axis of the
+            // focal-plane offset)
+            float xb0, xb1, xb2, yb0, yb1, yb2;
+            if constexpr (DoF) {
+                dofUVW(T, cam, dofTile, lcfc, lx, ly, dx + 1.0f, dy, xb0, xb1, xb2);
+                dofUVW(T, cam, dofTile, lcfc, lx, ly, dx, dy + 1.0f, yb0, yb1, yb2);
+            } else {
+                pinholeUVW(T, cam, ndir, ndu, ndv, cx + 1.0f, cy, xb0, xb1, xb2);
+                pinholeUVW(T, cam, ndir, ndu, ndv, cx, cy + 1.0f, yb0, yb1, yb2);
+            }
+            uv = V0.uv * bx + V1.uv * by + V2.uv * bz;
+            dUVdX = (V0.uv * xb0 + V1.uv * xb1 + V2.uv * xb2) - uv;
+            dUVdY = (V0.uv * yb0 + V1.uv * yb1 + V2.uv * yb2) - uv;
+        }
+        vector3 c;
+        if (textured) {
+            const SurfaceInputs in = surfaceInputs(mat, args.tex, uv, dUVdX, dUVdY);
+            c = Half ? ggxShadeHalf(in, pos, nrm, cam, args.env) : ggxShade(in, pos, nrm, cam, args.env);
+        } else {
+            const MaterialInputs in{mat};
+            c = Half ? ggxShadeHalf(in, pos, nrm, cam, args.env) : ggxShade(in, pos, nrm, cam, args.env);
+        }
         result += c * float(__popc(mask));
     }
     result = result * (1.0f / float(A));
@@ -306,20 +406,20 @@ __device__ inline void resolveBody(const ResolveArgs& args, uint32_t tile) {
     }
 }
 
-template <int A, bool DoF, bool Half>
+template <int A, bool DoF, bool Half, bool Tex>
 __global__ __launch_bounds__(128) void resolveKernel(ResolveArgs args) {
-    resolveBody<A, DoF, Half>(args, blockIdx.x);
+    resolveBody<A, DoF, Half, Tex>(args, blockIdx.x);
 }
 
 // every tile of the views whose units `start` lists (pinhole views of one MSAA rate and
 // shading precision)
-template <int A, bool Half>
+template <int A, bool Half, bool Tex>
 __global__ __launch_bounds__(128) void resolveBatchKernel(const ResolveArgs* __restrict__ all,
                                                           const uint32_t* __restrict__ start, uint32_t n,
                                                           uint32_t swizzle) {
     const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
     const uint32_t v = batchView(start, n, unit);
-    resolveBody<A, false, Half>(all[v], unit - start[v]);
+    resolveBody<A, false, Half, Tex>(all[v], unit - start[v]);
 }
 
 __device__ inline void remapBody(const RemapArgs& a, uint32_t x, uint32_t y) {
@@ -380,6 +480,7 @@ ResolveArgs resolveArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const m
     a.lensRadius = v->lens.radius;
     a.focalDistance = v->lens.focalDistance;
     a.tMax = v->foveated ? v->tMax.ptr : nullptr;
+    a.tex = {s->texels.ptr, s->texInfo.ptr, s->texLut.ptr, s->texLut.ptr + 256, s->textureCount};
     return a;
 }
 
@@ -399,17 +500,24 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
     dim3 grid(v->tileCount), block(128);
     const bool dof = v->dof;
     const bool half = v->halfShading;
+    const bool tex = s->textureCount > 0;
     switch (v->msaaRate) {
+#define HVVR_RESOLVE_TEX(A, D, F)                                                     \
+    if (tex)                                                                          \
+        resolveKernel<A, D, F, true><<<grid, block, 0, stream>>>(a);                  \
+    else                                                                              \
+        resolveKernel<A, D, F, false><<<grid, block, 0, stream>>>(a);
 #define HVVR_RESOLVE_CASE(A)                                                          \
     case A:                                                                           \
-        if (dof && half)                                                              \
-            resolveKernel<A, true, true><<<grid, block, 0, stream>>>(a);              \
-        else if (dof)                                                                 \
-            resolveKernel<A, true, false><<<grid, block, 0, stream>>>(a);             \
-        else if (half)                                                                \
-            resolveKernel<A, false, true><<<grid, block, 0, stream>>>(a);             \
-        else                                                                          \
-            resolveKernel<A, false, false><<<grid, block, 0, stream>>>(a);            \
+        if (dof && half) {                                                            \
+            HVVR_RESOLVE_TEX(A, true, true)                                           \
+        } else if (dof) {                                                             \
+            HVVR_RESOLVE_TEX(A, true, false)                                          \
+        } else if (half) {                                                            \
+            HVVR_RESOLVE_TEX(A, false, true)                                          \
+        } else {                                                                      \
+            HVVR_RESOLVE_TEX(A, false, false)                                         \
+        }                                                                             \
         break;
         HVVR_RESOLVE_CASE(1)
         HVVR_RESOLVE_CASE(2)
@@ -418,6 +526,7 @@ void launchResolve(hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m
         HVVR_RESOLVE_CASE(16)
         HVVR_RESOLVE_CASE(32)
 #undef HVVR_RESOLVE_CASE
+#undef HVVR_RESOLVE_TEX
         default: break;
     }
 }
@@ -428,15 +537,21 @@ void launchRemap(hvvr_gpu_view* v, hipStream_t stream) {
     remapKernel<<<grid, block, 0, stream>>>(remapArgs(v));
 }
 
-void launchResolveBatch(int msaaRate, bool half, const ResolveArgs* all, const uint32_t* start, uint32_t viewCount,
-                        uint32_t units, const LaunchShape& shape, hipStream_t stream) {
+void launchResolveBatch(int msaaRate, bool half, bool tex, const ResolveArgs* all, const uint32_t* start,
+                        uint32_t viewCount, uint32_t units, const LaunchShape& shape, hipStream_t stream) {
     switch (msaaRate) {
+#define HVVR_RESOLVE_BATCH_KERNEL(A, F, T) \
+    resolveBatchKernel<A, F, T><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle)
 #define HVVR_RESOLVE_BATCH_CASE(A)                                                                            \
     case A:                                                                                                   \
-        if (half)                                                                                             \
-            resolveBatchKernel<A, true><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle);  \
+        if (half && tex)                                                                                      \
+            HVVR_RESOLVE_BATCH_KERNEL(A, true, true);                                                         \
+        else if (half)                                                                                        \
+            HVVR_RESOLVE_BATCH_KERNEL(A, true, false);                                                        \
+        else if (tex)                                                                                         \
+            HVVR_RESOLVE_BATCH_KERNEL(A, false, true);                                                        \
         else                                                                                                  \
-            resolveBatchKernel<A, false><<<units, 128, 0, stream>>>(all, start, viewCount, shape.xcdSwizzle); \
+            HVVR_RESOLVE_BATCH_KERNEL(A, false, false);                                                       \
         break;
         HVVR_RESOLVE_BATCH_CASE(1)
         HVVR_RESOLVE_BATCH_CASE(2)
@@ -445,6 +560,7 @@ void launchResolveBatch(int msaaRate, bool half, const ResolveArgs* all, const u
         HVVR_RESOLVE_BATCH_CASE(16)
         HVVR_RESOLVE_BATCH_CASE(32)
 #undef HVVR_RESOLVE_BATCH_CASE
+#undef HVVR_RESOLVE_BATCH_KERNEL
         default: break;
     }
 }

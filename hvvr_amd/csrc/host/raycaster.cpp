@@ -372,12 +372,32 @@ Texture::Texture(const TextureData& data, uint32_t id) : _textureID(id), _data(d
 }
 
 Texture* Raycaster::createTexture(const TextureData& textureData) {
+    if (_nextTextureID >= SimpleMaterial::badTextureIndex)
+        throw RaycasterError("Raycaster::createTexture: texture IDs exhausted (65535 is reserved for 'none')");
     _textures.emplace_back(new Texture(textureData, _nextTextureID++));
+    _texturesDirty = true;
     return _textures.back().get();
 }
 
 void Raycaster::destroyTexture(Texture* texture) {
     eraseOwned(_textures, texture, "Raycaster::destroyTexture");
+    _texturesDirty = true;
+}
+
+void Raycaster::uploadTextures() {
+    // indexed by ID: destroyed IDs below the highest live one hold a black 1x1 stand-in
+    uint32_t count = 0;
+    for (const auto& t : _textures)
+        count = std::max(count, t->getID() + 1);
+    static const uint8_t black[4] = {0, 0, 0, 0};
+    std::vector<hvvr_gpu_texture_desc> descs(count, hvvr_gpu_texture_desc{uint32_t(TextureFormat::r8g8b8a8_unorm), 1,
+                                                                          1, 1, black});
+    for (const auto& t : _textures) {
+        const TextureData& d = t->getData();
+        descs[t->getID()] = {uint32_t(d.format), d.width, d.height, d.strideElements, d.data};
+    }
+    check(hvvr_gpu_scene_set_textures(_scene, count, descs.data()), "Raycaster: texture upload failed");
+    _texturesDirty = false;
 }
 
 Light* Raycaster::createLight(const LightUnion& lightUnion) {
@@ -444,6 +464,7 @@ void Raycaster::updateScene() {
         _sceneDirty = false;
         _transformsDirty = false;
         _lightsDirty = true;
+        _texturesDirty = true;
     }
     if (!_scene)
         return;
@@ -465,6 +486,8 @@ void Raycaster::updateScene() {
               "Raycaster: light upload failed");
         _lightsDirty = false;
     }
+    if (_texturesDirty)
+        uploadTextures();
     check(hvvr_gpu_scene_update(_scene), "Raycaster: scene update failed");
 }
 

@@ -162,8 +162,11 @@ size_t getTextureSize(uint32_t strideElements, uint32_t height, TextureFormat fo
 class Raycaster;
 
 // A texture the raycaster holds (texture.h:36-54); getID() is the index materials use in
-// their texture fields. Shading in this build is untextured: materials shade with their
-// constant colours whatever texture they name.
+// their texture fields (IDs count up from 0 in creation order, as the reference's atlas
+// does). The next updateScene uploads every live texture; shading reads them with wrap
+// addressing, trilinear mip filtering and up to 8 anisotropic probes. A material naming a
+// destroyed texture reads a black 1x1 stand-in; one naming an ID never created shades
+// with its constant colour.
 class Texture {
 public:
     Texture(const TextureData& data, uint32_t id);
@@ -304,6 +307,7 @@ public:
 
 protected:
     void updateScene();
+    void uploadTextures();
     bool growLists(hvvr_gpu_view* view);
     void transformsChanged() { _transformsDirty = true; }
 
@@ -316,6 +320,7 @@ protected:
     bool _sceneDirty = false;
     bool _transformsDirty = false;
     bool _lightsDirty = true;
+    bool _texturesDirty = false;
     hvvr_gpu_context* _ctx = nullptr;
     hvvr_gpu_scene* _scene = nullptr;
 };

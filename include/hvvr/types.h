@@ -71,6 +71,7 @@ struct vector4 {
     HVVR_HD vector4 operator+(const vector4& v) const { return vector4(x + v.x, y + v.y, z + v.z, w + v.w); }
     HVVR_HD vector4 operator-(const vector4& v) const { return vector4(x - v.x, y - v.y, z - v.z, w - v.w); }
     HVVR_HD vector4 operator*(float a) const { return vector4(x * a, y * a, z * a, w * a); }
+    HVVR_HD vector4 operator/(float a) const { return vector4(x / a, y / a, z / a, w / a); }
 };
 
 struct vector2ui {

@@ -82,6 +82,24 @@ int hvvr_gpu_scene_set_lights(hvvr_gpu_scene* scene, const void* lights, uint32_
 /* GPUSceneState::update (gpu_scene_state.cu:377-451): vertex transform, triangle precompute
  * and bottom-up BVH refit, issued on the context stream when the scene is dirty. */
 int hvvr_gpu_scene_update(hvvr_gpu_scene* scene);
+/* CreateTexture (texture.cu:210-272) for every texture of the scene at once, replacing any
+ * earlier set; texture i is the index materials name (SimpleMaterial texture fields). Level
+ * 0 is converted (8-bit formats kept as RGBA8, the rest as RGBA32F; a one-channel texture
+ * reads (r, 0, 0, 1)) and the mip chain built on the device. Shading samples them with
+ * wrap addressing, trilinear filtering and up to 8 anisotropic probes
+ * (hvvr_amd/csrc/gpu/texture.h has the exact arithmetic). Synchronous. */
+typedef struct {
+    uint32_t format; /* TextureFormat: 1 r8g8b8a8_unorm_srgb, 2 r8g8b8a8_unorm, 3 r16g16b16a16_unorm,
+                        4 r32g32b32a32_float, 5 r16g16b16a16_float, 6 r11g11b10_float, 7 r32_float */
+    uint32_t width, height, strideElements;
+    const void* data; /* strideElements x height texels of the format */
+} hvvr_gpu_texture_desc;
+int hvvr_gpu_scene_set_textures(hvvr_gpu_scene* scene, uint32_t count, const hvvr_gpu_texture_desc* textures);
+/* info[4] = width, height, mip levels, stored kind (0 RGBA8 unorm, 1 RGBA8 sRGB, 2 RGBA32F);
+ * *bytes = size of the texture's stored chain: level l at the sum of the earlier levels'
+ * sizes, each rounded up to 16 bytes */
+int hvvr_gpu_scene_texture_info(hvvr_gpu_scene* scene, uint32_t index, uint32_t info[4], uint64_t* bytes);
+int hvvr_gpu_scene_read_texture(hvvr_gpu_scene* scene, uint32_t index, void* chain);
 /* GPUSceneState::fetchUpdatedBVH (gpu_scene_state.cu:454-467) and friends: synchronous reads. */
 int hvvr_gpu_scene_read_nodes(hvvr_gpu_scene* scene, void* nodes);          /* BVHNode[nodeCount] */
 int hvvr_gpu_scene_read_triangles(hvvr_gpu_scene* scene, void* triangles);  /* PrecomputedTriangleIntersect[] */

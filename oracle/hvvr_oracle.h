@@ -190,6 +190,29 @@ void or_foveated_contrast(const uint32_t* unfiltered, const uint32_t* filtered, 
 float or_emap_apply(const float emap[6], float i);
 float or_emap_apply_inverse(const float emap[6], float e);
 
+/* ---- textures (or_texture.c; raycaster/texture.cu:78-272, shading.h:45-47, 99-127) ---- */
+/* one texture's mip chain: level l of max(1, w >> l) x max(1, h >> l) texels at byte
+ * offset[l] of the texel buffer; kind 0 = RGBA8 unorm, 1 = RGBA8 sRGB, 2 = RGBA32F */
+typedef struct { uint32_t width, height, levels, kind; uint64_t offset[16]; } o_texinfo;  /* 144 B */
+/* [0, 256): sRGB byte -> linear; [256, 512): mip-fraction thresholds 2^((j + 0.5) / 256) */
+void or_texture_tables(float out[512]);
+uint32_t or_texture_levels(uint32_t width, uint32_t height);
+/* level offsets from 0 (each level 16-byte aligned); returns the chain's bytes */
+uint64_t or_texture_layout(o_texinfo* t);
+/* levels 1.. from level 0 (d_mipmap) */
+void or_texture_build_mips(uint8_t* base, const o_texinfo* t);
+/* tex2DGrad(uv, ddx, ddy) with wrap addressing, trilinear and up to 8 anisotropic probes */
+void or_texture_sample(const uint8_t* base, const o_texinfo* t, float u, float v, float dxu, float dxv, float dyu,
+                       float dyv, float out[4]);
+/* level floor(lambda) and the mip fraction of lambda = log2(x) clamped to [0, levels - 1] */
+void or_texture_lod(float x, uint32_t levels, uint32_t* level, float* f);
+/* the textures later resolves read (offsets relative to `texels`); count 0 = none. The
+ * pointers are kept, not copied. */
+void or_set_textures(const uint8_t* texels, const o_texinfo* info, uint32_t count);
+/* texture `id` of the current set at (uv, ddx, ddy); 0 = no such texture (out untouched) */
+int or_texture_read(uint32_t id, float u, float v, float dxu, float dxv, float dyu, float dyv, float out[4]);
+int or_textures_present(uint32_t id);
+
 /* ---- refit ---- */
 /* World-space vertices, precomputed triangles and node bounds from scratch. nodes:
  * topology (bounds ignored), nodesOut gets bounds filled. */

@@ -176,7 +176,16 @@ def lighting_env(lights):
 
 
 def resolve_view(beams, m, offsets, A, tri_ids, tris, shade, verts, materials, env, precision=0, dof=None,
-                 counts=None):
+                 counts=None, textures=None):
+    """Resolve of every sample; `textures` = [(fmt, data), ...] the materials' texture
+    indices name (None: untextured)."""
+    if textures:
+        set_textures(textures)
+        try:
+            return resolve_view(beams, m, offsets, A, tri_ids, tris, shade, verts, materials, env, precision, dof,
+                                counts)
+        finally:
+            set_textures(None)
     beams = np.ascontiguousarray(beams)
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     offsets = np.ascontiguousarray(offsets, np.float32)
@@ -217,7 +226,7 @@ def refit(scene):
 
 
 def render_beams(scene, beams, tile_frusta, block_frusta, A, m, lens=(0.0, 1.0), cull_planes=None, precision=0,
-                 offsets=None):
+                 offsets=None, textures=None):
     """Oracle frame of a beam set (no remap): refit, frustum setup (tile frusta outside a
     cull plane skipped), traversal, intersect, resolve with empty tiles cleared, and the
     per-sample t-max of the foveated path. `offsets`: the frame's subsample offsets (the
@@ -234,14 +243,14 @@ def render_beams(scene, beams, tile_frusta, block_frusta, A, m, lens=(0.0, 1.0),
     tri_ids, depth, surv = intersect_view(beams, world_tiles, m, offsets, A, counts, lists, tris, dof)
     env, _ = lighting_env(scene.lights)
     colors, linear = resolve_view(beams, m, offsets, A, tri_ids, tris, scene.triShade, wv, scene.materials, env,
-                                  precision, dof, counts)
+                                  precision, dof, counts, textures)
     colors[np.repeat(counts == 0, 128)[: len(colors)]] = 0xFF000000
     tmax = tmax_view(beams, m, offsets, A, tri_ids, tris, counts, dof)
     return dict(nodes=nodes, tris=tris, counts=counts, lists=lists, tri_ids=tri_ids, depth=depth, survivors=surv,
                 colors=colors, linear=linear, tmax=tmax, offsets=offsets)
 
 
-def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=True):
+def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=True, textures=None):
     """Full oracle frame: refit, frustum setup, traversal, intersect, resolve, remap."""
     m = view.camera_to_world if camera_to_world is None else np.asarray(camera_to_world, np.float32)
     wv, tris, nodes = refit(scene)
@@ -258,7 +267,7 @@ def render_view(scene, view, A, camera_to_world=None, precision=0, with_colors=T
     if with_colors:
         env, _ = lighting_env(scene.lights)
         colors, linear = resolve_view(view.beams, m, offsets, A, tri_ids, tris, scene.triShade, wv, scene.materials,
-                                      env, precision, dof, counts)
+                                      env, precision, dof, counts, textures)
         # empty tiles: clear colour (ClearEmptyKernel, resolve.cu:527-542)
         empty = np.repeat(counts == 0, 128)
         colors[empty[: len(colors)]] = 0xFF000000
@@ -451,3 +460,123 @@ def cpu_baseline_run(scene, views, A, threads=0, budget_s=20.0, seed=0):
     if rc != 0:
         raise RuntimeError("CPU baseline: a tile traversal overflowed its stack")
     return st
+
+
+# ---- textures (or_texture.c) ----
+O_TEXINFO = np.dtype([("width", "<u4"), ("height", "<u4"), ("levels", "<u4"), ("kind", "<u4"),
+                      ("offset", "<u8", 16)])
+assert O_TEXINFO.itemsize == 144
+# TextureFormat (include/hvvr/raycaster.h): the natural numpy layout of each format's texels
+TEX_FORMATS = {"srgb8": 1, "unorm8": 2, "unorm16": 3, "float4": 4, "half4": 5, "r11g11b10": 6, "r32": 7}
+
+
+def _tex_lib():
+    L = lib()
+    if not getattr(L, "_tex_ready", False):
+        L.or_texture_sample.argtypes = [c_void_p, c_void_p] + [ctypes.c_float] * 6 + [c_void_p]
+        L.or_texture_lod.argtypes = [ctypes.c_float, c_uint32, c_void_p, c_void_p]
+        L.or_texture_layout.restype = c_uint64
+        L.or_texture_levels.restype = c_uint32
+        L._tex_ready = True
+    return L
+
+
+def _small_float(bits, mbits):
+    """unsigned float of 5 exponent and `mbits` mantissa bits (r11g11b10), as float64"""
+    e = (bits >> mbits).astype(np.int64)
+    m = (bits & ((1 << mbits) - 1)).astype(np.float64)
+    out = np.where(e == 0, m / (1 << mbits) * 2.0 ** -14, (1.0 + m / (1 << mbits)) * np.exp2(e - 15.0))
+    return np.where(e == 31, np.where(m > 0, np.nan, np.inf), out)
+
+
+def texture_level0(fmt, data):
+    """(kind, level 0 in the stored kind): RGBA8 bytes for srgb8 / unorm8 (kind 1 / 0),
+    RGBA32F (kind 2) for the rest; `data` is [h, w, 4] (r11g11b10 and r32: [h, w])."""
+    data = np.asarray(data)
+    if fmt in ("srgb8", "unorm8"):
+        return (1 if fmt == "srgb8" else 0), np.ascontiguousarray(data, np.uint8)
+    h, w = data.shape[:2]
+    out = np.zeros((h, w, 4), np.float32)
+    if fmt == "unorm16":
+        out[...] = data.astype(np.float32) / np.float32(65535.0)
+    elif fmt == "float4":
+        out[...] = data
+    elif fmt == "half4":
+        out[...] = data.astype(np.float16).astype(np.float32)
+    elif fmt == "r11g11b10":
+        c = data.astype(np.uint32)
+        out[..., 0] = _small_float(c & 0x7FF, 6)
+        out[..., 1] = _small_float((c >> 11) & 0x7FF, 6)
+        out[..., 2] = _small_float(c >> 22, 5)
+        out[..., 3] = 1.0
+    elif fmt == "r32":
+        out[..., 0] = data
+        out[..., 3] = 1.0
+    else:
+        raise ValueError(fmt)
+    return 2, out
+
+
+def texture_chain(fmt, data):
+    """(info, chain bytes) of one texture: level 0 converted, mips built (d_mipmap)."""
+    L = _tex_lib()
+    kind, level0 = texture_level0(fmt, data)
+    info = np.zeros(1, O_TEXINFO)
+    info["width"], info["height"] = level0.shape[1], level0.shape[0]
+    info["levels"] = L.or_texture_levels(c_uint32(level0.shape[1]), c_uint32(level0.shape[0]))
+    info["kind"] = kind
+    nbytes = L.or_texture_layout(_p(info))
+    chain = np.zeros(nbytes, np.uint8)
+    raw = level0.reshape(-1).view(np.uint8)
+    chain[: raw.size] = raw
+    L.or_texture_build_mips(_p(chain), _p(info))
+    return info, chain
+
+
+def texture_sample(info, chain, uv, ddx=(0.0, 0.0), ddy=(0.0, 0.0)):
+    out = np.zeros(4, np.float32)
+    _tex_lib().or_texture_sample(_p(chain), _p(info), float(uv[0]), float(uv[1]), float(ddx[0]), float(ddx[1]),
+                                 float(ddy[0]), float(ddy[1]), _p(out))
+    return out
+
+
+def texture_lod(x, levels):
+    lvl = np.zeros(1, np.uint32)
+    f = np.zeros(1, np.float32)
+    _tex_lib().or_texture_lod(float(x), c_uint32(levels), _p(lvl), _p(f))
+    return int(lvl[0]), float(f[0])
+
+
+def texture_tables():
+    out = np.zeros(512, np.float32)
+    lib().or_texture_tables(_p(out))
+    return out
+
+
+_tex_keep = None
+
+
+def set_textures(textures):
+    """Make [(fmt, data), ...] the textures later resolves read (None / [] = none); an
+    entry may also be a built (info, chain) pair of texture_chain. Returns [(info, chain)]
+    per texture."""
+    global _tex_keep
+    L = _tex_lib()
+    if not textures:
+        L.or_set_textures(None, None, c_uint32(0))
+        _tex_keep = None
+        return []
+    chains = [(f, d) if isinstance(f, np.ndarray) else texture_chain(f, d) for f, d in textures]
+    base, total = [], 0
+    for _, c in chains:
+        base.append(total)
+        total += (c.size + 255) & ~255
+    buf = np.zeros(max(total, 1), np.uint8)
+    info = np.zeros(len(chains), O_TEXINFO)
+    for i, (inf, c) in enumerate(chains):
+        buf[base[i]: base[i] + c.size] = c
+        info[i] = inf[0]
+        info["offset"][i, : int(inf[0]["levels"])] += np.uint64(base[i])
+    L.or_set_textures(_p(buf), _p(info), c_uint32(len(chains)))
+    _tex_keep = (buf, info)
+    return chains

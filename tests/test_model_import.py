@@ -183,3 +183,33 @@ def test_example_renders_a_bin_file(H, O, tmp_path):
     want = O.render_view(scene, view, 4, camera_to_world=H.transform_matrix(t, q, 1.0))["image"].reshape(h, w)
     assert (want != 0xFF000000).mean() > 0.2
     assert rgba_delta(img, want).max() <= 1
+
+
+@pytest.mark.gpu
+def test_example_renders_a_textured_bin_file(H, O, tmp_path):
+    # the heightfield scene with one sRGB texture (rows padded: stride 40 for width 32)
+    # named by every other material: the example uploads it through createTexture and the
+    # frame matches the oracle reading the same texture
+    src, path, out = tmp_path / "hf.bin", tmp_path / "hft.bin", tmp_path / "hft.rgba"
+    H.HostScene.save_bin(src, H.SCENE_HEIGHTFIELD_BOXES, 0)
+    _, lights, meshes = _parse(H, open(src, "rb").read())
+    rng = np.random.default_rng(8)
+    padded = rng.integers(0, 256, (32, 40, 4)).astype(np.uint8)
+    for m in meshes:
+        codes = m["materials"]["textureIDsAndShadingModel"]
+        codes[::2] = 1 | (0 << 48) | (0xFFFF << 32) | (0xFFFF << 16)
+        m["verts"]["uv"] *= 5.0
+    _write(H, path, [(1, 32, 32, 40, padded.tobytes())], lights, meshes)
+    w, h = 320, 240
+    t, q = (8.0, 8.0, 19.0), (-0.3420201, 0.0, 0.0, 0.9396926)
+    r = subprocess.run([BIN, str(path), str(out), "size=%dx%d" % (w, h), "msaa=4",
+                        "camera=" + ",".join(str(x) for x in (*t, *q))], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(out, np.uint32).reshape(h, w)
+    scene = H.HostScene.from_bin(path)
+    view = H.HostView.from_samples(H.grid_samples(w, h), w, h, (-w / h, -1.0, w / h, 1.0))
+    m = H.transform_matrix(t, q, 1.0)
+    want = O.render_view(scene, view, 4, camera_to_world=m, textures=[("srgb8", padded[:, :32])])["image"]
+    plain = O.render_view(scene, view, 4, camera_to_world=m)["image"]
+    assert (want.reshape(h, w) != plain.reshape(h, w)).mean() > 0.1  # the texture shows
+    assert rgba_delta(img, want.reshape(h, w)).max() <= 1
