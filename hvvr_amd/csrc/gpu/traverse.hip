@@ -538,17 +538,22 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             const uint32_t nInner = uint32_t(__builtin_popcount(inner));
             // emitting records keep their mask; the others keep it while a child is live
             bool open = valid && mask != 0u && (mask & leaf) == 0u;
+            // only a record whose children start inside this chunk can change in a later round
+            const bool local = open && first < uint32_t(b) + 64u;
             for (uint32_t round = 0; round < 64u; round++) {
-                bool live = false;
-                for (uint32_t j = 0; j < nInner; j++)
-                    live = live || metaMask(sh.recMeta[first + j]) != 0u;
-                const bool drop = open && !live;
+                // the children's metas in one LDS round trip (no short-circuit chain)
+                uint32_t any = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++)
+                    if (open && j < nInner)
+                        any |= sh.recMeta[first + j];
+                const bool drop = open && metaMask(any) == 0u;
                 waveLdsSync(); // every lane has read its children before any mask changes
                 if (drop)
                     sh.recMeta[r] = meta & ~(0xfu << 20);
                 waveLdsSync();
                 open = open && !drop;
-                if (ballot(drop) == 0ull)
+                if (ballot(drop) == 0ull || ballot(open && local) == 0ull)
                     break;
             }
         }
