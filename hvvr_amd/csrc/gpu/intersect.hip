@@ -125,6 +125,121 @@ __device__ inline bool frustaUVWPass(const SimpleRayFrustum& frustum, const Prec
     return !(uMax < 0 || uMin > 1 || vMax < 0 || vMin > 1 || wMax < 0 || wMin > 1);
 }
 
+// ---------------------------------------------------------------------------------
+// Occlusion skip (pinhole path). The lists arrive roughly front to back (the tile walk's
+// distance order), so after the first triangles most of a wave's subsamples already hold a
+// near hit. A survivor is skipped for a whole wave when no subsample ray of the wave can
+// pass its depth test, i.e. when the test's computed `depth * denom - t` is negative for
+// every subsample. This changes no result (the test would have rejected it), only the
+// work. Two sufficient conditions, with `far` the wave's largest current depth:
+//  - plane: t > far * (upper bound of every computed denom over the wave): then
+//    depth * denom <= far * denom < t for each subsample (denom < 0 gives depth * denom <= 0);
+//  - vertices: a ray the test accepts hits the triangle's plane at a point whose
+//    barycentrics are at least -E / den (E: bound on the rounding of the computed u, v, w;
+//    den: lower bound of the true denominators over the wave), so its ray parameter is at
+//    least (min over vertices of (v - origin) . f - (sum E / den) * edge spread) / max (D . f)
+//    for any direction f with D . f > 0; when that exceeds far (with the rounding of t and
+//    denom), the depth test fails.
+// The ray of subsample a is D = -(ndir + a.x ndu + a.y ndv) in exact arithmetic on the
+// lane's computed vectors; the bounds use L1 norms (never below the Euclidean ones) and
+// leave a relative margin of 1e-5 (more than a hundred roundings) on every quantity, so
+// each holds for the computed values the test compares, not only for exact geometry.
+struct WaveBundle {
+    vector3 dc, f;      // lane 0's centre ray, and it normalised
+    float r;            // bound on |D - dc| (L1) over the wave's subsample rays
+    float dsum;         // bound on |ndir| + |a.x||ndu| + |a.y||ndv| (L1)
+    float dcL1;
+    float dfMax, dfMin; // bounds on D . f
+};
+
+constexpr float kOccSlack = 1e-5f;
+constexpr float kUlp = 0x1p-24f;
+
+__device__ inline float l1(const vector3& v) { return fabsf(v.x) + fabsf(v.y) + fabsf(v.z); }
+
+__device__ inline float waveMaxF(float v) {
+#pragma unroll
+    for (int off = 32; off; off >>= 1)
+        v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// every lane of the wave active
+template <int A>
+__device__ inline WaveBundle waveBundle(const vector3& ndir, const vector3& ndu, const vector3& ndv,
+                                        const vector2* offsets) {
+    WaveBundle w;
+    w.dc = vector3(-bitsf(__builtin_amdgcn_readfirstlane(fbits(ndir.x))),
+                   -bitsf(__builtin_amdgcn_readfirstlane(fbits(ndir.y))),
+                   -bitsf(__builtin_amdgcn_readfirstlane(fbits(ndir.z))));
+    const float len = sqrtf(dot(w.dc, w.dc));
+    w.f = vector3(w.dc.x / len, w.dc.y / len, w.dc.z / len);
+    const float l1dir = l1(ndir), l1du = l1(ndu), l1dv = l1(ndv);
+    float r = 0.0f, dsum = 0.0f, dmax = -FLT_MAX, dmin = FLT_MAX;
+#pragma unroll
+    for (int i = 0; i < A; i++) {
+        const vector2 a = offsets[i];
+        const vector3 D = -(ndir + ndu * a.x + ndv * a.y);
+        dsum = fmaxf(dsum, l1dir + fabsf(a.x) * l1du + fabsf(a.y) * l1dv);
+        r = fmaxf(r, l1(D - w.dc));
+        const float df = dot(D, w.f);
+        dmax = fmaxf(dmax, df);
+        dmin = fminf(dmin, df);
+    }
+    w.dsum = waveMaxF(dsum) * (1.0f + kOccSlack);
+    w.r = (waveMaxF(r) + 8.0f * kUlp * w.dsum) * (1.0f + kOccSlack);
+    dmax = waveMaxF(dmax);
+    dmin = -waveMaxF(-dmin);
+    const float eDf = 16.0f * kUlp * w.dsum; // the dot's rounding and D's own
+    w.dfMax = dmax + eDf + kOccSlack * fabsf(dmax);
+    w.dfMin = dmin - eDf - kOccSlack * fabsf(dmin);
+    w.dcL1 = l1(w.dc) * (1.0f + kOccSlack);
+    return w;
+}
+
+// true when the triangle's depth test fails for every subsample of the wave
+__device__ inline bool occludedForWave(const TriTile& c, const WaveBundle& w, float far) {
+    const float nL1 = l1(c.normal) * (1.0f + kOccSlack);
+    const float denC = -dot(w.dc, c.normal);
+    // computed denom vs its exact value, and denC's own rounding
+    const float eDen = 64.0f * kUlp * (w.dsum + w.dcL1) * nL1 + kOccSlack * fabsf(denC);
+    const float spr = w.r * nL1 * (1.0f + kOccSlack);
+    // plane bound
+    const float hiDen = denC + spr + eDen;
+    const float denMax = hiDen + kOccSlack * fabsf(hiDen);
+    if (c.t > 0.0f) {
+        const float lim = far * denMax;
+        if (c.t > lim + kOccSlack * fabsf(lim))
+            return true;
+    }
+    // vertex bound
+    const float loDen = denC - spr - eDen;
+    const float denMin = loDen - kOccSlack * fabsf(loDen);
+    if (!(denMin > 0.0f) || !(w.dfMin > 0.0f))
+        return false;
+    const float ovL1 = l1(c.v0ToRayOrigin), e0L1 = l1(c.e0), e1L1 = l1(c.e1);
+    // The test's u, v, w use the edges and its denom and t the computed normal; for a
+    // triangle whose normal is not small against its edges (not a sliver) the computed
+    // normal's rounding (at most 4 ulp of |e0| |e1| per component) stays within 32 ulp of
+    // it, which the error terms here cover.
+    if (!(8.0f * nL1 >= e0L1 * e1L1))
+        return false;
+    const float spread = (e0L1 + e1L1) * (1.0f + kOccSlack);
+    const float g0 = -dot(c.v0ToRayOrigin, w.f);
+    const float g1 = g0 + dot(c.e0, w.f), g2 = g0 + dot(c.e1, w.f);
+    const float gmin = fminf(g0, fminf(g1, g2)) - 16.0f * kUlp * (ovL1 + spread) - kOccSlack * fabsf(g0);
+    // rounding of the computed u, v, w (cross and dot products, the fma chains, u's subtractions)
+    const float eBary = 128.0f * kUlp * w.dsum * (ovL1 * fmaxf(e0L1, e1L1) + nL1);
+    const float num = gmin - (eBary / denMin) * spread * (1.0f + kOccSlack);
+    if (!(num > 0.0f))
+        return false;
+    const float sLow = num / w.dfMax * (1.0f - kOccSlack);
+    const float eT = 64.0f * kUlp * ovL1 * nL1; // rounding of t, and of the normal in it
+    const float rhs = (sLow - eT / denMin) * (1.0f - kOccSlack);
+    const float lhs = far * (1.0f + eDen / denMin) * (1.0f + kOccSlack);
+    return rhs > 0.0f && lhs < rhs;
+}
+
 template <int A, bool DoF>
 __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
     using Cached = typename std::conditional<DoF, TriTileDoF, TriTile>::type;
@@ -174,6 +289,14 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
             planes[n][2] = dir.z;
             planes[n][3] = dot(fr.origins[n], dir);
         }
+    }
+
+    // each wave's ray bundle for the occlusion skip, kept in LDS (read every 16 survivors)
+    __shared__ WaveBundle bundles[2];
+    if constexpr (!DoF) {
+        const WaveBundle b = waveBundle<A>(ndir, ndu, ndv, args.offsets);
+        if (lane == 0)
+            bundles[wave] = b;
     }
 
     float depth[A];
@@ -253,7 +376,21 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
                 }
             }
         } else
-        for (uint32_t j = 0; j < total; j++) {
+        for (uint32_t j0 = 0; j0 < total; j0 += 16) {
+          // the wave's farthest current hit, and the next 16 survivors it cannot reach
+          float far = depth[0];
+#pragma unroll
+          for (int i = 1; i < A; i++)
+              far = fmaxf(far, depth[i]);
+          far = waveMaxF(far);
+          const uint32_t n = total - j0 < 16u ? total - j0 : 16u;
+          uint32_t todo = (1u << n) - 1u;
+          if (far < FLT_MAX) {
+              const bool skip = lane < n && occludedForWave(cache[j0 + lane], bundles[wave], far);
+              todo &= ~uint32_t(__ballot(skip));
+          }
+          for (; todo; todo &= todo - 1u) {
+            const uint32_t j = j0 + uint32_t(__builtin_ctz(todo));
             const TriTile& c = cache[j];
             const uint32_t idx = cacheIndex[j];
             // IntersectTriangleThread ctor + GetDifferentials (prim_tests.h:118-134, 175-188)
@@ -283,6 +420,7 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
                     hit[i] = idx;
                 }
             }
+          }
         }
         __syncthreads(); // before the next chunk overwrites the cache
     }
