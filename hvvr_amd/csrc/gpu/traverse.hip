@@ -240,8 +240,8 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // of anything), so a tile with one, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
-// Four configurations share the code. The standard one (704 records, a one-register
-// 64-entry stack, 9.6 KB of LDS) runs every tile; a tile it cannot hold is queued for
+// Four configurations share the code. The standard one (680 records, a one-register
+// 64-entry stack, 8 KB of LDS) runs every tile; a tile it cannot hold is queued for
 // the large one (2048 records, a four-register 256-entry stack, 16 KB of LDS), what that
 // cannot hold for the huge one (4608 records, eight registers, 512 entries, 36 KB), and
 // so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB); each
@@ -265,7 +265,7 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 // reuse it tile after tile), which halves their LDS and doubles the workgroups a CU holds
 // (the walk is latency-bound: throughput follows the resident waves).
 struct TileListSmall {
-    enum : uint32_t { RecCap = 704, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
+    enum : uint32_t { RecCap = 680, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
     using Stack = RegStack;
 };
 struct TileListLarge {
@@ -287,10 +287,11 @@ constexpr uint32_t recScratchWords() {
     return Cfg::LdsRecords ? 0u : ((Cfg::RecCap + Cfg::RecCap / 2 + 63u) & ~63u);
 }
 
+// (the standard configuration keeps its emission order in bits 10..19 of recMeta, which
+// its stored metas leave free: entry i of the order in recMeta[i])
 template <class Cfg, bool InLds = bool(Cfg::LdsRecords)>
 struct TileListLdsRecords {
     uint32_t recNode[Cfg::RecCap];
-    uint16_t order[Cfg::RecCap]; // records with emitted leaves, in emission order
 };
 template <class Cfg>
 struct TileListLdsRecords<Cfg, false> {};
@@ -310,7 +311,7 @@ __device__ inline uint32_t metaFirst(uint32_t m) {
 // stack entry of record r with meta m, and back
 template <class Cfg>
 __device__ inline uint32_t stackEntry(uint32_t r, uint32_t m) {
-    return Cfg::Packed ? (m | (r << 10)) : r;
+    return Cfg::Packed ? ((m & ~(0x3ffu << 10)) | (r << 10)) : r;
 }
 template <class Cfg>
 __device__ inline void fromStackEntry(uint32_t e, const TileListLds<Cfg>& sh, uint32_t& rec, uint32_t& meta) {
@@ -398,11 +399,18 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         else
             return gNode[i];
     };
-    auto order = [&](uint32_t i) -> uint16_t& {
+    // the emission order: entry i is the i-th record whose leaves are emitted
+    auto order = [&](uint32_t i) -> uint32_t {
         if constexpr (Cfg::LdsRecords)
-            return sh.order[i];
+            return (sh.recMeta[i] >> 10) & 0x3ffu;
         else
             return gOrder[i];
+    };
+    auto setOrder = [&](uint32_t i, uint32_t rec) {
+        if constexpr (Cfg::LdsRecords)
+            sh.recMeta[i] = (sh.recMeta[i] & ~(0x3ffu << 10)) | (rec << 10);
+        else
+            gOrder[i] = uint16_t(rec);
     };
     const uint32_t limit = g.stackLimit < uint32_t(Cfg::StackCap) ? g.stackLimit : uint32_t(Cfg::StackCap);
     bool overflow = n0 > limit;
@@ -598,7 +606,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
                 emitBuf = writeLane(emitBuf, curRec, nEmit & 63u, lane);
                 nEmit++;
                 if ((nEmit & 63u) == 0u)
-                    order(nEmit - 64u + lane) = uint16_t(emitBuf);
+                    setOrder(nEmit - 64u + lane, emitBuf);
             }
             // internal children are followed only from entries without a leaf mask
             const uint32_t inner = metaEntry(curMeta) == 0u ? mask & ~leafMask : 0u;
@@ -674,7 +682,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             }
         }
         if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
-            order((nEmit & ~63u) + lane) = uint16_t(emitBuf);
+            setOrder((nEmit & ~63u) + lane, emitBuf);
         waveLdsSync();
     }
     if (overflow) {
@@ -790,9 +798,11 @@ __device__ inline void tileListSmallOne(const TileListArgs& g, TileListLds<TileL
     }
 }
 
-// The LDS records (9856 bytes) let a CU hold 16 of these waves. A record capacity below
-// what a scene's densest tiles need costs more than the waves it adds: the tiles it sends
-// to the large configuration run after this kernel, each a serial walk of one wave.
+// The LDS records (680 x 12 bytes, 8160 bytes: the emission order shares the metas' words)
+// let a CU hold 20 of these waves. A record capacity below what a scene's densest tiles
+// need costs more than the waves it adds: the tiles it sends to the large configuration
+// run after this kernel, each a serial walk of one wave (the densest tile of the stereo
+// room eye holds 652 records).
 __global__ __launch_bounds__(64) void tileListKernel(TileListArgs g) {
     __shared__ TileListLds<TileListSmall> sh;
     tileListSmallOne(g, sh, blockIdx.x, threadIdx.x);
