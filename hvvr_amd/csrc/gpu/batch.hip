@@ -83,9 +83,14 @@ struct hvvr_gpu_batch {
     bool graphEnabled = false, refitEveryFrame = false, inFlight = false;
     std::vector<std::unique_ptr<Chain>> chains;
     std::vector<hipStream_t> extraStreams; // chain streams beyond the batch's own
-    std::vector<uint64_t> signature;      // what the chains and a captured graph were built for
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> signature;      // what the chains and the captured graphs were built for
+    // One graph per chain, replayed on the chain's stream (the chains run side by side;
+    // a single graph holding them all ran its branches one after the other), and one for
+    // the refit that precedes them.
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    hipGraph_t refitGraph = nullptr;
+    hipGraphExec_t refitExec = nullptr;
     std::vector<uint64_t> graphSignature;
 
     ~hvvr_gpu_batch() {
@@ -110,12 +115,20 @@ struct hvvr_gpu_batch {
             (void)hipStreamDestroy(stream);
     }
     void dropGraph() {
-        if (exec)
-            (void)hipGraphExecDestroy(exec);
-        if (graph)
-            (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        for (hipGraphExec_t e : execs)
+            if (e)
+                (void)hipGraphExecDestroy(e);
+        for (hipGraph_t g : graphs)
+            if (g)
+                (void)hipGraphDestroy(g);
+        if (refitExec)
+            (void)hipGraphExecDestroy(refitExec);
+        if (refitGraph)
+            (void)hipGraphDestroy(refitGraph);
+        execs.clear();
+        graphs.clear();
+        refitExec = nullptr;
+        refitGraph = nullptr;
         graphSignature.clear();
     }
     void dropChains() {
@@ -322,21 +335,60 @@ void enqueueChain(const hvvr_gpu_batch* b, Chain& c, bool tex) {
                          uint32_t(c.remapMembers.size()), c.uRemap, st);
 }
 
-// the frame on the batch's stream (eagerly, or into a capture): the optional refit, then
-// the chains, forked onto their streams and joined back
-void enqueueFrame(hvvr_gpu_batch* b, hvvr_gpu_scene* s) {
-    if (b->refitEveryFrame)
-        launchSceneRefit(s, b->stream);
+// the frame on the batch's stream: the optional refit, then the chains, forked onto their
+// streams and joined back; `graphs`: each chain (and the refit) as its captured graph
+void enqueueFrame(hvvr_gpu_batch* b, hvvr_gpu_scene* s, bool graphs) {
+    if (b->refitEveryFrame) {
+        if (graphs)
+            (void)hipGraphLaunch(b->refitExec, b->stream);
+        else
+            launchSceneRefit(s, b->stream);
+    }
     if (b->chains.size() > 1)
         (void)hipEventRecord(b->fork, b->stream);
     for (size_t k = 1; k < b->chains.size(); k++)
         (void)hipStreamWaitEvent(b->chains[k]->stream, b->fork, 0);
-    for (auto& c : b->chains)
-        enqueueChain(b, *c, s->textureCount > 0);
+    for (size_t k = 0; k < b->chains.size(); k++) {
+        Chain& c = *b->chains[k];
+        if (graphs)
+            (void)hipGraphLaunch(b->execs[k], c.stream ? c.stream : b->stream);
+        else
+            enqueueChain(b, c, s->textureCount > 0);
+    }
     for (size_t k = 1; k < b->chains.size(); k++) {
         (void)hipEventRecord(b->chains[k]->joined, b->chains[k]->stream);
         (void)hipStreamWaitEvent(b->stream, b->chains[k]->joined, 0);
     }
+}
+
+// captures the refit and each chain as a graph of its own
+int captureGraphs(hvvr_gpu_batch* b, hvvr_gpu_scene* s) {
+    auto capture = [&](hipStream_t st, auto&& body, hipGraph_t& g, hipGraphExec_t& e) -> int {
+        BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        body();
+        hipError_t err = hipStreamEndCapture(st, &g);
+        if (err != hipSuccess)
+            return batchFail(std::string("graph capture: ") + hipGetErrorString(err), HVVR_GPU_ERR_HIP);
+        err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        if (err != hipSuccess)
+            return batchFail(std::string("graph instantiation: ") + hipGetErrorString(err), HVVR_GPU_ERR_HIP);
+        return HVVR_GPU_OK;
+    };
+    if (b->refitEveryFrame) {
+        int rc = capture(b->stream, [&] { launchSceneRefit(s, b->stream); }, b->refitGraph, b->refitExec);
+        if (rc)
+            return rc;
+    }
+    b->graphs.assign(b->chains.size(), nullptr);
+    b->execs.assign(b->chains.size(), nullptr);
+    for (size_t k = 0; k < b->chains.size(); k++) {
+        Chain& c = *b->chains[k];
+        hipStream_t st = c.stream ? c.stream : b->stream;
+        int rc = capture(st, [&] { enqueueChain(b, c, s->textureCount > 0); }, b->graphs[k], b->execs[k]);
+        if (rc)
+            return rc;
+    }
+    return HVVR_GPU_OK;
 }
 
 int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
@@ -375,25 +427,20 @@ int renderBatch(hvvr_gpu_batch* b, hvvr_gpu_scene* s, const float* m16) {
         fillArgs(*c, s, m.data());
     BATCH_HIP(hipEventRecord(b->start, b->stream));
     if (b->graphEnabled) {
-        if (!b->exec || b->graphSignature != sig) {
+        if (b->execs.empty() || b->graphSignature != sig) {
             b->dropGraph();
-            BATCH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            enqueueFrame(b, s);
-            hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
-            if (e != hipSuccess)
-                return batchFail(std::string("graph capture: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
-            e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
+            int rc = captureGraphs(b, s);
+            if (rc) {
                 b->dropGraph();
-                return batchFail(std::string("graph instantiation: ") + hipGetErrorString(e), HVVR_GPU_ERR_HIP);
+                return rc;
             }
             b->graphSignature = sig;
         }
-        BATCH_HIP(hipGraphLaunch(b->exec, b->stream));
+        enqueueFrame(b, s, true);
     } else {
-        enqueueFrame(b, s);
-        BATCH_HIP(hipGetLastError());
+        enqueueFrame(b, s, false);
     }
+    BATCH_HIP(hipGetLastError());
     BATCH_HIP(hipEventRecord(b->stop, b->stream));
     if (b->refitEveryFrame) {
         s->dirty = false;
