@@ -95,7 +95,10 @@ def measured_traffic(kernel):
     m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
     if not m:
         return None
-    entry = json.loads(m.group(1)).get(kernel)
+    table = json.loads(m.group(1))
+    entry = table.get(kernel)
+    if entry is None:  # the summary names template kernels in full: match the name before '<'
+        entry = next((v for k, v in table.items() if k.split("<")[0] == kernel.split("<")[0]), None)
     return None if entry is None else int(entry["read_bytes"] + entry["write_bytes"])
 
 
@@ -136,7 +139,8 @@ def roofline(stage_ms, survivors, entries, visits):
                 "valu_lane_util_pct": pmc.get("valu_util"),
                 "note": f"FP32 VALU work (no matrix cores on this path) against the 157.3 TFLOP/s FP32 vector "
                         f"figure (78.6 without packed FP32, which the build disables); F = N_pair*(80+16A), "
-                        f"N_pair={int(survivors)} per launch"}
+                        f"N_pair={int(survivors)} per launch: every tile-cull survivor pair the reference tests "
+                        f"(the kernel skips those a wave's rays cannot reach in front of their current hits)"}
     bytes_ = 128 * visits + 4 * entries
     name = {"tile_lists": "tileListKernel", "resolve": "resolveKernel<4>"}.get(dom, dom)
     traffic = measured_traffic(name)
