@@ -6,6 +6,10 @@
 // lives in lane q of (t, m); t = distance key, m = the entry's packed record meta.
 // Capacity 64. A pop reads the top lane; an insert shifts the lanes above the insertion
 // point up by one. Every operation is wave-uniform: all 64 lanes must be active.
+// The lanes at and above `size` hold t = -inf (stackInit sets them, a pop restores its
+// lane), so with finite keys an insert's search needs no mask of the live lanes: the
+// entries whose key is not smaller than the new one are the bottom run, counted by one
+// ballot and a popcount.
 
 #include "wave_traverse.h"
 
@@ -47,8 +51,9 @@ __device__ inline uint32_t laneShiftUp(uint32_t v, uint32_t fill) {
 }
 
 // removes the top entry and returns its meta (size > 0)
-__device__ inline uint32_t stackPop(RegStack& st) {
+__device__ inline uint32_t stackPop(RegStack& st, uint32_t lane) {
     st.size--;
+    st.t = lane == st.size ? -INFINITY : st.t;
     return readLane(st.m, st.size);
 }
 
@@ -58,10 +63,11 @@ __device__ inline float stackTopKey(const RegStack& st) {
 
 // Inserts (key, meta) one above the highest entry whose key is not smaller, so the new
 // entry lands below the run of closer entries at the top (traverse_avx.h:505-515; the
-// sentinel below the bottom is never smaller). The caller keeps size < 64.
+// sentinel below the bottom is never smaller). The caller keeps size < 64, the keys
+// finite and the stack sorted (bottom to top, keys not increasing): the entries whose
+// key is not smaller are then positions 0 .. p-1, and the -inf lanes are never counted.
 __device__ inline void stackInsert(RegStack& st, float key, uint32_t meta, uint32_t lane) {
-    const uint64_t b = ballot(!(st.t < key)) & lanesBelowMask(st.size);
-    const uint32_t p = b ? 64u - uint32_t(__builtin_clzll(b)) : 0u;
+    const uint32_t p = uint32_t(__builtin_popcountll(ballot(!(st.t < key))));
     const float ut = bitsf(laneShiftUp(fbits(st.t), 0u));
     const uint32_t um = laneShiftUp(st.m, 0u);
     st.t = lane < p ? st.t : (lane == p ? key : ut);
@@ -117,6 +123,11 @@ __device__ inline uint32_t stackPop(RegStackN<N>& st) {
     stackRead(st, st.size - 1u, key, meta);
     st.size--;
     return meta;
+}
+
+template <int N>
+__device__ inline uint32_t stackPop(RegStackN<N>& st, uint32_t) {
+    return stackPop(st);
 }
 
 template <int N>

@@ -236,8 +236,9 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // stack arrives sorted and every insert lands at its sorted position), so a pop always
 // takes the least (key, newest) entry; an entry that changes nothing when popped can be
 // dropped without moving any other pop. The same holds for a record none of whose
-// subtree emits (a pass between phases A and B marks those dead too). A NaN key breaks that order (it inserts on top
-// of anything), so a tile with one, or with an unsorted block stack, goes to the
+// subtree emits (a pass between phases A and B marks those dead too). A NaN key breaks
+// that order (it inserts on top of anything), and the register stack's empty lanes hold
+// -inf, so a tile with a NaN or infinite key, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
 // Four configurations share the code. The standard one (680 records, a one-register
@@ -325,8 +326,8 @@ __device__ inline void fromStackEntry(uint32_t e, const TileListLds<Cfg>& sh, ui
 }
 
 
-__device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size, uint32_t) {
-    st.t = t;
+__device__ inline void stackInit(RegStack& st, float t, uint32_t m, uint32_t size, uint32_t lane) {
+    st.t = lane < size ? t : -INFINITY;
     st.m = m;
     st.size = size;
 }
@@ -427,7 +428,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             recNode(lane) = e.nodeMask & STACK_NODE_MASK;
             sh.recKey[lane] = fbits(e.tMin);
             sh.recMeta[lane] = packMeta(0u, 0u, 0u, e.nodeMask >> 28);
-            nanKey = e.tMin != e.tMin;
+            nanKey = !(fabsf(e.tMin) <= FLT_MAX); // NaN or infinite
         }
         overflow = ballot(nanKey) != 0ull;
         waveLdsSync();
@@ -519,7 +520,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             nrec += cntA + uint32_t(__builtin_popcountll(biB));
             head = end;
             waveLdsSync();
-            if (nrec > Cfg::RecCap || ballot((innerA && A.key != A.key) || (innerB && B.key != B.key)) != 0ull) {
+            if (nrec > Cfg::RecCap ||
+                ballot((innerA && !(fabsf(A.key) <= FLT_MAX)) || (innerB && !(fabsf(B.key) <= FLT_MAX))) != 0ull) {
                 overflow = true;
                 break;
             }
@@ -593,7 +595,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         uint32_t curRec = 0, curMeta = 0;
         bool walking = !overflow && st.size != 0u;
         if (walking)
-            fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
+            fromStackEntry<Cfg>(stackPop(st, lane), sh, curRec, curMeta);
         while (walking) {
             // every record is reached at most once; the bound keeps the loop finite whatever
             // the records hold
@@ -678,7 +680,7 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             if (pop) {
                 if (st.size == 0u)
                     break;
-                fromStackEntry<Cfg>(stackPop(st), sh, curRec, curMeta);
+                fromStackEntry<Cfg>(stackPop(st, lane), sh, curRec, curMeta);
             }
         }
         if (!overflow && (nEmit & 63u) != 0u && lane < (nEmit & 63u))
@@ -857,8 +859,8 @@ __global__ __launch_bounds__(64) void tileListQueueBatchKernel(const TileListArg
 
 // ---------------------------------------------------------------------------------
 // The reference's tile walk as written (traverseTiles, traverse_avx.h:412-527), one wave
-// per tile, for the tiles no LDS configuration holds and those with a NaN key or an
-// unsorted block stack. The walk is wave-uniform (one node per step, tested by every
+// per tile, for the tiles no LDS configuration holds and those with a NaN or infinite key
+// or an unsorted block stack. The walk is wave-uniform (one node per step, tested by every
 // lane alike); the distance-sorted stack, up to TILE_STACK_SIZE entries, lives in LDS and
 // the wave inserts into it together (LdsStack: the sequential insertion's position, one
 // ballot per 64 entries, then a 64-wide move). The current node's entry stays on top

@@ -13,7 +13,7 @@ namespace {
 using namespace hvvr;
 
 __device__ void probeInit(RegStack& st) {
-    st.t = 0.0f;
+    st.t = -INFINITY; // the empty lanes' key
     st.m = 0u;
     st.size = 0;
 }
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) void regStackProbe(const uint32_t* ops, const f
         if (op == 0u) {
             if (st.size == 0u)
                 continue;
-            const uint32_t rec = stackPop(st);
+            const uint32_t rec = stackPop(st, lane);
             if (lane == 0)
                 popped[i] = rec;
         } else if (st.size < Cap) {
