@@ -80,6 +80,7 @@ struct TileWalkArgs {
     uint32_t* tileLevel;
     uint32_t* tileWork; // as in TileListArgs
     uint32_t tileCount;
+    uint32_t stackLimit; // the view's register-stack limit (sets the configurations' capacity)
 };
 
 struct IntersectArgs {

@@ -896,6 +896,7 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
         waveLdsSync(); // the previous tile's LDS reads are done
         LdsStack st;
         stackBindLds(st, sh.t, sh.m);
+        bool irregular = false;
         {
             const StackEntry* bs = a.blockStacks + size_t(block) * BLOCK_STACK_SIZE;
             float t0 = 0.0f;
@@ -904,6 +905,10 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
                 t0 = bs[lane].tMin;
                 m0 = bs[lane].nodeMask;
             }
+            // what sends a tile here whatever its size: a NaN or infinite key, an unsorted
+            // block stack (the walk's own keys are checked as it pushes them)
+            const float above = __shfl_down(t0, 1);
+            irregular = ballot((lane < n0 && !(fabsf(t0) <= FLT_MAX)) || (lane + 1u < n0 && t0 < above)) != 0ull;
             stackInit(st, t0, m0, n0, lane);
         }
         const uint64_t outBase = emit ? a.offsets[tile] : 0;
@@ -952,8 +957,10 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
                     }
                     st.size--; // the children take the current entry's place
                     for (int k = 0; k < 4; k++)
-                        if (mask & (1u << k))
+                        if (mask & (1u << k)) {
+                            irregular |= !(fabsf(n.bounds[axis][k]) <= FLT_MAX);
                             stackInsert(st, -n.bounds[axis][k], (node + n.data[1 + k]) & STACK_NODE_MASK, lane);
+                        }
                     maxSize = st.size > maxSize ? st.size : maxSize;
                     // the closest entry may be an older one (possibly a block-level leaf entry)
                     topEntry = stackAt(st, st.size - 1u);
@@ -980,7 +987,19 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
                 a.offsets[tile] = uint32_t(start);
                 if (start + count > capacity)
                     atomicOr(&counters[1], 1ull);
-                a.tileLevel[tile] = 4u;
+                // the smallest queued configuration whose records and stack would hold this
+                // walk (the walk's stack is the unpruned one, never shallower), unless the
+                // tile can only be walked
+                // (with a view stack limit below 64 every configuration has that limit)
+                const bool full = a.stackLimit >= REG_STACK_CAP;
+                const uint32_t capL = full ? REG_STACK4_CAP : a.stackLimit;
+                const uint32_t capH = full ? REG_STACK8_CAP : a.stackLimit;
+                const uint32_t capG = full ? REG_STACK16_CAP : a.stackLimit;
+                a.tileLevel[tile] = irregular                                             ? 4u
+                                    : visits <= TileListLarge::RecCap && maxSize <= capL  ? 1u
+                                    : visits <= TileListHuge::RecCap && maxSize <= capH   ? 2u
+                                    : visits <= TileListGiant::RecCap && maxSize <= capG  ? 3u
+                                                                                          : 4u;
                 a.tileWork[tile] = visits; // node visits
                 a.tileWork[a.tileCount + tile] = maxSize;
             }
@@ -1054,6 +1073,7 @@ TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
     a.tileLevel = v->tileLevel.ptr;
     a.tileWork = v->tileWork.ptr;
     a.tileCount = v->tileCount;
+    a.stackLimit = v->waveStackLimit;
     return a;
 }
 

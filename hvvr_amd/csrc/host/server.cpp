@@ -237,9 +237,24 @@ struct hvvr_server {
         return rc ? gpuFail("render", rc) : HVVR_GPU_OK;
     }
 
-    int growLists(PooledView* pv, uint64_t entries) {
+    // Grows a view's list storage to `entries` (4 bytes each) within the budget, evicting
+    // idle views of any shape first. Returns 1 (nothing changed) when even then the growth
+    // does not fit.
+    int growLists(PooledView* pv, uint64_t entries, uint64_t capacity) {
         Shape& sh = shapes[pv->shape];
-        entries = std::min<uint64_t>(entries, 0xffffffffull);
+        const uint64_t growth = entries > capacity ? (entries - capacity) * 4u : 0u;
+        while (used + growth > budget) {
+            PooledView* victim = nullptr;
+            for (PooledView* v : pool)
+                if (!v->busy && v != pv && (!victim || v->lastUsed < victim->lastUsed))
+                    victim = v;
+            if (!victim)
+                return 1;
+            used -= victim->bytes;
+            pool.erase(std::find(pool.begin(), pool.end(), victim));
+            release(victim);
+            viewsEvicted++;
+        }
         int rc = hvvr_gpu_view_reserve_lists(pv->view, entries);
         uint64_t bytes = 0;
         if (!rc)
@@ -275,23 +290,35 @@ struct hvvr_server {
             int rc = hvvr_gpu_view_list_demand(r->pv->view, &need, &cap, &overflowed);
             if (rc)
                 return gpuFail("list demand", rc);
+            const char* failure = nullptr;
             if (overflowed && need > cap && r->launches == 1) {
                 // the lists did not fit: grow the view's storage (and that of the shape's
-                // new views) and render the frame again
-                rc = growLists(r->pv, need + need / 4);
-                if (rc)
-                    return rc;
-                rc = launch(r);
-                if (rc)
-                    return rc;
-                i++;
-                continue;
+                // new views) and render the frame again; list indices are 32-bit
+                const uint64_t want = need + need / 4;
+                if (need > 0xffffffffull) {
+                    failure = "the frame's triangle lists exceed 2^32 - 1 entries";
+                } else {
+                    rc = growLists(r->pv, std::min<uint64_t>(want, 0xffffffffull), cap);
+                    if (rc < 0)
+                        return rc;
+                    if (rc == 0) {
+                        rc = launch(r);
+                        if (rc)
+                            return rc;
+                        i++;
+                        continue;
+                    }
+                    failure = "the frame's triangle lists exceed the server's device budget";
+                }
             }
             r->completed = nowMs();
             r->pv->busy = false;
             r->pv->lastUsed = r->completed;
-            if (overflowed) {
-                failed[r->id] = "the frame overflowed a tile's traversal stack";
+            if (failure) {
+                failed[r->id] = failure;
+            } else if (overflowed) {
+                failed[r->id] = need > cap ? "the frame's triangle lists overflowed their storage again"
+                                           : "the frame overflowed a tile's traversal stack";
             } else {
                 memcpy(r->image, r->pv->staging, size_t(sh.width) * sh.height * 4);
                 double lat = r->completed - r->submitted;

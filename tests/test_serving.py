@@ -247,3 +247,45 @@ def test_device_grid_shapes_serve_like_host_shapes(H, ctx, scene2):
     assert srv.shape_bytes(b) == ctypes_bytes(H, gv)
     gv.close()
     srv.close()
+
+
+@pytest.mark.gpu
+def test_list_growth_stays_within_the_budget(H, ctx):
+    # The 1M-triangle room at 360x400 outgrows a new view's list storage. With no room in
+    # the budget for the growth the request fails with a budget error; with room once an
+    # idle view of another shape is evicted, it grows, renders, and equals a plain view.
+    dense = H.GpuScene(ctx, H.HostScene(H.SCENE_ROOM, 0, 4096))
+    view = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 360, 400)
+    small = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 64, 64)
+    m = view.camera_to_world
+    gv = H.GpuView(ctx, view, 1)
+    gv.render(dense)
+    need, cap, overflowed = gv.list_demand()
+    assert overflowed and need > cap
+    growth = ((need + need // 4) - cap) * 4
+    probe = H.Server(ctx, 1 << 40, 1)
+    big, lit = probe.shape_bytes(probe.add_shape(view, 1)), probe.shape_bytes(probe.add_shape(small, 1))
+    probe.close()
+
+    srv = H.Server(ctx, big + growth // 2, 1)
+    rid, _ = srv.submit(dense, srv.add_shape(view, 1), m)
+    srv.drain()
+    with pytest.raises(H.HvvrError, match="budget"):
+        srv.done(rid)
+    assert srv.stats().peakDeviceBytes <= big + growth // 2
+    srv.close()
+
+    budget = big + growth + lit // 2
+    srv = H.Server(ctx, budget, 1)
+    s_small, s_big = srv.add_shape(small, 1), srv.add_shape(view, 1)
+    r0, _ = srv.submit(dense, s_small, m)
+    srv.drain()
+    r1, img = srv.submit(dense, s_big, m)
+    srv.drain()
+    assert srv.done(r0)[0] and srv.done(r1)[0]
+    st = srv.stats()
+    assert st.viewsEvicted >= 1 and st.peakDeviceBytes <= budget
+    srv.close()
+    gv.reserve_lists(need)
+    gv.render(dense)
+    assert np.array_equal(img, gv.read_image())
