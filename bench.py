@@ -118,19 +118,20 @@ def measured_counters(kernel):
     return {}
 
 
-def roofline(stage_ms, survivors, entries, visits):
+def roofline(stage_ms, survivors, entries, visits, tested):
     """Roofline of the dominant kernel, per launch (one launch per eye).
     tile_lists: algorithmic bytes = 128 B per BVH node visit + 4 B per list entry written,
     against HBM; the node reads mostly hit L2, so beside it stand the HBM bytes the
     counters measured (`traffic`, its fraction of 8 TB/s) and the L2 hit rate: the kernel
     is bound by the issue and dependency latency of its stack walk, not by either;
-    intersect: FP32 VALU flops = N_pair (80 + 16 A), N_pair = tile-cull survivors x 128,
+    intersect: FP32 VALU flops = N_pair (80 + 16 A) with N_pair the (sample, survivor) pairs
+    the kernel tested (the tile-cull survivors less those its occlusion skip left out),
     against the FP32 vector figure."""
     dom = STAGES[int(np.argmax(stage_ms))]
     t = stage_ms[STAGES.index(dom)] * 1e-3
     if dom == "intersect":
         name = "intersectKernel<4>"
-        flops = survivors * (80 + 16 * A)
+        flops = tested * (80 + 16 * A)
         ach = flops / t
         pmc = measured_counters(name)
         return {"kernel": name, "bound": "valu", "achieved": round(ach / 1e12, 3),
@@ -139,8 +140,8 @@ def roofline(stage_ms, survivors, entries, visits):
                 "valu_lane_util_pct": pmc.get("valu_util"),
                 "note": f"FP32 VALU work (no matrix cores on this path) against the 157.3 TFLOP/s FP32 vector "
                         f"figure (78.6 without packed FP32, which the build disables); F = N_pair*(80+16A), "
-                        f"N_pair={int(survivors)} per launch: every tile-cull survivor pair the reference tests "
-                        f"(the kernel skips those a wave's rays cannot reach in front of their current hits)"}
+                        f"N_pair={int(tested)} (sample, survivor) pairs tested per launch of the {int(survivors)} "
+                        f"tile-cull survivor pairs (the rest a wave's rays cannot reach in front of their hits)"}
     bytes_ = 128 * visits + 4 * entries
     name = {"tile_lists": "tileListKernel", "resolve": "resolveKernel<4>"}.get(dom, dom)
     traffic = measured_traffic(name)
@@ -511,7 +512,8 @@ def main():
             raise SystemExit("traversal budget overflow")
 
     visits = sum(s.nodeVisits for s in stats) / EYES
-    roof = roofline(stage_ms, survivors, entries, visits)
+    tested = sum(s.testedPairs for s in stats) / EYES
+    roof = roofline(stage_ms, survivors, entries, visits, tested)
 
     out = {
         "metric": "Msamples/s shaded at 2x1080x1200 MSAA4, 262k tris; p99 frame latency (ms)",
@@ -534,6 +536,7 @@ def main():
         "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage_ms)},
         "list_entries_per_eye": int(entries),
         "survivor_pairs_per_eye": int(survivors),
+        "tested_pairs_per_eye": int(tested),
         "node_visits_per_eye": int(visits),
         "node_tests_per_s": round(visits * 4 / (stage_ms[STAGES.index("tile_lists")] * 1e-3), 1),
         "roofline": roof,

@@ -456,7 +456,7 @@ int hvvr::makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool devi
     step(v->tileQueues.resize(size_t(4) * v->tileCount));
     step(v->recScratch.resize(tileListScratchWords()));
     step(v->tileLevel.resize(v->tileCount));
-    step(v->tileWork.resize(size_t(3) * v->tileCount));
+    step(v->tileWork.resize(size_t(4) * v->tileCount));
     step(v->tileOffsets.resize(v->tileCount));
     v->listCapacity = std::max<uint64_t>(uint64_t(v->tileCount) * 1024, 1u << 20);
     step(v->triIndices.resize(v->listCapacity));
@@ -970,7 +970,7 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + 10 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
+                 B * 4 + 11 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
                  S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 16 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
@@ -1154,6 +1154,7 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     st->hugeTiles = uint32_t(counters[6]);
     st->giantTiles = uint32_t(counters[7]);
     st->walkTiles = uint32_t(counters[8]);
+    st->testedPairs = counters[9];
     st->tileCount = v->tileCount;
     for (uint32_t c : counts)
         st->occupiedTiles += c ? 1u : 0u;

@@ -309,6 +309,8 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
     __syncthreads();
 
     uint32_t survivorsSum = 0;
+    // survivors this wave skipped (wave-uniform: a scalar register)
+    uint32_t skippedSum = 0;
     for (uint32_t chunk = 0; chunk < listCount; chunk += 128) {
         // ---- cull one list entry per thread ----
         bool keep = false;
@@ -387,7 +389,9 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
           uint32_t todo = (1u << n) - 1u;
           if (far < FLT_MAX) {
               const bool skip = lane < n && occludedForWave(cache[j0 + lane], bundles[wave], far);
-              todo &= ~uint32_t(__ballot(skip));
+              const uint32_t skipMask = uint32_t(__ballot(skip));
+              todo &= ~skipMask;
+              skippedSum += uint32_t(__builtin_popcount(skipMask));
           }
           for (; todo; todo &= todo - 1u) {
             const uint32_t j = j0 + uint32_t(__builtin_ctz(todo));
@@ -424,8 +428,13 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
         }
         __syncthreads(); // before the next chunk overwrites the cache
     }
-    if (tid == 0)
+    if (lane == 0)
+        waveSurvivors[wave] = skippedSum; // the loop's last read of it preceded a barrier
+    __syncthreads();
+    if (tid == 0) {
         args.tileSurvivors[tile] = survivorsSum;
+        args.tileTested[tile] = 2u * survivorsSum - waveSurvivors[0] - waveSurvivors[1];
+    }
 
     if (args.hits) {
         // per-subsample hit output: the nearest hit's ray parameter as the test left it,
@@ -458,14 +467,22 @@ __device__ inline void intersectBody(const IntersectArgs& args, uint32_t tile) {
     emitGBuffer<A>(hit, waveBase, lane);
 }
 
+// Waves per SIMD the register allocator is held to. Left alone, it gives the 4x and 8x
+// pinhole kernels 98 and 106 VGPRs (4 waves); held to 6 and 5 waves they take 77 and 89
+// with no spills, and the 4x kernel runs 9% faster. Elsewhere (1) the natural allocation
+// already avoids spills and a tighter one would not.
+constexpr int intersectWaves(int A, bool DoF) { return DoF ? 1 : A == 4 ? 6 : A == 8 ? 5 : 1; }
+
 template <int A, bool DoF>
-__global__ __launch_bounds__(128) void intersectKernel(IntersectArgs args) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(intersectWaves(A, DoF))))
+void intersectKernel(IntersectArgs args) {
     intersectBody<A, DoF>(args, blockIdx.x);
 }
 
 // every tile of the views whose units `start` lists (pinhole views of one MSAA rate)
 template <int A>
-__global__ __launch_bounds__(128) void intersectBatchKernel(const IntersectArgs* __restrict__ all,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(intersectWaves(A, false))))
+void intersectBatchKernel(const IntersectArgs* __restrict__ all,
                                                             const uint32_t* __restrict__ start, uint32_t n,
                                                             uint32_t swizzle) {
     const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
@@ -483,6 +500,7 @@ IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, con
     a.worldTiles = v->tileFrustaWorld.ptr;
     a.tris = s->triIntersect.ptr;
     a.tileSurvivors = v->tileWork.ptr + size_t(2) * v->tileCount;
+    a.tileTested = v->tileWork.ptr + size_t(3) * v->tileCount;
     a.hits = v->hitOutput ? v->hits.ptr : nullptr;
     a.listCapacity = v->listCapacity;
     a.cameraToWorld = m;

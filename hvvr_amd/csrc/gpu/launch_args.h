@@ -92,6 +92,7 @@ struct IntersectArgs {
     const SimpleRayFrustum* worldTiles;
     const PrecomputedTriangleIntersect* tris;
     uint32_t* tileSurvivors; // per tile: list entries surviving the tile cull
+    uint32_t* tileTested;    // per tile: (wave, survivor) tests run (the rest were skipped as occluded)
     // optional per-subsample output [sampleCount * A]: (ray parameter of the nearest hit,
     // FLT_MAX on a miss; barycentrics u, v, w of that hit, 0 on a miss); null = not written
     vector4* hits;

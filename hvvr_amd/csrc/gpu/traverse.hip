@@ -1132,24 +1132,27 @@ void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream) {
 enum : uint32_t { kReduceThreads = 256 };
 __global__ __launch_bounds__(kReduceThreads) void tileWorkReduceKernel(const uint32_t* __restrict__ work, uint32_t n,
                                                                        unsigned long long* __restrict__ counters) {
-    __shared__ unsigned long long sums[2][kReduceThreads];
+    __shared__ unsigned long long sums[3][kReduceThreads];
     __shared__ uint32_t deepest[kReduceThreads];
     const uint32_t tid = threadIdx.x;
-    unsigned long long visits = 0, survivors = 0;
+    unsigned long long visits = 0, survivors = 0, tested = 0;
     uint32_t deep = 0;
     for (uint32_t i = tid; i < n; i += kReduceThreads) {
         visits += work[i];
         deep = max(deep, work[n + i]);
         survivors += work[2 * size_t(n) + i];
+        tested += work[3 * size_t(n) + i];
     }
     sums[0][tid] = visits;
     sums[1][tid] = survivors;
+    sums[2][tid] = tested;
     deepest[tid] = deep;
     __syncthreads();
     for (uint32_t half = kReduceThreads / 2; half > 0; half >>= 1) {
         if (tid < half) {
             sums[0][tid] += sums[0][tid + half];
             sums[1][tid] += sums[1][tid + half];
+            sums[2][tid] += sums[2][tid + half];
             deepest[tid] = max(deepest[tid], deepest[tid + half]);
         }
         __syncthreads();
@@ -1158,6 +1161,7 @@ __global__ __launch_bounds__(kReduceThreads) void tileWorkReduceKernel(const uin
         counters[0] = sums[1][0] * TILE_SIZE;
         counters[2] = deepest[0];
         counters[4] = sums[0][0];
+        counters[9] = sums[2][0] * 64u; // (wave, survivor) tests x the wave's 64 samples
     }
 }
 

@@ -346,6 +346,9 @@ typedef struct {
     uint32_t hugeTiles;        /* ... the large one to the huge one */
     uint32_t giantTiles;       /* ... the huge one to the giant one */
     uint32_t walkTiles;        /* ... the giant one to the LDS-stack walk */
+    uint64_t testedPairs;      /* (sample, survivor) pairs the intersect tested: survivorPairs less
+                                  the wave-level tests skipped because no ray of the wave could
+                                  reach the triangle in front of its current hits */
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled
