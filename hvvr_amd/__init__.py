@@ -376,7 +376,8 @@ class ViewStats(ctypes.Structure):
     _fields_ = [("listEntries", c_uint64), ("survivorPairs", c_uint64), ("occupiedTiles", c_uint32),
                 ("tileCount", c_uint32), ("maxTileStack", c_uint32), ("overflow", c_uint32),
                 ("nodeVisits", c_uint64), ("largeTiles", c_uint32), ("hugeTiles", c_uint32),
-                ("giantTiles", c_uint32), ("walkTiles", c_uint32), ("testedPairs", c_uint64)]
+                ("giantTiles", c_uint32), ("walkTiles", c_uint32), ("testedPairs", c_uint64),
+                ("colossalTiles", c_uint32)]
 
 
 class GpuContext:

@@ -206,7 +206,7 @@ int planChain(Chain& c) {
     c.offCounters = take(n * sizeof(uint64_t*));
     c.offFrustum = take(n * sizeof(FrustumSetupArgs));
     c.offBlock = take(n * sizeof(BlockTraverseArgs));
-    c.offTile = take(4 * n * sizeof(TileListArgs));
+    c.offTile = take(5 * n * sizeof(TileListArgs));
     c.offWalk = take(n * sizeof(TileWalkArgs));
     for (Group& g : c.groups) {
         g.intersectOff = take(g.members.size() * sizeof(IntersectArgs));
@@ -274,7 +274,7 @@ void fillArgs(Chain& c, const hvvr_gpu_scene* s, const matrix4x4* m) {
         counters[i] = v->counters.ptr;
         frustum[i] = frustumSetupArgs(v, mi);
         block[i] = blockTraverseArgs(v, s);
-        for (int level = 0; level < 4; level++) {
+        for (int level = 0; level < 5; level++) {
             tile[level * n + i] = tileListArgs(v, s, level);
             // the chain's persistent workgroups take every view's queue in turn: one
             // scratch (its first view's) serves them

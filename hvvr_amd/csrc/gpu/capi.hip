@@ -13,13 +13,13 @@
 
 // Shared scratch of hvvr_gpu_view_render_shared_scratch, one copy per GPU for the life of
 // the process: the record scratch of the queued tile-list configurations
-// (kSharedRecordWords words, as many persistent workgroups as it holds), then the four tile
-// queues, [4][tileCount] words. Kernels and host copies use the device address from
+// (kSharedRecordWords words, as many persistent workgroups as it holds), then the five tile
+// queues, [5][tileCount] words. Kernels and host copies use the device address from
 // hipGetSymbolAddress (sharedScratchOf below), never this name.
 enum : uint32_t {
-    kSharedRecordWords = 15u << 18,                       // 15 MiB
+    kSharedRecordWords = 59u << 16,                       // 14.75 MiB
     kSharedMaxTiles = 1u << 16,                           // 8,388,608 samples
-    kSharedScratchWords = kSharedRecordWords + 4u * kSharedMaxTiles // 16 MiB in all
+    kSharedScratchWords = kSharedRecordWords + 5u * kSharedMaxTiles // 16 MiB in all
 };
 static __device__ __align__(256) uint32_t scratch[kSharedScratchWords];
 
@@ -453,7 +453,7 @@ int hvvr::makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool devi
     step(v->blockStacks.resize(size_t(v->blockCount) * BLOCK_STACK_SIZE));
     step(v->blockStackSize.resize(v->blockCount));
     step(v->tileCounts.resize(v->tileCount));
-    step(v->tileQueues.resize(size_t(4) * v->tileCount));
+    step(v->tileQueues.resize(size_t(5) * v->tileCount));
     step(v->recScratch.resize(tileListScratchWords()));
     step(v->tileLevel.resize(v->tileCount));
     step(v->tileWork.resize(size_t(4) * v->tileCount));
@@ -970,7 +970,7 @@ int hvvr_gpu_view_estimate_bytes(const hvvr_gpu_view_desc* d, const hvvr_gpu_fov
     const uint64_t listCapacity = std::max<uint64_t>(T * 1024, 1u << 20);
     uint64_t b = S * sizeof(DirectionalBeam) + 2 * T * sizeof(SimpleRayFrustum) + B * sizeof(SimpleRayFrustum) +
                  T * sizeof(FrustumPre) + B * sizeof(FrustumPre) + B * BLOCK_STACK_SIZE * sizeof(StackEntry) +
-                 B * 4 + 11 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
+                 B * 4 + 12 * T * 4 + uint64_t(tileListScratchWords()) * 4 + listCapacity * 4 +
                  S * A * sizeof(GBufferSubsample) + S * 4 + pixels * 4 + 16 * 8;
     if (d->remap)
         b += pixels * d->split * 4;
@@ -1153,8 +1153,9 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* v, hvvr_gpu_view_stats* st) {
     st->largeTiles = uint32_t(counters[5]);
     st->hugeTiles = uint32_t(counters[6]);
     st->giantTiles = uint32_t(counters[7]);
-    st->walkTiles = uint32_t(counters[8]);
-    st->testedPairs = counters[9];
+    st->colossalTiles = uint32_t(counters[8]);
+    st->walkTiles = uint32_t(counters[9]);
+    st->testedPairs = counters[10];
     st->tileCount = v->tileCount;
     for (uint32_t c : counts)
         st->occupiedTiles += c ? 1u : 0u;

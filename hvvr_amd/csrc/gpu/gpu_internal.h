@@ -149,12 +149,13 @@ struct hvvr_gpu_view {
     hvvr::DeviceBuffer<uint32_t> recScratch;            // record node indices and emission order of the
                                                         // queued tile-list configurations, per workgroup
     hvvr::DeviceBuffer<uint32_t> tileLevel;             // [tileCount] tile-list configuration hint
-    hvvr::DeviceBuffer<uint32_t> tileWork;              // [3][tileCount] per tile, last frame: BVH nodes
+    hvvr::DeviceBuffer<uint32_t> tileWork;              // [4][tileCount] per tile, last frame: BVH nodes
                                                         // tested by the tile traversal, its deepest stack,
-                                                        // triangles surviving intersect's tile cull
+                                                        // triangles surviving intersect's tile cull,
+                                                        // (wave, survivor) tests intersect ran
     const hvvr_gpu_scene* hintScene = nullptr;          // the scene the hints were last written for
-    hvvr::DeviceBuffer<uint32_t> tileQueues;            // [4][tileCount] tiles queued for the large, huge
-                                                        // and giant LDS configurations and the walk
+    hvvr::DeviceBuffer<uint32_t> tileQueues;            // [5][tileCount] tiles queued for the large, huge,
+                                                        // giant and colossal LDS configurations and the walk
     hvvr::DeviceBuffer<uint32_t> tileCounts;            // [tileCount] list length
     hvvr::DeviceBuffer<uint32_t> tileOffsets;           // [tileCount] list start in triIndices
     hvvr::DeviceBuffer<uint32_t> triIndices;            // list storage, tiles in allocation order
@@ -205,16 +206,16 @@ namespace hvvr {
 
 // Per-frame scratch of the tile-list kernels taken from outside the view (the shared
 // scratch of hvvr_gpu_view_render_shared_scratch): the queued configurations' record
-// scratch, `groups[L]` persistent workgroups' worth for the large, huge and giant
-// configurations (L = 0, 1, 2), and the four tile queues, [4][tileCount].
+// scratch, `groups[L]` persistent workgroups' worth for the large, huge, giant and
+// colossal configurations (L = 0..3), and the five tile queues, [5][tileCount].
 struct TileListScratch {
     uint32_t* rec;
     uint32_t* queues;
-    uint32_t groups[3];
+    uint32_t groups[4];
 };
-// the persistent workgroups of the large, huge and giant configurations whose record
-// scratch fits `words` 32-bit words (at most as many as a view's own scratch holds)
-void tileListGroupsFor(size_t words, uint32_t groups[3]);
+// the persistent workgroups of the large, huge, giant and colossal configurations whose
+// record scratch fits `words` 32-bit words (at most as many as a view's own scratch holds)
+void tileListGroupsFor(size_t words, uint32_t groups[4]);
 
 // hvvr_gpu_view_create; with `deviceSetup` the sample hierarchy is allocated, not uploaded
 int makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool deviceSetup, hvvr_gpu_view** out);

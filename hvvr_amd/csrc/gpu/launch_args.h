@@ -160,8 +160,9 @@ struct LaunchShape {
     uint32_t streams = 1; // batches: chains the views are dealt to, each on its own stream
 };
 // the most persistent workgroups each queued configuration launches (its record scratch
-// is sized for them): as many as the chip holds at once
-enum : uint32_t { kMaxLargeGroups = 2560, kMaxHugeGroups = 1024, kMaxGiantGroups = 512 };
+// is sized for them): as many as the chip holds at once; the colossal configuration, which
+// takes a CU's whole LDS and a few tiles of a dense scene, a quarter of the CUs
+enum : uint32_t { kMaxLargeGroups = 2560, kMaxHugeGroups = 1024, kMaxGiantGroups = 512, kMaxColossalGroups = 64 };
 // 32-bit words of record scratch one view (or one batch chain) needs
 size_t tileListScratchWords();
 

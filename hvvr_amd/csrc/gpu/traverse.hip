@@ -241,13 +241,16 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // -inf, so a tile with a NaN or infinite key, or with an unsorted block stack, goes to the
 // global-stack kernel, as does a tile whose records or stack exceed the capacity.
 //
-// Four configurations share the code. The standard one (680 records, a one-register
+// Five configurations share the code. The standard one (680 records, a one-register
 // 64-entry stack, 8 KB of LDS) runs every tile; a tile it cannot hold is queued for
 // the large one (2048 records, a four-register 256-entry stack, 16 KB of LDS), what that
 // cannot hold for the huge one (4608 records, eight registers, 512 entries, 36 KB), and
-// so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB); each
-// takes its queue with persistent workgroups, as many as fit the chip at once. What none
-// holds goes to the wave-per-tile walk with its stack in LDS.
+// so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB) and the
+// colossal one (20480 records, thirty-two registers, 2048 entries, all 160 KB of a CU's
+// LDS); each takes its queue with persistent workgroups, as many as fit the chip at once
+// (the colossal one with a few: it takes the odd tile of a dense scene seen through a
+// wide tile frustum). What none holds goes to the wave-per-tile walk with its stack in
+// LDS.
 // ---------------------------------------------------------------------------------
 // Record meta: first child record (bits 0..19), child mask (20..23), leaf mask (24..27),
 // stack-entry leaf mask (28..31).
@@ -280,6 +283,10 @@ struct TileListHuge {
 struct TileListGiant {
     enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 3 };
     using Stack = RegStack16;
+};
+struct TileListColossal {
+    enum : uint32_t { RecCap = 20480, StackCap = REG_STACK32_CAP, Packed = 0, LdsRecords = 0, Level = 4 };
+    using Stack = RegStack32;
 };
 
 // 32-bit words of one workgroup's scratch region: node indices, then 16-bit order entries
@@ -378,9 +385,9 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         // A tile whose last list on this scene needed a still bigger configuration goes
         // straight to that one's queue (queue h - 1, its length in counters[4 + h]). The
         // standard configuration takes every tile first and keeps no hints: its code stays
-        // as short as it is. Hints are 0 (none) or 1..4; anything else is ignored.
+        // as short as it is. Hints are 0 (none) or 1..5; anything else is ignored.
         const uint32_t hint = g.useHints ? g.tileLevel[tile] : 0u;
-        if (hint > uint32_t(Cfg::Level) && hint <= 4u) {
+        if (hint > uint32_t(Cfg::Level) && hint <= 5u) {
             if (lane == 0) {
                 const unsigned long long slot = atomicAdd(&g.counters[4 + hint], 1ull);
                 g.nextQueue[size_t(hint - 1u - Cfg::Level) * g.tileCount + slot] = tile;
@@ -749,9 +756,10 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         if constexpr (Cfg::Level > 0) {
             // the smallest queued configuration this tile's records and stack fit, for its
             // next frame
-            const uint32_t level = nrec <= TileListLarge::RecCap && deepest <= REG_STACK4_CAP  ? 1u
-                                   : nrec <= TileListHuge::RecCap && deepest <= REG_STACK8_CAP ? 2u
-                                                                                               : 3u;
+            const uint32_t level = nrec <= TileListLarge::RecCap && deepest <= REG_STACK4_CAP    ? 1u
+                                   : nrec <= TileListHuge::RecCap && deepest <= REG_STACK8_CAP   ? 2u
+                                   : nrec <= TileListGiant::RecCap && deepest <= REG_STACK16_CAP ? 3u
+                                                                                                 : 4u;
             g.tileLevel[tile] = level;
         }
     }
@@ -995,11 +1003,13 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
                 const uint32_t capL = full ? REG_STACK4_CAP : a.stackLimit;
                 const uint32_t capH = full ? REG_STACK8_CAP : a.stackLimit;
                 const uint32_t capG = full ? REG_STACK16_CAP : a.stackLimit;
-                a.tileLevel[tile] = irregular                                             ? 4u
-                                    : visits <= TileListLarge::RecCap && maxSize <= capL  ? 1u
-                                    : visits <= TileListHuge::RecCap && maxSize <= capH   ? 2u
-                                    : visits <= TileListGiant::RecCap && maxSize <= capG  ? 3u
-                                                                                          : 4u;
+                const uint32_t capC = full ? REG_STACK32_CAP : a.stackLimit;
+                a.tileLevel[tile] = irregular                                               ? 5u
+                                    : visits <= TileListLarge::RecCap && maxSize <= capL    ? 1u
+                                    : visits <= TileListHuge::RecCap && maxSize <= capH     ? 2u
+                                    : visits <= TileListGiant::RecCap && maxSize <= capG    ? 3u
+                                    : visits <= TileListColossal::RecCap && maxSize <= capC ? 4u
+                                                                                            : 5u;
                 a.tileWork[tile] = visits; // node visits
                 a.tileWork[a.tileCount + tile] = maxSize;
             }
@@ -1011,7 +1021,7 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
 
 __global__ __launch_bounds__(64) void tileWalkKernel(TileWalkArgs a, int emit) {
     __shared__ TileWalkLds sh;
-    const uint32_t queued = uniform(uint32_t(a.counters[8]));
+    const uint32_t queued = uniform(uint32_t(a.counters[9]));
     for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
         waveLdsSync(); // the previous tile's LDS reads are done
         tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
@@ -1024,7 +1034,7 @@ __global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __
     uint32_t base = 0;
     for (uint32_t v = 0; v < n; v++) {
         const TileWalkArgs& a = all[v];
-        const uint32_t queued = uniform(uint32_t(a.counters[8]));
+        const uint32_t queued = uniform(uint32_t(a.counters[9]));
         for (uint32_t qi = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x; qi < queued; qi += gridDim.x) {
             waveLdsSync();
             tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
@@ -1064,7 +1074,7 @@ TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s) {
     a.tilePre = v->tilePre.ptr;
     a.blockStacks = v->blockStacks.ptr;
     a.blockStackSize = v->blockStackSize.ptr;
-    a.queue = v->tileQueues.ptr + 3 * size_t(v->tileCount);
+    a.queue = v->tileQueues.ptr + 4 * size_t(v->tileCount);
     a.counts = v->tileCounts.ptr;
     a.offsets = v->tileOffsets.ptr;
     a.triOut = v->triIndices.ptr;
@@ -1161,7 +1171,7 @@ __global__ __launch_bounds__(kReduceThreads) void tileWorkReduceKernel(const uin
         counters[0] = sums[1][0] * TILE_SIZE;
         counters[2] = deepest[0];
         counters[4] = sums[0][0];
-        counters[9] = sums[2][0] * 64u; // (wave, survivor) tests x the wave's 64 samples
+        counters[10] = sums[2][0] * 64u; // (wave, survivor) tests x the wave's 64 samples
     }
 }
 
@@ -1171,7 +1181,7 @@ void launchTileWorkReduce(hvvr_gpu_view* v, hipStream_t stream) {
 }
 
 TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level) {
-    uint32_t* queues = v->tileQueues.ptr; // [4][tileCount]: large, huge, giant, walk
+    uint32_t* queues = v->tileQueues.ptr; // [5][tileCount]: large, huge, giant, colossal, walk
     const size_t T = v->tileCount;
     TileListArgs g;
     g.nodes = s->nodes.ptr;
@@ -1191,7 +1201,7 @@ TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int l
     g.nextQueue = queues + level * T;
     g.nextCounter = uint32_t(5 + level);
     const bool full = v->waveStackLimit >= REG_STACK_CAP;
-    const uint32_t caps[4] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP};
+    const uint32_t caps[5] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP, REG_STACK32_CAP};
     g.stackLimit = level == 0 || !full ? v->waveStackLimit : caps[level];
     g.recScratch = v->recScratch.ptr;
     g.tileLevel = v->tileLevel.ptr;
@@ -1209,14 +1219,15 @@ size_t tileListScratchWords() {
     const size_t large = size_t(kMaxLargeGroups) * recScratchWords<TileListLarge>();
     const size_t huge = size_t(kMaxHugeGroups) * recScratchWords<TileListHuge>();
     const size_t giant = size_t(kMaxGiantGroups) * recScratchWords<TileListGiant>();
-    return std::max(large, std::max(huge, giant));
+    const size_t colossal = size_t(kMaxColossalGroups) * recScratchWords<TileListColossal>();
+    return std::max(std::max(large, huge), std::max(giant, colossal));
 }
 
-void tileListGroupsFor(size_t words, uint32_t groups[3]) {
-    const size_t per[3] = {recScratchWords<TileListLarge>(), recScratchWords<TileListHuge>(),
-                           recScratchWords<TileListGiant>()};
-    const uint32_t most[3] = {kMaxLargeGroups, kMaxHugeGroups, kMaxGiantGroups};
-    for (int k = 0; k < 3; k++)
+void tileListGroupsFor(size_t words, uint32_t groups[4]) {
+    const size_t per[4] = {recScratchWords<TileListLarge>(), recScratchWords<TileListHuge>(),
+                           recScratchWords<TileListGiant>(), recScratchWords<TileListColossal>()};
+    const uint32_t most[4] = {kMaxLargeGroups, kMaxHugeGroups, kMaxGiantGroups, kMaxColossalGroups};
+    for (int k = 0; k < 4; k++)
         groups[k] = uint32_t(std::min<size_t>(most[k], words / per[k]));
 }
 
@@ -1239,11 +1250,14 @@ void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stre
     const uint32_t large = scratch ? scratch->groups[0] : std::min(shape.largeGroups, uint32_t(kMaxLargeGroups));
     const uint32_t huge = scratch ? scratch->groups[1] : std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups));
     const uint32_t giant = scratch ? scratch->groups[2] : std::min(shape.giantGroups, uint32_t(kMaxGiantGroups));
+    const uint32_t colossal = scratch ? scratch->groups[3] : uint32_t(kMaxColossalGroups);
     tileListKernel<<<T, 64, 0, stream>>>(tileListArgsWith(v, s, 0, scratch));
-    // 28 KB of LDS: five workgroups per CU; 63 KB: two; 140 KB: one
+    // 16 KB of LDS: ten workgroups per CU; 36 KB: four; 80 KB: two; 160 KB: one
     tileListQueueKernel<TileListLarge><<<clampGroups(large, T), 64, 0, stream>>>(tileListArgsWith(v, s, 1, scratch));
     tileListQueueKernel<TileListHuge><<<clampGroups(huge, T), 64, 0, stream>>>(tileListArgsWith(v, s, 2, scratch));
     tileListQueueKernel<TileListGiant><<<clampGroups(giant, T), 64, 0, stream>>>(tileListArgsWith(v, s, 3, scratch));
+    tileListQueueKernel<TileListColossal>
+        <<<clampGroups(colossal, T), 64, 0, stream>>>(tileListArgsWith(v, s, 4, scratch));
 }
 
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
@@ -1252,7 +1266,7 @@ void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream
     const uint32_t T = v->tileCount;
     TileWalkArgs a = tileWalkArgs(v, s);
     if (scratch)
-        a.queue = scratch->queues + 3 * size_t(T);
+        a.queue = scratch->queues + 4 * size_t(T);
     for (int emit = 0; emit < 2; emit++)
         tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a, emit); // 32 KB of LDS each
 }
@@ -1269,6 +1283,8 @@ void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32
     tileListQueueBatchKernel<TileListGiant>
         <<<clampGroups(std::min(shape.giantGroups, uint32_t(kMaxGiantGroups)), units), 64, 0, stream>>>(
             all + 3 * viewCount, viewCount);
+    tileListQueueBatchKernel<TileListColossal>
+        <<<clampGroups(uint32_t(kMaxColossalGroups), units), 64, 0, stream>>>(all + 4 * viewCount, viewCount);
 }
 
 void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,

@@ -160,7 +160,7 @@ int hvvr_gpu_view_render(hvvr_gpu_view* view, hvvr_gpu_scene* scene, const float
 /* No counterpart in the reference. For callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops that forbid hipMalloc): hvvr_gpu_view_render with
  * the tile-list kernels' per-frame scratch (the queued configurations' record scratch and
- * the four tile queues) taken from one 16 MiB device array that the library holds per GPU
+ * the five tile queues) taken from one 16 MiB device array that the library holds per GPU
  * for the life of the process, instead of the view's own buffers. Allocates nothing; the
  * results are those of hvvr_gpu_view_render. Views of more than 65,536 tiles (8,388,608
  * samples) are rejected (HVVR_GPU_ERR_INVALID). Every call on one GPU uses the same array:
@@ -345,10 +345,11 @@ typedef struct {
     uint32_t largeTiles;       /* tiles the standard LDS configuration passed to the large one */
     uint32_t hugeTiles;        /* ... the large one to the huge one */
     uint32_t giantTiles;       /* ... the huge one to the giant one */
-    uint32_t walkTiles;        /* ... the giant one to the LDS-stack walk */
+    uint32_t walkTiles;        /* ... the colossal one to the LDS-stack walk */
     uint64_t testedPairs;      /* (sample, survivor) pairs the intersect tested: survivorPairs less
                                   the wave-level tests skipped because no ray of the wave could
                                   reach the triangle in front of its current hits */
+    uint32_t colossalTiles;    /* tiles the giant configuration passed to the colossal one */
 } hvvr_gpu_view_stats;
 int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
 /* Per-stage GPU milliseconds of the view's last frame rendered with profiling enabled

@@ -61,7 +61,7 @@ def test_dense_scene_lists_grow_and_match(H, O, ctx, dense, dense_view, dense_re
     assert len(bad) == 0, f"{len(bad)} samples differ, first {bad[:5]}"
     st = gv.stats()
     print(f"tiles {st.tileCount}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
-          f"walk {st.walkTiles}, "
+          f"colossal {st.colossalTiles}, walk {st.walkTiles}, "
           f"deepest stack {st.maxTileStack}", flush=True)
     assert st.overflow == 0 and st.survivorPairs == dense_ref["survivors"]
     if limit is None:
@@ -71,8 +71,9 @@ def test_dense_scene_lists_grow_and_match(H, O, ctx, dense, dense_view, dense_re
     else:
         # every tile passes every queue to the walk; on the second frame the large
         # configuration's kernel sends it there directly (the hint from the first)
-        assert first.walkTiles == first.largeTiles == first.hugeTiles == first.giantTiles > 0
-        assert st.largeTiles == st.walkTiles == first.walkTiles and st.hugeTiles == st.giantTiles == 0
+        assert first.walkTiles == first.largeTiles == first.hugeTiles == first.giantTiles == first.colossalTiles > 0
+        assert st.largeTiles == st.walkTiles == first.walkTiles
+        assert st.hugeTiles == st.giantTiles == st.colossalTiles == 0
     gv.close()
     gs.close()
 
@@ -127,8 +128,42 @@ def test_giant_configuration_lists_match(H, O, ctx):
         gv.render(gs)
         st = gv.stats()
         print(f"frame {frame}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
-              f"walk {st.walkTiles}, deepest stack {st.maxTileStack}", flush=True)
+              f"colossal {st.colossalTiles}, walk {st.walkTiles}, deepest stack {st.maxTileStack}", flush=True)
         assert st.overflow == 0 and st.giantTiles > 0, "no tile reached the giant configuration"
+        counts, lists = gv.read_tile_lists()
+        assert np.array_equal(counts, ref["counts"]), f"frame {frame}: tile list lengths differ"
+        assert np.array_equal(lists, ref["lists"]), f"frame {frame}: tile list contents/order differ"
+        tri, mask = gv.read_gbuffer()
+        bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
+        assert len(bad) == 0, f"frame {frame}: {len(bad)} samples differ, first {bad[:5]}"
+        assert st.survivorPairs == ref["survivors"]
+    gv.close()
+    gs.close()
+
+
+def test_colossal_configuration_lists_match(H, O, ctx):
+    # the 2M-triangle room at 96x96: each tile's frustum is wide, and some tiles need more
+    # than the giant configuration's 10,240 records (up to ~19k here) and are built by the
+    # colossal one (20,480 records, a thirty-two-register stack, a CU's whole LDS) instead
+    # of the one-wave walk; lists, ids and masks match the oracle bit for bit on the first
+    # frame after the list growth and on a frame that follows the per-tile hints
+    scene = H.HostScene(H.SCENE_ROOM, 0, 8192)
+    view = H.HostView(H.VIEW_PRESET_ROOM_EYE, 0, 96, 96)
+    ref = O.render_view(scene, view, 1, with_colors=False)
+    gs = H.GpuScene(ctx, scene)
+    gv = H.GpuView(ctx, view, 1)
+    gv.render(gs)
+    need, cap, overflowed = gv.list_demand()
+    assert need == int(ref["counts"].sum())
+    if overflowed:
+        gv.reserve_lists(need)
+    rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+    for frame in range(2):
+        gv.render(gs)
+        st = gv.stats()
+        print(f"frame {frame}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
+              f"colossal {st.colossalTiles}, walk {st.walkTiles}, deepest stack {st.maxTileStack}", flush=True)
+        assert st.overflow == 0 and st.colossalTiles > 0, "no tile reached the colossal configuration"
         counts, lists = gv.read_tile_lists()
         assert np.array_equal(counts, ref["counts"]), f"frame {frame}: tile list lengths differ"
         assert np.array_equal(lists, ref["lists"]), f"frame {frame}: tile list contents/order differ"

@@ -45,13 +45,14 @@ def test_shared_scratch_render_matches_render(H, ctx):
         _render_grown(gv, gs, gv.render_shared_scratch)
         got, st = _results(gv)
         print(f"{w}x{h}: tiles {st.tileCount}, queued large {st.largeTiles}, huge {st.hugeTiles}, "
-              f"giant {st.giantTiles}, walk {st.walkTiles}", flush=True)
+              f"giant {st.giantTiles}, colossal {st.colossalTiles}, walk {st.walkTiles}", flush=True)
         assert st.largeTiles > 0, "no tile used the queued configurations' scratch"
         for k in want:
             assert np.array_equal(got[k], want[k]), f"{w}x{h}: {k} differs from hvvr_gpu_view_render"
-        assert (st.largeTiles, st.hugeTiles, st.giantTiles, st.walkTiles, st.listEntries, st.survivorPairs) == \
-            (want_st.largeTiles, want_st.hugeTiles, want_st.giantTiles, want_st.walkTiles, want_st.listEntries,
-             want_st.survivorPairs)
+        assert (st.largeTiles, st.hugeTiles, st.giantTiles, st.colossalTiles, st.walkTiles, st.listEntries,
+                st.survivorPairs) == \
+            (want_st.largeTiles, want_st.hugeTiles, want_st.giantTiles, want_st.colossalTiles, want_st.walkTiles,
+             want_st.listEntries, want_st.survivorPairs)
         gv.close()
         ref_view.close()
     gs.close()
