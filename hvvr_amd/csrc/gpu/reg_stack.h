@@ -89,8 +89,7 @@ struct RegStackN {
 using RegStack4 = RegStackN<4>;
 using RegStack8 = RegStackN<8>;
 using RegStack16 = RegStackN<16>;
-using RegStack32 = RegStackN<32>;
-enum : uint32_t { REG_STACK4_CAP = 256, REG_STACK8_CAP = 512, REG_STACK16_CAP = 1024, REG_STACK32_CAP = 2048 };
+enum : uint32_t { REG_STACK4_CAP = 256, REG_STACK8_CAP = 512, REG_STACK16_CAP = 1024 };
 
 template <int N>
 __device__ inline void stackInit(RegStackN<N>& st, float t, uint32_t m, uint32_t size, uint32_t) {

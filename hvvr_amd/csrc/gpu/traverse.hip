@@ -246,11 +246,12 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // the large one (2048 records, a four-register 256-entry stack, 16 KB of LDS), what that
 // cannot hold for the huge one (4608 records, eight registers, 512 entries, 36 KB), and
 // so on to the giant one (10240 records, sixteen registers, 1024 entries, 80 KB) and the
-// colossal one (20480 records, thirty-two registers, 2048 entries, all 160 KB of a CU's
-// LDS); each takes its queue with persistent workgroups, as many as fit the chip at once
-// (the colossal one with a few: it takes the odd tile of a dense scene seen through a
-// wide tile frustum). What none holds goes to the wave-per-tile walk with its stack in
-// LDS.
+// colossal one (20480 records, the giant one's stack, all 160 KB of a CU's LDS); each
+// takes its queue with persistent workgroups, as many as fit the chip at once (the
+// colossal one with a few: it takes the odd tile of a dense scene seen through a wide
+// tile frustum, whose records, not its stack, outgrow the giant one; a 32-register stack
+// made its unrolled walk 20% slower). What none holds goes to the wave-per-tile walk with
+// its stack in LDS.
 // ---------------------------------------------------------------------------------
 // Record meta: first child record (bits 0..19), child mask (20..23), leaf mask (24..27),
 // stack-entry leaf mask (28..31).
@@ -285,8 +286,8 @@ struct TileListGiant {
     using Stack = RegStack16;
 };
 struct TileListColossal {
-    enum : uint32_t { RecCap = 20480, StackCap = REG_STACK32_CAP, Packed = 0, LdsRecords = 0, Level = 4 };
-    using Stack = RegStack32;
+    enum : uint32_t { RecCap = 20480, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 4 };
+    using Stack = RegStack16;
 };
 
 // 32-bit words of one workgroup's scratch region: node indices, then 16-bit order entries
@@ -758,8 +759,8 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
             // next frame
             const uint32_t level = nrec <= TileListLarge::RecCap && deepest <= REG_STACK4_CAP    ? 1u
                                    : nrec <= TileListHuge::RecCap && deepest <= REG_STACK8_CAP   ? 2u
-                                   : nrec <= TileListGiant::RecCap && deepest <= REG_STACK16_CAP ? 3u
-                                                                                                 : 4u;
+                                   : nrec <= TileListGiant::RecCap ? 3u
+                                                                   : 4u;
             g.tileLevel[tile] = level;
         }
     }
@@ -1003,7 +1004,7 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
                 const uint32_t capL = full ? REG_STACK4_CAP : a.stackLimit;
                 const uint32_t capH = full ? REG_STACK8_CAP : a.stackLimit;
                 const uint32_t capG = full ? REG_STACK16_CAP : a.stackLimit;
-                const uint32_t capC = full ? REG_STACK32_CAP : a.stackLimit;
+                const uint32_t capC = full ? REG_STACK16_CAP : a.stackLimit;
                 a.tileLevel[tile] = irregular                                               ? 5u
                                     : visits <= TileListLarge::RecCap && maxSize <= capL    ? 1u
                                     : visits <= TileListHuge::RecCap && maxSize <= capH     ? 2u
@@ -1201,7 +1202,7 @@ TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int l
     g.nextQueue = queues + level * T;
     g.nextCounter = uint32_t(5 + level);
     const bool full = v->waveStackLimit >= REG_STACK_CAP;
-    const uint32_t caps[5] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP, REG_STACK32_CAP};
+    const uint32_t caps[5] = {v->waveStackLimit, REG_STACK4_CAP, REG_STACK8_CAP, REG_STACK16_CAP, REG_STACK16_CAP};
     g.stackLimit = level == 0 || !full ? v->waveStackLimit : caps[level];
     g.recScratch = v->recScratch.ptr;
     g.tileLevel = v->tileLevel.ptr;

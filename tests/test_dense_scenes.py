@@ -144,7 +144,7 @@ def test_giant_configuration_lists_match(H, O, ctx):
 def test_colossal_configuration_lists_match(H, O, ctx):
     # the 2M-triangle room at 96x96: each tile's frustum is wide, and some tiles need more
     # than the giant configuration's 10,240 records (up to ~19k here) and are built by the
-    # colossal one (20,480 records, a thirty-two-register stack, a CU's whole LDS) instead
+    # colossal one (20,480 records, a sixteen-register stack, a CU's whole LDS) instead
     # of the one-wave walk; lists, ids and masks match the oracle bit for bit on the first
     # frame after the list growth and on a frame that follows the per-tile hints
     scene = H.HostScene(H.SCENE_ROOM, 0, 8192)
