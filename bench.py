@@ -118,7 +118,7 @@ def measured_counters(kernel):
     return {}
 
 
-def roofline(stage_ms, survivors, entries, visits, tested):
+def roofline(stage_ms, survivors, entries, visits, tested, kernel_ms=None):
     """Roofline of the dominant kernel, per launch (one launch per eye).
     tile_lists: algorithmic bytes = 128 B per BVH node visit + 4 B per list entry written,
     against HBM; the node reads mostly hit L2, so beside it stand the HBM bytes the
@@ -126,9 +126,14 @@ def roofline(stage_ms, survivors, entries, visits, tested):
     is bound by the issue and dependency latency of its stack walk, not by either;
     intersect: FP32 VALU flops = N_pair (80 + 16 A) with N_pair the (sample, survivor) pairs
     the kernel tested (the tile-cull survivors less those its occlusion skip left out),
-    against the FP32 vector figure."""
+    against the FP32 vector figure. `kernel_ms`: the standard tile-list kernel's own average
+    duration over the timed frames (HIP events its dispatch records), used for tile_lists
+    when given; otherwise the stage's events."""
     dom = STAGES[int(np.argmax(stage_ms))]
     t = stage_ms[STAGES.index(dom)] * 1e-3
+    timed_live = dom == "tile_lists" and kernel_ms is not None
+    if timed_live:
+        t = kernel_ms * 1e-3
     if dom == "intersect":
         name = "intersectKernel<4>"
         flops = tested * (80 + 16 * A)
@@ -148,6 +153,8 @@ def roofline(stage_ms, survivors, entries, visits, tested):
     pmc = measured_counters(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(bytes_ / t / 1e9, 3), "peak": PEAK_HBM / 1e9,
             "unit": "GB/s", "frac": round(bytes_ / t / PEAK_HBM, 5), "traffic": traffic,
+            "kernel_ms": round(t * 1e3, 4),
+            "kernel_ms_source": "timed frames, dispatch events" if timed_live else "per-stage events",
             "hbm_counter_frac": None if traffic is None else round(traffic / t / PEAK_HBM, 5),
             "l2_hit_pct": pmc.get("l2_hit"), "salu_share": pmc.get("salu_share"),
             "valu_busy_pct": pmc.get("valu_busy"),
@@ -469,11 +476,13 @@ def main():
     sync()
 
     # timed region: K stereo frames, each synchronised so its latency is measured too; in
-    # the plain and batched pipelines the views record per-stage HIP events on the
-    # library's stream (no extra synchronisation), read after each frame's synchronise
-    ctx.set_profiling(batch is None)
+    # the plain and batched pipelines the standard tile-list kernel's dispatch records HIP
+    # events (profiling level 2: no packet added to the stream), read after each frame's
+    # synchronise. Per-stage events (level 1) put a marker between every two kernels and
+    # slow the frame ~10%, so the stage times come from a pass after the timed region.
+    ctx.set_profiling(2 if batch is None else 0)
     lat = []
-    stage_ms = np.zeros(len(STAGES))
+    kernel_ms = []
     sync()
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -483,27 +492,28 @@ def main():
         lat.append(time.perf_counter() - f0)
         if batch is None:
             for gv in gvs:
-                stage_ms += gv.stage_times()
+                kernel_ms.append(gv.tile_list_kernel_ms())
     sync()
     elapsed = time.perf_counter() - t_start
     ctx.set_profiling(False)
     samples_per_frame = EYES * W * HGT * A
     value = samples_per_frame * args.steps / elapsed / 1e6
     lat_ms = np.array(lat) * 1e3
-    if batch is not None:
-        # per-stage times of the same views rendered one at a time (a graph replay records
-        # no per-stage events)
-        ctx.set_profiling(True)
-        for _ in range(5):
+    # per-stage times: the same views, one frame after the other in the same submission,
+    # per-stage events on (a graph replay records none)
+    stage_ms = np.zeros(len(STAGES))
+    ctx.set_profiling(1)
+    for _ in range(5):
+        if batch is None:
+            frame()
+        else:
             for gv in gvs:
                 gv.render(gs)
-            ctx.synchronize()
-            for gv in gvs:
-                stage_ms += gv.stage_times()
-        ctx.set_profiling(False)
-        stage_ms /= 5 * EYES
-    else:
-        stage_ms /= args.steps * EYES  # average per launch (one launch per eye)
+        ctx.synchronize()
+        for gv in gvs:
+            stage_ms += gv.stage_times()
+    ctx.set_profiling(0)
+    stage_ms /= 5 * EYES  # average per launch (one launch per eye)
     stats = [gv.stats() for gv in gvs]
     survivors = sum(s.survivorPairs for s in stats) / EYES
     entries = sum(s.listEntries for s in stats) / EYES
@@ -513,7 +523,8 @@ def main():
 
     visits = sum(s.nodeVisits for s in stats) / EYES
     tested = sum(s.testedPairs for s in stats) / EYES
-    roof = roofline(stage_ms, survivors, entries, visits, tested)
+    roof = roofline(stage_ms, survivors, entries, visits, tested,
+                    float(np.mean(kernel_ms)) if kernel_ms else None)
 
     out = {
         "metric": "Msamples/s shaded at 2x1080x1200 MSAA4, 262k tris; p99 frame latency (ms)",

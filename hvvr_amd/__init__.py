@@ -390,7 +390,10 @@ class GpuContext:
         _check_gpu(self.lib.hvvr_gpu_context_synchronize(self.handle))
 
     def set_profiling(self, enable):
-        _check_gpu(self.lib.hvvr_gpu_context_set_profiling(self.handle, c_int(1 if enable else 0)))
+        """True / 1: per-stage events and the tile-list kernel's dispatch timing; 2: only the
+        dispatch timing (no packet added to the stream); False / 0: off."""
+        level = (1 if enable else 0) if isinstance(enable, bool) else int(enable)
+        _check_gpu(self.lib.hvvr_gpu_context_set_profiling(self.handle, c_int(level)))
 
     def close(self):
         if self.handle:
@@ -671,6 +674,12 @@ class GpuView:
         out = np.zeros(len(STAGES), np.float32)
         n = _check_gpu(self.lib.hvvr_gpu_view_stage_times(self.handle, _ptr(out), c_int(len(STAGES))))
         return out[:n]
+
+    def tile_list_kernel_ms(self):
+        """GPU ms of the standard tile-list kernel in the last profiled frame (dispatch events)."""
+        ms = ctypes.c_float(0.0)
+        _check_gpu(self.lib.hvvr_gpu_view_tile_list_kernel_time(self.handle, byref(ms)))
+        return float(ms.value)
 
     def close(self):
         if self.handle:

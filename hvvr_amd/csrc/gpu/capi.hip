@@ -220,7 +220,9 @@ int hvvr_gpu_context_synchronize(hvvr_gpu_context* c) {
 int hvvr_gpu_context_set_profiling(hvvr_gpu_context* c, int enable) {
     if (!c)
         return fail("null context");
-    c->profiling = enable != 0;
+    if (enable < 0 || enable > 2)
+        return fail("profiling level must be 0, 1 or 2");
+    c->profiling = enable;
     return HVVR_GPU_OK;
 }
 
@@ -470,8 +472,12 @@ int hvvr::makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool devi
         step(v->occRank.resize(v->tileCount));
     }
     step(v->counters.resize(16));
+    // the per-stage events only time the stages: no system-scope fence, whose cache
+    // writeback and invalidation at every stage boundary would slow the frame they time
     for (auto& ev : v->stageEvents)
-        step(hipEventCreate(&ev));
+        step(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
+    for (auto& ev : v->kernelEvents)
+        step(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
     if (e == hipSuccess)
         e = hipMemsetAsync(v->image.ptr, 0, v->image.bytes(), st);
     // no hints yet (0), and zero work: neither may hold what a freed allocation left
@@ -560,7 +566,8 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     v->lastCameraToWorld = m;
     if (v->jitterSpin && jitter)
         subsampleOffsets(v->msaaRate, jitter[0] * 6.28f, v->offsets);
-    bool prof = c->profiling;
+    const bool prof = c->profiling == 1;
+    const bool timed = c->profiling != 0;
     auto mark = [&](int i) {
         if (prof)
             (void)hipEventRecord(v->stageEvents[i], st);
@@ -570,7 +577,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     mark(1);
     launchBlockTraverse(v, s, st);
     mark(2);
-    launchTileLists(v, s, st, scratch);
+    launchTileLists(v, s, st, scratch, timed ? v->kernelEvents : nullptr);
     mark(3);
     launchTileListFallback(v, s, st, scratch);
     if (v->dof)
@@ -592,6 +599,7 @@ static int renderOne(hvvr_gpu_view* v, hvvr_gpu_scene* s, const float* m16, cons
     mark(7);
     HVVR_HIP_CHECK(hipGetLastError());
     v->stagesRecorded = prof;
+    v->kernelTimed = timed;
     v->rendered = true;
     v->hitsValid = v->hitOutput;
     v->hintScene = s; // this frame writes every tile's configuration hint
@@ -1172,6 +1180,16 @@ int hvvr_gpu_view_stage_times(hvvr_gpu_view* v, float* ms, int maxCount) {
     for (int i = 0; i < n; i++)
         HVVR_HIP_CHECK(hipEventElapsedTime(&ms[i], v->stageEvents[i], v->stageEvents[i + 1]));
     return n;
+}
+
+int hvvr_gpu_view_tile_list_kernel_time(hvvr_gpu_view* v, float* ms) {
+    if (!v || !ms)
+        return fail("null argument");
+    if (!v->kernelTimed)
+        return fail("no profiled frame: enable profiling on the context before rendering", HVVR_GPU_ERR_STATE);
+    HVVR_HIP_CHECK(hipEventSynchronize(v->kernelEvents[1]));
+    HVVR_HIP_CHECK(hipEventElapsedTime(ms, v->kernelEvents[0], v->kernelEvents[1]));
+    return HVVR_GPU_OK;
 }
 
 } // extern "C"

@@ -89,7 +89,9 @@ struct FoveatedState {
 struct hvvr_gpu_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool profiling = false; // views record per-stage HIP events on the stream
+    // 1: views record per-stage HIP events on the stream and time their standard tile-list
+    // kernel's dispatch; 2: only the dispatch timing; 0: neither
+    int profiling = 0;
     // batched renders: views fan out from `stream` onto these and join back
     hipStream_t viewStreams[hvvr::kViewStreams] = {};
     hipEvent_t fork = nullptr;
@@ -185,6 +187,10 @@ struct hvvr_gpu_view {
     hvvr::FoveatedState fov;
     hipEvent_t stageEvents[hvvr::kStageCount + 1] = {};
     bool stagesRecorded = false;
+    // start and end of the standard tile-list kernel's dispatch (profiling levels 1 and 2):
+    // recorded by the dispatch itself, no extra packets on the stream
+    hipEvent_t kernelEvents[2] = {};
+    bool kernelTimed = false;
     bool rendered = false;
     // own-stream mode (hvvr_gpu_view_set_own_stream): renders and reads go to `ownStream`,
     // after the scene's `ready` event; `done` marks the end of the last queued work
@@ -193,6 +199,9 @@ struct hvvr_gpu_view {
     hipStream_t lastStream = nullptr; // the stream of the last render
     ~hvvr_gpu_view() {
         for (auto& ev : stageEvents)
+            if (ev)
+                (void)hipEventDestroy(ev);
+        for (auto& ev : kernelEvents)
             if (ev)
                 (void)hipEventDestroy(ev);
         if (done)
@@ -224,9 +233,10 @@ int makeView(hvvr_gpu_context* ctx, const hvvr_gpu_view_desc* d, bool deviceSetu
 void launchSceneRefit(hvvr_gpu_scene* s, hipStream_t stream);
 void launchFrustumSetup(hvvr_gpu_view* v, const matrix4x4& m, hipStream_t stream);
 void launchBlockTraverse(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream);
-// `scratch` null: the view's own record scratch and tile queues
+// `scratch` null: the view's own record scratch and tile queues; `timing` non-null: the
+// standard kernel's dispatch records its start and end in timing[0] and timing[1]
 void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
-                     const TileListScratch* scratch = nullptr);
+                     const TileListScratch* scratch = nullptr, hipEvent_t* timing = nullptr);
 void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream,
                             const TileListScratch* scratch = nullptr);
 void launchOccupiedRank(hvvr_gpu_view* v, hipStream_t stream);

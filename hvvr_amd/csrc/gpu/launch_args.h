@@ -51,7 +51,7 @@ struct TileListArgs {
     unsigned long long* counters;
     uint32_t stackLimit;
     uint32_t* recScratch; // per-workgroup record scratch of the queued configurations
-    // per tile, the queued configuration (1..3, 4 = the walk) whose capacity the tile's
+    // per tile, the queued configuration (1..4, 5 = the walk) whose capacity the tile's
     // last list on this scene needed; the queued kernels send a tile with a higher hint
     // straight to that queue (every configuration builds the same list, so a hint only
     // saves work)

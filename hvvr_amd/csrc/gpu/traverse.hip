@@ -18,6 +18,8 @@
 //                             LDS (count pass, allocation, emit pass)
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "device_common.h"
 #include "gpu_internal.h"
 #include "launch_args.h"
@@ -1245,14 +1247,19 @@ static TileListArgs tileListArgsWith(const hvvr_gpu_view* v, const hvvr_gpu_scen
     return g;
 }
 
-void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream, const TileListScratch* scratch) {
+void launchTileLists(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream_t stream, const TileListScratch* scratch,
+                     hipEvent_t* timing) {
     const LaunchShape shape;
     const uint32_t T = v->tileCount;
     const uint32_t large = scratch ? scratch->groups[0] : std::min(shape.largeGroups, uint32_t(kMaxLargeGroups));
     const uint32_t huge = scratch ? scratch->groups[1] : std::min(shape.hugeGroups, uint32_t(kMaxHugeGroups));
     const uint32_t giant = scratch ? scratch->groups[2] : std::min(shape.giantGroups, uint32_t(kMaxGiantGroups));
     const uint32_t colossal = scratch ? scratch->groups[3] : uint32_t(kMaxColossalGroups);
-    tileListKernel<<<T, 64, 0, stream>>>(tileListArgsWith(v, s, 0, scratch));
+    if (timing)
+        hipExtLaunchKernelGGL(tileListKernel, dim3(T), dim3(64), 0, stream, timing[0], timing[1], 0,
+                              tileListArgsWith(v, s, 0, scratch));
+    else
+        tileListKernel<<<T, 64, 0, stream>>>(tileListArgsWith(v, s, 0, scratch));
     // 16 KB of LDS: ten workgroups per CU; 36 KB: four; 80 KB: two; 160 KB: one
     tileListQueueKernel<TileListLarge><<<clampGroups(large, T), 64, 0, stream>>>(tileListArgsWith(v, s, 1, scratch));
     tileListQueueKernel<TileListHuge><<<clampGroups(huge, T), 64, 0, stream>>>(tileListArgsWith(v, s, 2, scratch));

@@ -49,7 +49,9 @@ int hvvr_gpu_context_create(int device, hvvr_gpu_context** out);
 int hvvr_gpu_context_destroy(hvvr_gpu_context* ctx);
 int hvvr_gpu_context_synchronize(hvvr_gpu_context* ctx);
 /* 1 = every render records per-stage HIP events on the stream (no synchronisation), read
- * back with hvvr_gpu_view_stage_times */
+ * back with hvvr_gpu_view_stage_times, and times its standard tile-list kernel's dispatch
+ * (hvvr_gpu_view_tile_list_kernel_time); 2 = only the dispatch timing, which adds no packet
+ * to the stream and leaves the frame's own timing alone; 0 = off. Other values fail. */
 int hvvr_gpu_context_set_profiling(hvvr_gpu_context* ctx, int enable);
 
 /* ---- scene: replaces GPUSceneState (raycaster/gpu_scene_state.h:31-73) ---- */
@@ -358,6 +360,11 @@ int hvvr_gpu_view_get_stats(hvvr_gpu_view* view, hvvr_gpu_view_stats* stats);
  * did not fit in LDS, [4] intersect, [5] resolve, [6] remap (foveated: polar remap and
  * reconstruction). Returns the count. */
 int hvvr_gpu_view_stage_times(hvvr_gpu_view* view, float* ms, int maxCount);
+/* No counterpart in the reference. GPU milliseconds of the view's standard tile-list kernel
+ * (its one launch over every tile, without the queued configurations) in the last frame
+ * rendered with profiling level 1 or 2, from HIP events the kernel's dispatch records
+ * (waits for them). */
+int hvvr_gpu_view_tile_list_kernel_time(hvvr_gpu_view* view, float* ms);
 
 #ifdef __cplusplus
 }

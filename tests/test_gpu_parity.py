@@ -133,6 +133,34 @@ def test_render_is_deterministic(H, ctx, scene2, view2_small):
     assert np.array_equal(c, gv.read_sample_colors())
 
 
+def test_profiling_levels(H, ctx, scene2, view2_small):
+    # level 1: per-stage events and the tile-list kernel's own dispatch timing; level 2: only
+    # the dispatch timing (stage times are then refused); 0: neither; results unchanged
+    gs, gv = _render(H, ctx, scene2, view2_small, 4)
+    want = gv.read_gbuffer()
+    try:
+        ctx.set_profiling(1)
+        gv.render(gs)
+        stages = gv.stage_times()
+        k1 = gv.tile_list_kernel_ms()
+        assert 0.0 < k1 <= stages[2] * 1.05 + 0.01, (k1, stages)
+        ctx.set_profiling(2)
+        gv.render(gs)
+        assert gv.tile_list_kernel_ms() > 0.0
+        with pytest.raises(H.HvvrError):
+            gv.stage_times()
+        with pytest.raises(H.HvvrError):
+            ctx.set_profiling(3)
+        ctx.set_profiling(0)
+        gv.render(gs)
+        with pytest.raises(H.HvvrError):
+            gv.tile_list_kernel_ms()
+    finally:
+        ctx.set_profiling(0)
+    got = gv.read_gbuffer()
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
 def test_animated_transforms_refit(H, O, ctx, scene1, view1_small):
     gs, gv = _render(H, ctx, scene1, view1_small, 4)
     rng = np.random.default_rng(7)
