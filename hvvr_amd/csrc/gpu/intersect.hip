@@ -487,7 +487,7 @@ void intersectBatchKernel(const IntersectArgs* __restrict__ all,
                                                             uint32_t swizzle) {
     const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
     const uint32_t v = batchView(start, n, unit);
-    intersectBody<A, false>(all[v], unit - start[v]);
+    intersectBody<A, false>(batchArgs(all, v), unit - start[v]);
 }
 
 IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m) {

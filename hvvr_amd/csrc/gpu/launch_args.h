@@ -186,6 +186,22 @@ void launchRemapBatch(const RemapArgs* all, const uint32_t* start, uint32_t view
                       hipStream_t stream);
 
 #ifdef __HIPCC__
+// One view's arguments from a batch's argument array, read as constant memory: invariant
+// scalar loads, which the register allocator repeats where it needs a field instead of
+// keeping the whole struct live (read through a plain pointer, the batch tile-list kernel
+// took 121 VGPRs and 4 waves per SIMD where its single-view twin takes 92 and 5).
+// (Address spaces exist only in the device compilation; the host pass, which never runs
+// this, reads the plain pointer.)
+template <class T>
+__device__ inline T batchArgs(const T* all, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstT = const __attribute__((address_space(4))) T;
+    return ((ConstT*)all)[v];
+#else
+    return all[v];
+#endif
+}
+
 // the view whose work units [start[v], start[v+1]) hold `unit` (wave-uniform search)
 __device__ inline uint32_t batchView(const uint32_t* __restrict__ start, uint32_t viewCount, uint32_t unit) {
     uint32_t lo = 0, hi = viewCount;

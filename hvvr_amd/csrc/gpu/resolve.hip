@@ -456,7 +456,7 @@ __global__ void remapKernel(RemapArgs a) {
 __global__ __launch_bounds__(128) void remapBatchKernel(const RemapArgs* __restrict__ all,
                                                         const uint32_t* __restrict__ start, uint32_t n) {
     const uint32_t v = batchView(start, n, blockIdx.x);
-    const RemapArgs& a = all[v];
+    const RemapArgs a = batchArgs(all, v);
     const uint32_t local = blockIdx.x - start[v], bx = (a.width + 15) / 16;
     remapBody(a, (local % bx) * 16 + (threadIdx.x & 15u), (local / bx) * 8 + (threadIdx.x >> 4));
 }

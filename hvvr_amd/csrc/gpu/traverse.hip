@@ -94,7 +94,7 @@ __global__ __launch_bounds__(128) void frustumSetupKernel(FrustumSetupArgs a) {
 __global__ __launch_bounds__(128) void frustumSetupBatchKernel(const FrustumSetupArgs* __restrict__ all,
                                                                const uint32_t* __restrict__ start, uint32_t n) {
     const uint32_t v = batchView(start, n, blockIdx.x);
-    frustumSetupBody(all[v], (blockIdx.x - start[v]) * 128u + threadIdx.x);
+    frustumSetupBody(batchArgs(all, v), (blockIdx.x - start[v]) * 128u + threadIdx.x);
 }
 
 struct CullEntry {
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
                                                                const uint32_t* __restrict__ start, uint32_t n) {
     __shared__ CullEntry cull[BLOCK_STACK_SIZE + 4];
     const uint32_t v = batchView(start, n, blockIdx.x);
-    blockTraverseBody(all[v], cull, blockIdx.x - start[v]);
+    blockTraverseBody(batchArgs(all, v), cull, blockIdx.x - start[v]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(64) void tileListBatchKernel(const TileListArgs* __
     __shared__ TileListLds<TileListSmall> sh;
     const uint32_t unit = xcdUnit(blockIdx.x, gridDim.x, swizzle);
     const uint32_t v = batchView(start, n, unit);
-    tileListSmallOne(all[v], sh, unit - start[v], threadIdx.x);
+    tileListSmallOne(batchArgs(all, v), sh, unit - start[v], threadIdx.x);
 }
 
 // The tiles a smaller configuration queued, persistent workgroups: this workgroup takes
@@ -1036,7 +1036,7 @@ __global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __
     __shared__ TileWalkLds sh;
     uint32_t base = 0;
     for (uint32_t v = 0; v < n; v++) {
-        const TileWalkArgs& a = all[v];
+        const TileWalkArgs a = batchArgs(all, v);
         const uint32_t queued = uniform(uint32_t(a.counters[9]));
         for (uint32_t qi = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x; qi < queued; qi += gridDim.x) {
             waveLdsSync();
