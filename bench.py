@@ -405,11 +405,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--pipeline", choices=["plain", "batched", "graph"], default="plain",
-                    help="plain: one view after the other on one stream; batched: both eyes in one "
-                         "submission (hvvr_gpu_render_views), overlapping on separate streams; graph: both "
-                         "eyes as one batch (each kernel launched once for both), captured as a HIP graph "
-                         "and replayed, launch shape autotuned")
+    ap.add_argument("--pipeline", choices=["plain", "batched", "graph"], default="batched",
+                    help="batched (the default): both eyes in one submission (hvvr_gpu_render_views, what "
+                         "Raycaster::render issues for its cameras), overlapping on separate streams; plain: "
+                         "one view after the other on one stream; graph: both eyes as one batch (each kernel "
+                         "launched once for both), captured as a HIP graph and replayed, launch shape autotuned")
     ap.add_argument("--shading", type=int, choices=[32, 16], default=32,
                     help="shading precision: 32 = FP32 (the oracle's arithmetic), 16 = packed-FP16 colour terms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
