@@ -25,15 +25,27 @@
 
 namespace hvvr {
 
+// The shading's divisions, normalisations and powers use the hardware's approximate forms
+// (v_rcp_f32, v_rsq_f32, v_log_f32 / v_exp_f32, about 1 ulp): the reference builds its
+// kernels with fast math (raycaster.vcxproj: FastMath), where they are approximate too.
+// Colours stay within 1/255 of the oracle's correctly rounded FP32 (and of FP64); the
+// intersection and the depth and barycentric readbacks keep exact divisions.
+__device__ inline float rcpFast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline vector3 normalizeFast(const vector3& v) { return v * __builtin_amdgcn_rsqf(dot(v, v)); }
+// linearToSRgb (util_graphics.h) with the power as exp2(log2(x) / 2.4)
+__device__ inline float linearToSRgbFast(float x) {
+    return (x <= 0.0031308f) ? (x * 12.92f)
+                             : (__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 2.4f)) * 1.055f - 0.055f);
+}
 
 __device__ inline vector3 ggxSpecular(vector3 N, vector3 L, vector3 V, float roughness, vector3 F0) {
     // GGX_SchlickFresnel_SchlickSmithGGX (shading_helpers.h:89-112)
     float alpha = roughness * roughness;
-    vector3 H = normalize(L + V);
+    vector3 H = normalizeFast(L + V);
     float alpha2 = alpha * alpha;
     float NdotH = saturate(dot(N, H));
     float q = NdotH * NdotH * (alpha2 - 1.0f) + 1.0f;
-    float D = alpha2 / (3.141592653589f * (q * q));
+    float D = alpha2 * rcpFast(3.141592653589f * (q * q));
     float LdotH = saturate(dot(L, H));
     float x = 1.0f - LdotH;
     float x2 = x * x;
@@ -42,7 +54,7 @@ __device__ inline vector3 ggxSpecular(vector3 N, vector3 L, vector3 V, float rou
     float NdotL = saturate(dot(N, L));
     float NdotV = saturate(dot(N, V));
     float k = alpha * 0.5f;
-    float vis = (1.0f / (NdotL * (1.0f - k) + k)) * (1.0f / (NdotV * (1.0f - k) + k));
+    float vis = rcpFast(NdotL * (1.0f - k) + k) * rcpFast(NdotV * (1.0f - k) + k);
     return ((F * D) * vis) * 0.25f;
 }
 
@@ -131,11 +143,11 @@ template <class In>
 __device__ inline vector3 ggxShade(const In& in, vector3 pos, vector3 nrm, vector3 camPos,
                                    const LightingEnvironment& env) {
     // GGXShade (shading.h:84-161)
-    vector3 normal = normalize(nrm);
+    vector3 normal = normalizeFast(nrm);
     if (in.model() == ShadingModelEmissive)
         return in.emissiveColor();
     vector3 base = in.base();
-    vector3 V = normalize(camPos - pos);
+    vector3 V = normalizeFast(camPos - pos);
     vector3 F0 = in.F0();
     float roughness = in.roughness();
     vector3 rad = in.emissive();
@@ -151,8 +163,9 @@ __device__ inline vector3 ggxShade(const In& in, vector3 pos, vector3 nrm, vecto
             break;
         const LightPoint& l = env.pointLights[i];
         vector3 delta = l.Position - pos;
-        float r = length(delta);
-        vector3 L = normalize(delta);
+        const float d2 = dot(delta, delta), inv = __builtin_amdgcn_rsqf(d2);
+        float r = d2 * inv;
+        vector3 L = delta * inv;
         float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
         rad += pbrBrdf(L, normal, V, base, roughness, F0, l.Color * fdist);
     }
@@ -161,8 +174,9 @@ __device__ inline vector3 ggxShade(const In& in, vector3 pos, vector3 nrm, vecto
             break;
         const LightSpot& l = env.spotLights[i];
         vector3 delta = l.Position - pos;
-        float r = length(delta);
-        vector3 L = normalize(delta);
+        const float d2 = dot(delta, delta), inv = __builtin_amdgcn_rsqf(d2);
+        float r = d2 * inv;
+        vector3 L = delta * inv;
         float fdist = saturate((l.FalloffEnd - r) * l.FalloffScale);
         vector3 lit = pbrBrdf(L, normal, V, base, roughness, F0, l.Color * fdist);
         float cosA = dot(l.Direction, -L);
@@ -179,13 +193,9 @@ __device__ inline vector3 ggxShade(const In& in, vector3 pos, vector3 nrm, vecto
 // FP32: D * vis spans many decades (up to ~1e7 for a glossy highlight seen edge-on). The
 // two colour vectors, bounded by the light colour, are formed with packed FP16 operations
 // (v_pk_mul_f16 / v_pk_add_f16); the radiance sum stays FP32. The scalar terms use the
-// hardware reciprocal and reciprocal square root (about 1 ulp in FP32, far below the FP16
-// colour terms' resolution) in place of correctly rounded divisions and square roots:
-// three divisions and a normalisation per light are most of the shading's instructions.
+// hardware reciprocal and reciprocal square root, as the FP32 path does.
 typedef _Float16 half4v __attribute__((ext_vector_type(4)));
 
-__device__ inline float rcpFast(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline vector3 normalizeFast(const vector3& v) { return v * __builtin_amdgcn_rsqf(dot(v, v)); }
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 __device__ inline half4v toHalf4(const vector3& v) {
@@ -265,7 +275,7 @@ __device__ inline vector3 ggxShadeHalf(const In& in, vector3 pos, vector3 nrm, v
 __device__ inline float acesFilm(float x) {
     // ACESFilm (shading_helpers.h:158-170)
     const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
-    return saturate((x * (a * x + b)) / (x * (c * x + d) + e));
+    return saturate((x * (a * x + b)) * rcpFast(x * (c * x + d) + e));
 }
 
 // Tex: the scene has textures (a variant of its own: the texture reads' registers would
@@ -378,9 +388,9 @@ __device__ inline void resolveBody(const ResolveArgs& args, uint32_t tile) {
         result += c * float(__popc(mask));
     }
     result = result * (1.0f / float(A));
-    float r = linearToSRgb(acesFilm(result.x));
-    float gch = linearToSRgb(acesFilm(result.y));
-    float b = linearToSRgb(acesFilm(result.z));
+    float r = linearToSRgbFast(acesFilm(result.x));
+    float gch = linearToSRgbFast(acesFilm(result.y));
+    float b = linearToSRgbFast(acesFilm(result.z));
     uint32_t ri = uint32_t(r * 255.0f), gi = uint32_t(gch * 255.0f), bi = uint32_t(b * 255.0f);
     args.sampleResults[sample] = ri | (gi << 8) | (bi << 16) | 0xff000000u;
 
