@@ -157,11 +157,23 @@ constexpr float kUlp = 0x1p-24f;
 
 __device__ inline float l1(const vector3& v) { return fabsf(v.x) + fabsf(v.y) + fabsf(v.z); }
 
+// The wave's largest value, wave-uniform; every lane active. Within each row of 16 lanes,
+// four DPP row shifts leave the row's maximum in its last lane; row_bcast:15 and :31 carry
+// rows 0-2 into lane 63, which readlane broadcasts. (A butterfly of shuffles took six LDS
+// round trips per call.)
+template <int Ctrl, int RowMask>
+__device__ inline float dppMaxStep(float v) {
+    const int b = int(fbits(v));
+    return fmaxf(v, bitsf(uint32_t(__builtin_amdgcn_update_dpp(b, b, Ctrl, RowMask, 0xf, false))));
+}
 __device__ inline float waveMaxF(float v) {
-#pragma unroll
-    for (int off = 32; off; off >>= 1)
-        v = fmaxf(v, __shfl_xor(v, off));
-    return v;
+    v = dppMaxStep<0x111, 0xf>(v); // row_shr:1
+    v = dppMaxStep<0x112, 0xf>(v); // row_shr:2
+    v = dppMaxStep<0x114, 0xf>(v); // row_shr:4
+    v = dppMaxStep<0x118, 0xf>(v); // row_shr:8
+    v = dppMaxStep<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+    v = dppMaxStep<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+    return bitsf(uint32_t(__builtin_amdgcn_readlane(int(fbits(v)), 63)));
 }
 
 // every lane of the wave active
