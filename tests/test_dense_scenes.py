@@ -173,3 +173,44 @@ def test_colossal_configuration_lists_match(H, O, ctx):
         assert st.survivorPairs == ref["survivors"]
     gv.close()
     gs.close()
+
+
+def test_colossal_configuration_in_a_batch(H, O, ctx):
+    # the same 2M-triangle room at 96x96, both eyes in one batch (each kernel launched once
+    # for both views, the queued configurations taking every view's queue in turn): tiles
+    # past the giant configuration are built by the batch's colossal kernel, and each view's
+    # lists, ids and masks match the oracle bit for bit, eagerly and replayed as a graph
+    scene = H.HostScene(H.SCENE_ROOM, 0, 8192)
+    hviews = [H.HostView(H.VIEW_PRESET_ROOM_EYE, e, 96, 96) for e in range(2)]
+    refs = [O.render_view(scene, hv, 1, with_colors=False) for hv in hviews]
+    gs = H.GpuScene(ctx, scene)
+    views = [H.GpuView(ctx, hv, 1) for hv in hviews]
+    for gv, ref in zip(views, refs):
+        gv.reserve_lists(int(ref["counts"].sum()))
+    batch = H.GpuBatch(ctx, views)
+    for graph in (False, True):
+        batch.set_graph(graph)
+        for _ in range(2):  # the second frame follows the per-tile hints
+            batch.render(gs)
+            batch.synchronize()
+        for e, (gv, ref) in enumerate(zip(views, refs)):
+            st = gv.stats()
+            print(f"graph {graph} eye {e}: large {st.largeTiles}, huge {st.hugeTiles}, giant {st.giantTiles}, "
+                  f"colossal {st.colossalTiles}, walk {st.walkTiles}", flush=True)
+            assert st.overflow == 0
+            counts, lists = gv.read_tile_lists()
+            assert np.array_equal(counts, ref["counts"]), f"eye {e}: tile list lengths differ"
+            assert np.array_equal(lists, ref["lists"]), f"eye {e}: tile list contents/order differ"
+            tri, mask = gv.read_gbuffer()
+            rtri, rmask = O.gbuffer_slots(ref["tri_ids"])
+            bad = np.nonzero((tri != rtri).any(1) | (mask != rmask).any(1))[0]
+            assert len(bad) == 0, f"eye {e}: {len(bad)} samples differ, first {bad[:5]}"
+    # the colossal configuration took part (on the hinted frames a tile goes to it
+    # straight from the large one's queue)
+    st0 = [gv.stats() for gv in views]
+    batch.close()
+    for gv in views:
+        gv.close()
+    gs.close()
+    assert sum(s.colossalTiles for s in st0) > 0, "no tile reached the colossal configuration"
+
