@@ -218,9 +218,11 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 // keeping the stack sorted by distance and emitting each visited node's surviving leaf
 // triangles in visit order. A node's test depends only on the tile frustum and the
 // node, not on when the walk reaches it, so the walk is split into three phases:
-//   A. tests: breadth-first over the nodes the walk will visit, 32 nodes per step (lane
-//      = node slot x child, each lane testing one child against every frustum axis,
-//      two node slots per lane group). Each visited node becomes an LDS record: its
+//   A. tests: breadth-first over the nodes the walk will visit, 16 x Slots nodes per step
+//      (lane = node slot x child, each lane testing one child against every frustum
+//      axis; two node slots per lane group in the standard configuration, whose VGPRs
+//      bound its occupancy, four in the queued ones, whose LDS does). Each visited node
+//      becomes an LDS record: its
 //      final child mask (coarse test, refined when a leaf child passes), its leaf mask,
 //      the leaf mask of its stack entry and the index of its first internal child's
 //      record; the internal children that pass are appended as new records, in child
@@ -272,23 +274,23 @@ __device__ inline uint32_t packMeta(uint32_t first, uint32_t mask, uint32_t leaf
 // reuse it tile after tile), which halves their LDS and doubles the workgroups a CU holds
 // (the walk is latency-bound: throughput follows the resident waves).
 struct TileListSmall {
-    enum : uint32_t { RecCap = 680, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0 };
+    enum : uint32_t { RecCap = 680, StackCap = REG_STACK_CAP, Packed = 1, LdsRecords = 1, Level = 0, Slots = 2 };
     using Stack = RegStack;
 };
 struct TileListLarge {
-    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0, LdsRecords = 0, Level = 1 };
+    enum : uint32_t { RecCap = 2048, StackCap = REG_STACK4_CAP, Packed = 0, LdsRecords = 0, Level = 1, Slots = 4 };
     using Stack = RegStack4;
 };
 struct TileListHuge {
-    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0, LdsRecords = 0, Level = 2 };
+    enum : uint32_t { RecCap = 4608, StackCap = REG_STACK8_CAP, Packed = 0, LdsRecords = 0, Level = 2, Slots = 4 };
     using Stack = RegStack8;
 };
 struct TileListGiant {
-    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 3 };
+    enum : uint32_t { RecCap = 10240, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 3, Slots = 4 };
     using Stack = RegStack16;
 };
 struct TileListColossal {
-    enum : uint32_t { RecCap = 20480, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 4 };
+    enum : uint32_t { RecCap = 20480, StackCap = REG_STACK16_CAP, Packed = 0, LdsRecords = 0, Level = 4, Slots = 4 };
     using Stack = RegStack16;
 };
 
@@ -445,93 +447,106 @@ __device__ bool tileListBody(const TileListArgs& g, TileListLds<Cfg>& sh, uint32
         const uint32_t s = lane >> 2, c = lane & 3u;
         const uint32_t axis = uint32_t(f->distanceAxis);
         const uint64_t groupBelow = (1ull << (4 * s)) - 1ull;
+        constexpr uint32_t S = Cfg::Slots; // records per lane group and step
         for (uint32_t head = 0; head < nrec && !overflow;) {
-            // this step tests records [head, end), two per lane group (so their node loads
+            // this step tests records [head, end), S per lane group (so their node loads
             // overlap); the children it appends start at nrec
-            const uint32_t end = head + 32 < nrec ? head + 32 : nrec;
-            const uint32_t rA = head + s, rB = head + 16 + s;
-            const bool validA = rA < end, validB = rB < end;
-            uint32_t nodeA = 0, entryA = 0, nodeB = 0, entryB = 0;
-            if (validA) {
-                nodeA = recNode(rA);
-                entryA = metaEntry(sh.recMeta[rA]);
-            }
-            if (validB) {
-                nodeB = recNode(rB);
-                entryB = metaEntry(sh.recMeta[rB]);
-            }
-            const ChildBox A = loadChildBox(nodes, nodeA, c, axis);
-            const ChildBox B = loadChildBox(nodes, nodeB, c, axis);
-            // (`&` rather than `&&` throughout: no branch per lane condition)
-            bool passA = validA & childCoarsePass(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
-            bool passB = validB & childCoarsePass(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
-            uint32_t maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
-            uint32_t maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
-            // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
-            const bool refineA = passA & ((maskA & A.leafMask) != 0u);
-            const bool refineB = passB & ((maskB & B.leafMask) != 0u);
-            const uint64_t needA = ballot(refineA), needB = ballot(refineB);
-            if ((needA | needB) != 0ull) {
-                const uint32_t nA = uint32_t(__builtin_popcountll(needA));
-                const uint32_t nB = uint32_t(__builtin_popcountll(needB));
-                if (nA + nB < 64u) {
-                    // Few boxes need the 12-axis test (typically a sixth of the lanes): move
-                    // them to lanes 0 .. nA + nB - 1 with forward permutes (every other lane
-                    // sends to lane 63, which then holds no needed box), test them in one
-                    // pass, and read each verdict back from the ballot.
-                    const uint32_t dA = refineA ? lanesBelow(needA) : 63u;
-                    const uint32_t dB = refineB ? nA + lanesBelow(needB) : 63u;
-                    auto gather = [&](float a, float b) {
-                        const uint32_t xa = uint32_t(__builtin_amdgcn_ds_permute(int(dA << 2), int(fbits(a))));
-                        const uint32_t xb = uint32_t(__builtin_amdgcn_ds_permute(int(dB << 2), int(fbits(b))));
-                        return bitsf(lane < nA ? xa : xb);
-                    };
-                    const float g0 = gather(A.b0, B.b0), g1 = gather(A.b1, B.b1), g2 = gather(A.b2, B.b2);
-                    const float g3 = gather(A.b3, B.b3), g4 = gather(A.b4, B.b4), g5 = gather(A.b5, B.b5);
-                    const bool cull = childRefineCull(f, g0, g1, g2, g3, g4, g5);
-                    const uint64_t culled = ballot(cull & (lane < nA + nB));
-                    passA &= !(refineA & (((culled >> dA) & 1ull) != 0ull));
-                    passB &= !(refineB & (((culled >> dB) & 1ull) != 0ull));
-                } else {
-                    const bool cullA = childRefineCull(f, A.b0, A.b1, A.b2, A.b3, A.b4, A.b5);
-                    const bool cullB = childRefineCull(f, B.b0, B.b1, B.b2, B.b3, B.b4, B.b5);
-                    passA &= !(refineA & cullA);
-                    passB &= !(refineB & cullB);
+            const uint32_t end = head + 16 * S < nrec ? head + 16 * S : nrec;
+            uint32_t recI[S], nodeR[S], entryR[S];
+            bool valid[S];
+#pragma unroll
+            for (uint32_t k = 0; k < S; k++) {
+                recI[k] = head + 16 * k + s;
+                valid[k] = recI[k] < end;
+                nodeR[k] = 0;
+                entryR[k] = 0;
+                if (valid[k]) {
+                    nodeR[k] = recNode(recI[k]);
+                    entryR[k] = metaEntry(sh.recMeta[recI[k]]);
                 }
             }
-            maskA = uint32_t(ballot(passA) >> (4 * s)) & 0xfu;
-            maskB = uint32_t(ballot(passB) >> (4 * s)) & 0xfu;
+            ChildBox X[S];
+#pragma unroll
+            for (uint32_t k = 0; k < S; k++)
+                X[k] = loadChildBox(nodes, nodeR[k], c, axis);
+            // (`&` rather than `&&` throughout: no branch per lane condition)
+            bool pass[S], refine[S];
+            uint64_t need[S];
+            uint32_t nNeed = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < S; k++) {
+                pass[k] = valid[k] & childCoarsePass(f, X[k].b0, X[k].b1, X[k].b2, X[k].b3, X[k].b4, X[k].b5);
+                const uint32_t m = uint32_t(ballot(pass[k]) >> (4 * s)) & 0xfu;
+                // refine only nodes with a passing leaf child (traverse_avx.h:447-452)
+                refine[k] = pass[k] & ((m & X[k].leafMask) != 0u);
+                need[k] = ballot(refine[k]);
+                nNeed += uint32_t(__builtin_popcountll(need[k]));
+            }
+            if (nNeed != 0u) {
+                if (nNeed < 64u) {
+                    // Few boxes need the 12-axis test (typically a sixth of the lanes): move
+                    // them to lanes 0 .. nNeed - 1 with forward permutes (every other lane
+                    // sends to lane 63, which then holds no needed box), test them in one
+                    // pass, and read each verdict back from the ballot.
+                    uint32_t d[S];
+                    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f, g4 = 0.0f, g5 = 0.0f;
+                    uint32_t base = 0;
+#pragma unroll
+                    for (uint32_t k = 0; k < S; k++) {
+                        const uint32_t nk = uint32_t(__builtin_popcountll(need[k]));
+                        d[k] = refine[k] ? base + lanesBelow(need[k]) : 63u;
+                        const bool mine = lane >= base && lane < base + nk; // slot k's boxes land here
+                        auto take = [&](float& gq, float v) {
+                            const uint32_t x = uint32_t(__builtin_amdgcn_ds_permute(int(d[k] << 2), int(fbits(v))));
+                            gq = mine ? bitsf(x) : gq;
+                        };
+                        take(g0, X[k].b0);
+                        take(g1, X[k].b1);
+                        take(g2, X[k].b2);
+                        take(g3, X[k].b3);
+                        take(g4, X[k].b4);
+                        take(g5, X[k].b5);
+                        base += nk;
+                    }
+                    const bool cull = childRefineCull(f, g0, g1, g2, g3, g4, g5);
+                    const uint64_t culled = ballot(cull & (lane < nNeed));
+#pragma unroll
+                    for (uint32_t k = 0; k < S; k++)
+                        pass[k] &= !(refine[k] & (((culled >> d[k]) & 1ull) != 0ull));
+                } else {
+#pragma unroll
+                    for (uint32_t k = 0; k < S; k++) {
+                        const bool cullK = childRefineCull(f, X[k].b0, X[k].b1, X[k].b2, X[k].b3, X[k].b4, X[k].b5);
+                        pass[k] &= !(refine[k] & cullK);
+                    }
+                }
+            }
             // internal children are pushed only from entries without a leaf mask
-            // (traverse_avx.h:476-483)
-            const bool innerA = passA && ((A.leafMask >> c) & 1u) == 0u && entryA == 0u;
-            const bool innerB = passB && ((B.leafMask >> c) & 1u) == 0u && entryB == 0u;
-            const uint64_t biA = ballot(innerA), biB = ballot(innerB);
-            const uint32_t cntA = uint32_t(__builtin_popcountll(biA));
-            const uint32_t slotA = nrec + lanesBelow(biA);
-            const uint32_t slotB = nrec + cntA + lanesBelow(biB);
-            if (innerA && slotA < Cfg::RecCap) {
-                recNode(slotA) = nodeA + A.childOffset;
-                sh.recKey[slotA] = fbits(A.key);
-                sh.recMeta[slotA] = 0u;
+            // (traverse_avx.h:476-483); slot k's children follow those of slots below it
+            uint32_t before = 0;
+            bool badKey = false;
+#pragma unroll
+            for (uint32_t k = 0; k < S; k++) {
+                const uint32_t maskK = uint32_t(ballot(pass[k]) >> (4 * s)) & 0xfu;
+                const bool inner = pass[k] && ((X[k].leafMask >> c) & 1u) == 0u && entryR[k] == 0u;
+                const uint64_t bi = ballot(inner);
+                const uint32_t slot = nrec + before + lanesBelow(bi);
+                if (inner && slot < Cfg::RecCap) {
+                    recNode(slot) = nodeR[k] + X[k].childOffset;
+                    sh.recKey[slot] = fbits(X[k].key);
+                    sh.recMeta[slot] = 0u;
+                }
+                if (valid[k] && c == 0) {
+                    const uint32_t first = nrec + before + uint32_t(__builtin_popcountll(bi & groupBelow));
+                    sh.recMeta[recI[k]] = packMeta(first, maskK, X[k].leafMask, entryR[k]);
+                }
+                before += uint32_t(__builtin_popcountll(bi));
+                badKey |= inner && !(fabsf(X[k].key) <= FLT_MAX);
             }
-            if (innerB && slotB < Cfg::RecCap) {
-                recNode(slotB) = nodeB + B.childOffset;
-                sh.recKey[slotB] = fbits(B.key);
-                sh.recMeta[slotB] = 0u;
-            }
-            if (validA && c == 0) {
-                const uint32_t first = nrec + uint32_t(__builtin_popcountll(biA & groupBelow));
-                sh.recMeta[rA] = packMeta(first, maskA, A.leafMask, entryA);
-            }
-            if (validB && c == 0) {
-                const uint32_t first = nrec + cntA + uint32_t(__builtin_popcountll(biB & groupBelow));
-                sh.recMeta[rB] = packMeta(first, maskB, B.leafMask, entryB);
-            }
-            nrec += cntA + uint32_t(__builtin_popcountll(biB));
+            nrec += before;
             head = end;
             waveLdsSync();
-            if (nrec > Cfg::RecCap ||
-                ballot((innerA && !(fabsf(A.key) <= FLT_MAX)) || (innerB && !(fabsf(B.key) <= FLT_MAX))) != 0ull) {
+            if (nrec > Cfg::RecCap || ballot(badKey) != 0ull) {
                 overflow = true;
                 break;
             }
