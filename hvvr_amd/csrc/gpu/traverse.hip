@@ -222,11 +222,10 @@ __global__ __launch_bounds__(64) void blockTraverseBatchKernel(const BlockTraver
 //      (lane = node slot x child, each lane testing one child against every frustum
 //      axis; two node slots per lane group in the standard configuration, whose VGPRs
 //      bound its occupancy, four in the queued ones, whose LDS does). Each visited node
-//      becomes an LDS record: its
-//      final child mask (coarse test, refined when a leaf child passes), its leaf mask,
-//      the leaf mask of its stack entry and the index of its first internal child's
-//      record; the internal children that pass are appended as new records, in child
-//      order, with their distance keys.
+//      becomes an LDS record: its final child mask (coarse test, refined when a leaf
+//      child passes), its leaf mask, the leaf mask of its stack entry and the index of
+//      its first internal child's record; the internal children that pass are appended
+//      as new records, in child order, with their distance keys.
 //   B. order: the reference's stack walk replayed over the records, with the stack in
 //      one VGPR pair: pops, the single-child shortcut, the sorted insertion (one ballot
 //      compares every entry), and the sequence of records whose leaves are emitted.
