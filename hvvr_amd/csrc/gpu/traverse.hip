@@ -150,9 +150,16 @@ __device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* 
         top++;
     };
     for (;;) {
+        // Child c's size estimate, distance and offset, loaded with the test operands: the
+        // inserts take child k's from lane k, with no memory round trip each.
+        const BVHNode* __restrict__ nd = nodes + node;
+        const float sizeA = nd->bounds[sizeAxis][c], sizeB = nd->bounds[sizeAxis + 1][c];
+        const float distC = -nd->bounds[axis][c];
+        const uint32_t offC = nd->data[1 + c];
+        const uint32_t leafMask = nd->leafMask;
         NodeLanes nl;
         loadNodeLanes(nodes, node, c, projAxis, nl);
-        const uint32_t leafMask = nodes[node].leafMask;
+        const float sizeC = ftz(sizeA + sizeB);
         uint32_t mask = waveTestChildren(w, nl, role);
         mask = waveTestRefine(w, nl, role, mask);
         if (mask) {
@@ -164,8 +171,8 @@ __device__ inline void blockTraverseBody(const BlockTraverseArgs& a, CullEntry* 
             for (int k = 0; k < 4; k++) {
                 if (!(mask & (1u << k)))
                     continue;
-                const float size = ftz(nodes[node].bounds[sizeAxis][k] + nodes[node].bounds[sizeAxis + 1][k]);
-                insert(daz(fbits(size)), fbits(size), -nodes[node].bounds[axis][k], node + nodes[node].data[1 + k]);
+                const uint32_t size = readLane(fbits(sizeC), uint32_t(k));
+                insert(daz(size), size, bitsf(readLane(fbits(distC), uint32_t(k))), node + readLane(offC, uint32_t(k)));
             }
         }
         if (top >= BLOCK_STACK_SIZE - 4 || top == 0 || readLane(sBits, top - 1u) <= 0xfu)

@@ -81,39 +81,38 @@ __device__ inline void loadNodeLanes(const BVHNode* nodes, uint32_t node, uint32
     nl.b5 = base[5 * 4 + c];
     nl.projMax = base[(2 * projAxis) * 4 + c];
     nl.projNegMin = base[(2 * projAxis + 1) * 4 + c];
+    // held in every lane here: otherwise the compiler moves these two loads into the
+    // projection lanes' branch of the test, where they cost a memory round trip of their own
+    asm volatile("" : "+v"(nl.projMax), "+v"(nl.projNegMin));
 }
 
 __device__ inline uint32_t laneProjAxis(uint32_t role) {
     return (role >= 4 && role < 7) ? role - 4 : 0u;
 }
 
-// testBVHNodeChildren (traverse_avx.h:104-164), one pass over 28 lanes
+// testBVHNodeChildren (traverse_avx.h:104-164), one pass over 28 lanes. Every lane
+// evaluates both tests and keeps its role's verdict: with selects instead of branches the
+// projection operands load with the others, not behind a branch of their own.
 __device__ inline uint32_t waveTestChildren(const WaveFrustum& w, const NodeLanes& n, uint32_t role) {
-    bool cull = false;
-    if (role < 4) {
-        float dmax = fmaf(w.pn0, n.b0, fmaf(w.pn1, n.b2, w.pn2 * n.b4));
-        float dneg = fmaf(w.pp0, n.b1, fmaf(w.pp1, n.b3, w.pp2 * n.b5));
-        cull = !((dmax + dneg) <= w.pw);
-    } else if (role < 7) {
-        cull = !(n.projMax >= w.projLo && n.projNegMin >= w.projHi);
-    }
+    const float dmax = fmaf(w.pn0, n.b0, fmaf(w.pn1, n.b2, w.pn2 * n.b4));
+    const float dneg = fmaf(w.pp0, n.b1, fmaf(w.pp1, n.b3, w.pp2 * n.b5));
+    const bool planeCull = !((dmax + dneg) <= w.pw);
+    const bool projCull = !((n.projMax >= w.projLo) & (n.projNegMin >= w.projHi));
+    const bool cull = role < 4 ? planeCull : (role < 7 ? projCull : false);
     return 0xfu & ~foldChildren(ballot(cull));
 }
 
 // testBVHNodeChildrenRefine (traverse_avx.h:166-217), one pass over 48 lanes
 __device__ inline uint32_t waveTestRefine(const WaveFrustum& w, const NodeLanes& n, uint32_t role, uint32_t prev) {
-    bool cull = false;
-    if (role < 12) {
-        float px = w.ex > 0.0f ? n.b0 : -n.b1;
-        float py = w.ey > 0.0f ? n.b2 : -n.b3;
-        float pz = w.ez > 0.0f ? n.b4 : -n.b5;
-        float nx = w.ex > 0.0f ? -n.b1 : n.b0;
-        float ny = w.ey > 0.0f ? -n.b3 : n.b2;
-        float nz = w.ez > 0.0f ? -n.b5 : n.b4;
-        float maxBox = fmaf(w.ex, px, fmaf(w.ey, py, w.ez * pz));
-        float minBox = fmaf(w.ex, nx, fmaf(w.ey, ny, w.ez * nz));
-        cull = (maxBox < w.eMin) || (w.eMax < minBox);
-    }
+    const float px = w.ex > 0.0f ? n.b0 : -n.b1;
+    const float py = w.ey > 0.0f ? n.b2 : -n.b3;
+    const float pz = w.ez > 0.0f ? n.b4 : -n.b5;
+    const float nx = w.ex > 0.0f ? -n.b1 : n.b0;
+    const float ny = w.ey > 0.0f ? -n.b3 : n.b2;
+    const float nz = w.ez > 0.0f ? -n.b5 : n.b4;
+    const float maxBox = fmaf(w.ex, px, fmaf(w.ey, py, w.ez * pz));
+    const float minBox = fmaf(w.ex, nx, fmaf(w.ey, ny, w.ez * nz));
+    const bool cull = (role < 12) & ((maxBox < w.eMin) | (w.eMax < minBox));
     return prev & ~foldChildren(ballot(cull));
 }
 
