@@ -898,7 +898,9 @@ __global__ __launch_bounds__(64) void tileListQueueBatchKernel(const TileListArg
 // ballot per 64 entries, then a 64-wide move). The current node's entry stays on top
 // while it is tested, as in the reference: a node with several passing internal
 // children replaces its entry by them, each at its sorted position; a single one is
-// visited directly. Count pass (emit = 0) and emit pass (emit = 1) walk alike.
+// visited directly. Count pass (emit = 0) and emit pass (emit = 1) walk alike, one after
+// the other in the same wave: the count pass claims the list space and hands its start to
+// the emit pass in `base`.
 // ---------------------------------------------------------------------------------
 struct TileWalkLds {
     float t[TILE_STACK_SIZE];
@@ -906,7 +908,7 @@ struct TileWalkLds {
 };
 
 __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint32_t tile, int emit,
-                                   uint32_t lane) {
+                                   uint32_t lane, uint64_t& base) {
     const BVHNode* __restrict__ nodes = a.nodes;
     const uint64_t capacity = a.capacity;
     unsigned long long* __restrict__ counters = a.counters;
@@ -943,7 +945,7 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
             irregular = ballot((lane < n0 && !(fabsf(t0) <= FLT_MAX)) || (lane + 1u < n0 && t0 < above)) != 0ull;
             stackInit(st, t0, m0, n0, lane);
         }
-        const uint64_t outBase = emit ? a.offsets[tile] : 0;
+        const uint64_t outBase = emit ? base : 0;
         uint32_t count = 0, visits = 0, maxSize = n0;
         bool listOverflow = false, stackOverflow = false;
         const int axis = f.distanceAxis;
@@ -1011,11 +1013,12 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
             }
         }
         const bool overflow = stackOverflow || ballot(listOverflow) != 0ull;
+        unsigned long long start = 0;
         if (lane == 0) {
             if (!emit) {
                 // the count pass claims the tile's list space; the emit pass fills it
                 counts[tile] = count;
-                const unsigned long long start = atomicAdd(&counters[3], (unsigned long long)count);
+                start = atomicAdd(&counters[3], (unsigned long long)count);
                 a.offsets[tile] = uint32_t(start);
                 if (start + count > capacity)
                     atomicOr(&counters[1], 1ull);
@@ -1040,29 +1043,33 @@ __device__ inline void tileWalkOne(const TileWalkArgs& a, TileWalkLds& sh, uint3
             if (overflow)
                 atomicOr(&counters[1], 1ull);
         }
+        if (!emit)
+            base = uint64_t(readLane(uint32_t(start), 0)) | (uint64_t(readLane(uint32_t(start >> 32), 0)) << 32);
     }
 }
 
-__global__ __launch_bounds__(64) void tileWalkKernel(TileWalkArgs a, int emit) {
+// the count pass, then the emit pass, of one queued tile
+__device__ inline void tileWalkTile(const TileWalkArgs& a, TileWalkLds& sh, uint32_t tile, uint32_t lane) {
+    uint64_t base = 0;
+    tileWalkOne(a, sh, tile, 0, lane, base);
+    tileWalkOne(a, sh, tile, 1, lane, base);
+}
+
+__global__ __launch_bounds__(64) void tileWalkKernel(TileWalkArgs a) {
     __shared__ TileWalkLds sh;
     const uint32_t queued = uniform(uint32_t(a.counters[9]));
-    for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
-        waveLdsSync(); // the previous tile's LDS reads are done
-        tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
-    }
+    for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x)
+        tileWalkTile(a, sh, uniform(a.queue[qi]), threadIdx.x);
 }
 
-__global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __restrict__ all, uint32_t n,
-                                                          int emit) {
+__global__ __launch_bounds__(64) void tileWalkBatchKernel(const TileWalkArgs* __restrict__ all, uint32_t n) {
     __shared__ TileWalkLds sh;
     uint32_t base = 0;
     for (uint32_t v = 0; v < n; v++) {
         const TileWalkArgs a = batchArgs(all, v);
         const uint32_t queued = uniform(uint32_t(a.counters[9]));
-        for (uint32_t qi = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x; qi < queued; qi += gridDim.x) {
-            waveLdsSync();
-            tileWalkOne(a, sh, uniform(a.queue[qi]), emit, threadIdx.x);
-        }
+        for (uint32_t qi = (blockIdx.x + gridDim.x - base % gridDim.x) % gridDim.x; qi < queued; qi += gridDim.x)
+            tileWalkTile(a, sh, uniform(a.queue[qi]), threadIdx.x);
         base += queued;
     }
 }
@@ -1296,8 +1303,7 @@ void launchTileListFallback(hvvr_gpu_view* v, const hvvr_gpu_scene* s, hipStream
     TileWalkArgs a = tileWalkArgs(v, s);
     if (scratch)
         a.queue = scratch->queues + 4 * size_t(T);
-    for (int emit = 0; emit < 2; emit++)
-        tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a, emit); // 32 KB of LDS each
+    tileWalkKernel<<<clampGroups(shape.walkGroups, T), 64, 0, stream>>>(a); // 32 KB of LDS each
 }
 
 void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
@@ -1318,8 +1324,7 @@ void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32
 
 void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
                          hipStream_t stream) {
-    for (int emit = 0; emit < 2; emit++)
-        tileWalkBatchKernel<<<clampGroups(shape.walkGroups, maxTiles), 64, 0, stream>>>(all, viewCount, emit);
+    tileWalkBatchKernel<<<clampGroups(shape.walkGroups, maxTiles), 64, 0, stream>>>(all, viewCount);
 }
 
 } // namespace hvvr
