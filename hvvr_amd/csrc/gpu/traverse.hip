@@ -6,16 +6,16 @@
 //   frustumSetupKernel        one thread per tile or block frustum: camera->world
 //                             transform (traversal.h:38-47) + precompute (traverse_avx.cpp)
 //   blockTraverseWaveKernel   one wave64 per block: cooperative node tests, the <= 64-entry
-//                             cull stack in LDS, then the distance sort of
+//                             cull stack in registers, then the distance sort of
 //                             BlockFrame::sort as a parallel rank computation
 //   tileListKernel            one wave64 per tile, in three phases (see below); writes the
 //                             tile's list at an offset claimed with one atomic add, so no
 //                             count pass and no scan are needed
-//   tileListLargeKernel       the same for the tiles tileListKernel queued, with 2.7x the
-//                             records and 4x the stack
-//   tileTraverseKernel        one thread per tile with the full 4096-entry stack in global
-//                             memory; only runs the tiles tileListKernel could not hold in
-//                             LDS (count pass, allocation, emit pass)
+//   tileListQueueKernel<Cfg>  the same for the tiles a smaller configuration queued (large,
+//                             huge, giant, colossal: more records and deeper stacks)
+//   tileWalkKernel            one wave64 per tile with a 4096-entry stack in LDS, for the
+//                             tiles no configuration holds (count pass, allocation, emit
+//                             pass)
 #include <algorithm>
 
 #include <hip/hip_ext.h>
