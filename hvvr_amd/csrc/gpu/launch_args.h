@@ -64,8 +64,6 @@ struct TileListArgs {
     uint32_t* tileWork;
 };
 
-struct TileWalkArgs;
-
 struct TileWalkArgs {
     const BVHNode* nodes;
     const FrustumPre* tilePre;
@@ -138,7 +136,7 @@ struct RemapArgs {
 // one view's arguments (host)
 FrustumSetupArgs frustumSetupArgs(const hvvr_gpu_view* v, const matrix4x4& m);
 BlockTraverseArgs blockTraverseArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s);
-// level 0..3: the standard, large, huge and giant tile-list configurations
+// level 0..4: the standard, large, huge, giant and colossal tile-list configurations
 TileListArgs tileListArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, int level);
 TileWalkArgs tileWalkArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s);
 IntersectArgs intersectArgs(const hvvr_gpu_view* v, const hvvr_gpu_scene* s, const matrix4x4& m);
@@ -172,7 +170,7 @@ void launchFrustumSetupBatch(const FrustumSetupArgs* all, const uint32_t* start,
                              hipStream_t stream);
 void launchBlockTraverseBatch(const BlockTraverseArgs* all, const uint32_t* start, uint32_t viewCount,
                               uint32_t units, hipStream_t stream);
-// all[level * viewCount + view]: the four configurations' arguments
+// all[level * viewCount + view]: the five configurations' arguments
 void launchTileListsBatch(const TileListArgs* all, const uint32_t* start, uint32_t viewCount, uint32_t units,
                           const LaunchShape& shape, hipStream_t stream);
 void launchTileWalkBatch(const TileWalkArgs* all, uint32_t viewCount, uint32_t maxTiles, const LaunchShape& shape,
